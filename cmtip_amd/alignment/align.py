@@ -1,0 +1,274 @@
+"""Orientation matching on the GPU (cmtip/alignment/align.py).
+
+Signatures and defaults follow the reference; extra knobs are keyword-only.  The
+slices of every reference orientation are computed by the fused slicing kernel
+(coordinates generated in-kernel from the quaternions), the weighted distances and
+their argmin by the MFMA matching kernels; nothing is copied to the host until the
+matched quaternions are returned.
+
+Reference orientations: the reference draws them with skopi's get_uniform_quat
+(align.py:108), which is not available; pass ``ref_orientations`` to use a given
+set, otherwise a seeded uniform set (cmtip_amd.synth.random_quaternions) is drawn.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _arrays, _lib, synth
+from .._lib import check, ptr, stream
+from ..nufft.plan import DEFAULT_EPS, get_plan
+
+EMPTY_KEY = -1  # all ones bits: the largest unsigned 64-bit key
+
+
+def generate_weights(pixel_position_reciprocal, order=0):
+    """w = (1e-10 |s|)^-order / sum (align.py:11-24); shape of |s| over axis 0."""
+    as_torch = _arrays.is_torch(pixel_position_reciprocal)
+    s = _arrays.to_dev(pixel_position_reciprocal, torch.float32)
+    s_magnitudes = torch.linalg.vector_norm(s, dim=0) * 1e-10
+    weights = 1.0 / (s_magnitudes ** order)
+    weights = weights / torch.sum(weights)
+    return _arrays.out(weights, as_torch)
+
+
+def _norms(X, w2):
+    out = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+    check(_lib.lib().cmtip_weighted_norms(ptr(X), X.shape[0], X.shape[1], ptr(w2), ptr(out), stream()),
+          "cmtip_weighted_norms")
+    return out
+
+
+def calc_eudist(model_slices, slices):
+    """Euclidean distance matrix (n_model, n_slices), float32 (align.py:27-36; sklearn
+    euclidean_distances), with the cross term on MFMA."""
+    as_torch = _arrays.is_torch(model_slices, slices)
+    A = _arrays.to_dev(model_slices, torch.float32)
+    B = _arrays.to_dev(slices, torch.float32)
+    A = A.reshape(A.shape[0], -1)
+    B = B.reshape(B.shape[0], -1)
+    assert A.shape[1] == B.shape[1]
+    ones = torch.ones(A.shape[1], dtype=torch.float32, device=A.device)
+    an, bn = _norms(A, ones), _norms(B, ones)
+    D = torch.empty((A.shape[0], B.shape[0]), dtype=torch.float32, device=A.device)
+    check(_lib.lib().cmtip_eudist_f32(ptr(A), A.shape[0], ptr(B), B.shape[0], A.shape[1], ptr(an), ptr(bn), ptr(D),
+                                      stream()), "cmtip_eudist_f32")
+    return _arrays.out(D, as_torch)
+
+
+def _s64(pixel_position_reciprocal):
+    s = _arrays.to_dev(pixel_position_reciprocal, torch.float64)
+    return s.reshape(3, -1).contiguous()
+
+
+def _ext32(reciprocal_extent):
+    if isinstance(reciprocal_extent, torch.Tensor):
+        reciprocal_extent = reciprocal_extent.item()
+    return float(np.float32(reciprocal_extent))
+
+
+class Slicer:
+    """Slices of one autocorrelation at many orientations (compute_slices, align.py:39-65).
+
+    The fine grid is prepared once per autocorrelation; ``slice`` then writes
+    Re(T2(ac)) at the rotated pixels of a batch of quaternions.
+    """
+
+    def __init__(self, ac, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS):
+        self.ac = _arrays.to_dev(ac, torch.float32)
+        M = self.ac.shape[0]
+        self.plan = get_plan(2, M, -1, eps)
+        self.s = _s64(pixel_position_reciprocal)
+        self.P = self.s.shape[1]
+        self.ext = _ext32(reciprocal_extent)
+        self.plan.slice_prepare(self.ac)
+
+    def slice(self, quats, out=None, out_bf16=None, moments=None):
+        q = _arrays.to_dev(quats, torch.float32).reshape(-1, 4)
+        if out is None:
+            out = torch.empty((q.shape[0], self.P), dtype=torch.float32, device=q.device)
+        assert out.is_contiguous() and out.numel() == q.shape[0] * self.P
+        self.plan.slice_real(q, self.s, self.P, self.ext, out, out_bf16, moments)
+        return out
+
+
+def compute_slices(orientations, pixel_position_reciprocal, reciprocal_extent, ac, use_gpu=False, *,
+                   eps=DEFAULT_EPS):
+    """Flattened model slices (n_orient * P,) float32 (align.py:39-65)."""
+    as_torch = _arrays.is_torch(orientations, ac)
+    sl = Slicer(ac, pixel_position_reciprocal, reciprocal_extent, eps)
+    out = sl.slice(orientations)
+    return _arrays.out(out.reshape(-1), as_torch)
+
+
+def _std_from_moments(mom, count):
+    mean = mom[0] / count
+    return torch.sqrt(torch.clamp(mom[1] / count - mean * mean, min=0.0))
+
+
+def reference_orientations(n_ref_orientations, seed=0):
+    return synth.random_quaternions(n_ref_orientations, seed)
+
+
+class _Matcher:
+    """State shared by one alignment call: images, weights and per-image best keys."""
+
+    def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk):
+        self.P = pixel_position_reciprocal.shape[-1]
+        B = _arrays.to_dev(slices_, torch.float32).reshape(-1, self.P)
+        self.B = B
+        self.n_img = B.shape[0]
+        w = generate_weights(_arrays.to_dev(pixel_position_reciprocal, torch.float32), order=order)
+        self.w2 = (w.reshape(-1) * w.reshape(-1)).contiguous()
+        self.precision = precision
+        self.k = int(topk)
+        dev = B.device
+        self.Bw = torch.empty_like(B) if precision == "fp32" else None
+        self.Bw16 = torch.empty(B.shape, dtype=torch.int16, device=dev) if precision == "bf16" else None
+        check(_lib.lib().cmtip_match_prepare_images(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.Bw),
+                                                    ptr(self.Bw16), stream()), "cmtip_match_prepare_images")
+        self.bn = _norms(B, self.w2)
+        mom = torch.zeros(2, dtype=torch.float64, device=dev)
+        check(_lib.lib().cmtip_moments(ptr(B), B.numel(), ptr(mom), stream()), "cmtip_moments")
+        self.std_data = _std_from_moments(mom, B.numel())
+
+    def run(self, slicer, ref, nbatches, verbose=False):
+        """Per-image best key over all references (int64 holding the unsigned key)."""
+        n_ref = ref.shape[0]
+        quot = n_ref // nbatches
+        dev = self.B.device
+        L = _lib.lib()
+        best = torch.full((self.n_img,), EMPTY_KEY, dtype=torch.int64, device=dev)
+        if self.precision == "bf16":
+            cand = torch.full((self.n_img, self.k), EMPTY_KEY, dtype=torch.int64, device=dev)
+            A_all = torch.empty((n_ref, self.P), dtype=torch.float32, device=dev)
+            gamma_ref = torch.empty(n_ref, dtype=torch.float64, device=dev)
+            A16 = torch.empty((quot, self.P), dtype=torch.int16, device=dev)
+        else:
+            A = torch.empty((quot, self.P), dtype=torch.float32, device=dev)
+        for nb in range(nbatches):
+            start, end = nb * quot, (nb + 1) * quot
+            mom = torch.zeros(2, dtype=torch.float64, device=dev)
+            Ab = A_all[start:end] if self.precision == "bf16" else A
+            slicer.slice(ref[start:end], out=Ab, out_bf16=A16 if self.precision == "bf16" else None, moments=mom)
+            # data/model std ratio (align.py:122-124), rounded to float32 as numpy keeps it
+            gamma = (self.std_data / _std_from_moments(mom, Ab.numel())).float().double().reshape(1)
+            if verbose:
+                print(f"Data/Model std ratio: {float(gamma):.7g}.", flush=True)
+            an = _norms(Ab, self.w2)
+            if self.precision == "bf16":
+                gamma_ref[start:end] = gamma
+                check(L.cmtip_match_topk_bf16(ptr(A16), quot, start, ptr(self.Bw16), self.n_img, self.P, ptr(an),
+                                              ptr(self.bn), ptr(gamma), self.k, ptr(cand), stream()),
+                      "cmtip_match_topk_bf16")
+            else:
+                check(L.cmtip_match_argmin_f32(ptr(Ab), quot, start, ptr(self.Bw), self.n_img, self.P, ptr(an),
+                                               ptr(self.bn), ptr(gamma), ptr(best), stream()),
+                      "cmtip_match_argmin_f32")
+        if self.precision == "bf16":
+            check(L.cmtip_match_rescore(ptr(A_all), n_ref, ptr(self.B), self.n_img, self.P, ptr(self.w2),
+                                        ptr(gamma_ref), self.k, ptr(cand), ptr(best), stream()),
+                  "cmtip_match_rescore")
+        return best
+
+
+def _round_up_refs(n_ref_orientations, nbatches):
+    quot, remainder = divmod(n_ref_orientations, nbatches)
+    if remainder != 0:
+        n_ref_orientations = (quot + 1) * nbatches
+    return n_ref_orientations
+
+
+def _ref_set(n_ref_orientations, ref_orientations, true_orientations, seed):
+    if ref_orientations is None:
+        ref = reference_orientations(n_ref_orientations, seed)
+    else:
+        ref = np.asarray(ref_orientations.cpu() if isinstance(ref_orientations, torch.Tensor) else ref_orientations,
+                         dtype=np.float32)
+        assert ref.shape == (n_ref_orientations, 4), "ref_orientations must hold n_ref_orientations rows"
+    if true_orientations is not None:
+        t = np.asarray(true_orientations.cpu() if isinstance(true_orientations, torch.Tensor)
+                       else true_orientations, dtype=np.float32)
+        # align.py:109-111: the true quaternions replace the tail, then a global shuffle
+        ref = np.vstack((ref[:-len(t)], t))
+        np.random.shuffle(ref)
+    return np.ascontiguousarray(ref, dtype=np.float32)
+
+
+def key_index(best):
+    """Reference index of a packed best key (low 32 bits)."""
+    return best & 0xFFFFFFFF
+
+
+def key_d2(best):
+    """Squared distance of a packed best key (high 32 bits, float32 bits)."""
+    return (best >> 32).to(torch.int32).view(torch.float32)
+
+
+def match_orientations(generation, pixel_position_reciprocal, reciprocal_extent, slices_, ac, n_ref_orientations,
+                       nbatches=1, order=0, use_gpu=False, true_orientations=None, *, ref_orientations=None,
+                       precision="fp32", topk=8, eps=DEFAULT_EPS, seed=0, verbose=False, return_index=False):
+    """Quaternions (n_images, 4) float32 of the best-matching reference slices (align.py:68-132).
+
+    precision "fp32": cross term in FP32 on MFMA, argmin fused.  "bf16": cross term in
+    BF16 on MFMA with FP32 accumulation, the ``topk`` best per image rescored in FP32.
+    """
+    assert precision in ("fp32", "bf16")
+    n_ref_orientations = _round_up_refs(n_ref_orientations, nbatches)
+    ref = _ref_set(n_ref_orientations, ref_orientations, true_orientations, seed)
+    as_torch = _arrays.is_torch(slices_, ac)
+    slicer = Slicer(ac, pixel_position_reciprocal, reciprocal_extent, eps)
+    m = _Matcher(pixel_position_reciprocal, slices_, order, precision, topk)
+    ref_dev = torch.from_numpy(ref).to(m.B.device)
+    best = m.run(slicer, ref_dev, nbatches, verbose)
+    index = key_index(best)
+    q = ref_dev[index]
+    if return_index:
+        return _arrays.out(q, as_torch), _arrays.out(index, as_torch)
+    return _arrays.out(q, as_torch)
+
+
+def match_orientations_mult(generation, pixel_position_reciprocal, reciprocal_extent, slices_, ac_mult,
+                            n_ref_orientations, nbatches=1, order=0, use_gpu=False, true_orientations=None, *,
+                            ref_orientations=None, precision="fp32", topk=8, eps=DEFAULT_EPS, seed=0,
+                            verbose=False):
+    """(quaternions, conformations) over several autocorrelation volumes (align.py:135-213)."""
+    assert precision in ("fp32", "bf16")
+    n_ref_orientations = _round_up_refs(n_ref_orientations, nbatches)
+    ref = _ref_set(n_ref_orientations, ref_orientations, true_orientations, seed)
+    as_torch = _arrays.is_torch(slices_, ac_mult)
+    m = _Matcher(pixel_position_reciprocal, slices_, order, precision, topk)
+    ref_dev = torch.from_numpy(ref).to(m.B.device)
+    bests = []
+    for ac in ac_mult:
+        slicer = Slicer(ac, pixel_position_reciprocal, reciprocal_extent, eps)
+        bests.append(m.run(slicer, ref_dev, nbatches, verbose))
+    bests = torch.stack(bests)  # (n_conf, n_img)
+    d2 = key_d2(bests)
+    # argmin over conformations of the per-volume minimum distance (first on ties, :206)
+    conformations = torch.argmin(d2, dim=0)
+    idx = key_index(bests.gather(0, conformations[None, :])[0])
+    q = ref_dev[idx]
+    return _arrays.out(q, as_torch), _arrays.out(conformations, as_torch)
+
+
+def match_orientations_batch(generation, pixel_position_reciprocal, reciprocal_extent, slices_, ac,
+                             n_ref_orientations, nbatches=1, batch_size=400, order=0, true_orientations=None, *,
+                             ref_orientations=None, precision="fp32", topk=8, eps=DEFAULT_EPS, seed=0):
+    """Aligns images in batches of batch_size (align.py:216-266); returns (n_images, 4) float64
+    as the reference's np.zeros accumulator does (:243)."""
+    n = slices_.shape[0]
+    out = np.zeros((n, 4))
+    quot, remainder = divmod(n, batch_size)
+    if quot == 0:
+        quot = 1
+    for i in range(quot):
+        start, end = i * batch_size, (i + 1) * batch_size
+        if i == quot - 1:
+            end = n
+        q = match_orientations(generation, pixel_position_reciprocal, reciprocal_extent, slices_[start:end], ac,
+                               n_ref_orientations, nbatches=nbatches, order=order,
+                               true_orientations=true_orientations, ref_orientations=ref_orientations,
+                               precision=precision, topk=topk, eps=eps, seed=seed)
+        out[start:end] = q.cpu().numpy() if isinstance(q, torch.Tensor) else q
+    return out

@@ -1,0 +1,746 @@
+// Autocorrelation solve: the W x = d system of cmtip/autocorrelation/autocorrelation.py
+// and its scipy CG loop, on the device.
+//
+// core_problem_convolution (autocorrelation.py:32-63) is a circular convolution on
+// an L^3 grid (L even, L >= 2M).  Circular convolution commutes with shifts, so the
+// ifftshift of the zero-padded input and the fftshift + crop of the output
+// (:49-56) cancel: u is placed at positions [0, M) of every axis and the result is
+// read from the same positions.  The five passes are: FFT axis 2 (M^2 lines, M
+// nonzero inputs each), FFT axis 1 (M L lines), FFT axis 0 -> multiply by the
+// kernel spectrum -> inverse FFT axis 0 in one LDS-resident pass (L^2 lines),
+// inverse axis 1, inverse axis 2 with the crop, support, scale and the rho*u term
+// folded into the store.  fourier_reg (:88-110) is the same five-pass shape on the
+// M^3 grid with the (ifftshifted) antisupport mask as the multiplier.
+#include <math.h>
+
+#include <vector>
+
+#include "fft.h"
+
+using namespace cmtip;
+
+struct cmtip_acsolver_s {
+    int M = 0, L = 0;
+    Fft1D fftL, fftM;
+    DevBuf Fk;       // L^3 float, kernel spectrum in FFT position order
+    DevBuf mask;     // M^3 float, ifftshift(F_antisupport) in FFT position order
+    DevBuf tmp1;     // max(M^2 L, M^3) float2
+    DevBuf tmp2;     // M L^2 float2
+    DevBuf r, p, q;  // M^3 float CG vectors
+    DevBuf scal;     // FP64 CG scalars
+    const float* support = nullptr;
+};
+
+namespace {
+
+constexpr int kT = 256;
+enum { S_RHO = 0, S_RHO_PREV = 1, S_PQ = 2, S_RR_NEW = 3, S_X2 = 4, S_B2 = 5, S_COUNT = 8 };
+
+unsigned nblk(int64_t n, int64_t cap = 8192) {
+    int64_t b = (n + kT - 1) / kT;
+    if (b > cap) b = cap;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+// ---------------------------------------------------------------- gen_F_antisupport
+// mask = round(sqrt(H^2 + K^2 + L^2), 4) > pi on np.linspace(-pi, pi, M)^3
+// (autocorrelation.py:75-78), in array order.  linspace as numpy computes it:
+// i * step - pi, with the last sample set to pi exactly; np.around(x, 4) is
+// rint(x * 1e4) / 1e4.
+__global__ void antisupport_kernel(float* __restrict__ mask, int M, double step) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+    const int64_t M3 = (int64_t)M * M * M;
+    if (t >= M3) return;
+    const int i0 = (int)(t / ((int64_t)M * M)), i1 = (int)((t / M) % M), i2 = (int)(t % M);
+    auto lin = [&](int i) -> double { return (i == M - 1 && M > 1) ? M_PI : (double)i * step + (-M_PI); };
+    const double a = lin(i0), b = lin(i1), c = lin(i2);
+    const double q = sqrt(a * a + b * b + c * c);
+    const double qr = rint(q * 1e4) / 1e4;
+    mask[t] = qr > M_PI ? 1.0f : 0.0f;
+}
+
+// out[p0][p1][p2] = in[(p0 + h) % M][(p1 + h) % M][(p2 + h) % M], h = M // 2 (ifftshift)
+__global__ void ifftshift3_kernel(const float* __restrict__ in, float* __restrict__ out, int M) {
+    const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+    const int64_t M3 = (int64_t)M * M * M;
+    if (t >= M3) return;
+    const int p0 = (int)(t / ((int64_t)M * M)), p1 = (int)((t / M) % M), p2 = (int)(t % M);
+    const int h = M / 2;
+    const int i0 = (p0 + h) % M, i1 = (p1 + h) % M, i2 = (p2 + h) % M;
+    out[t] = in[((int64_t)i0 * M + i1) * M + i2];
+}
+
+// ---------------------------------------------------------------- support filter
+// scipy.ndimage.gaussian_filter(ac, 0.5): radius int(4 * 0.5 + 0.5) = 2, weights
+// exp(-x^2 / (2 sigma^2)) normalised, mode 'reflect' (edge sample repeated), one axis
+// at a time; symmetric weights are applied as w0 x[i] + sum_j wj (x[i-j] + x[i+j]),
+// the order of scipy's correlate1d.  round_f32 rounds after each axis (float32 input).
+__device__ __forceinline__ int reflect(int i, int M) {
+    // half-sample symmetric reflection, valid for |overhang| <= M
+    if (i < 0) i = -i - 1;
+    if (i >= M) i = 2 * M - i - 1;
+    return i < 0 ? 0 : (i >= M ? M - 1 : i);
+}
+
+__global__ void gauss_axis_kernel(const double* __restrict__ in, double* __restrict__ out, int M, int axis,
+                                  double w0, double w1, double w2, int round_f32) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+    const int64_t M3 = (int64_t)M * M * M;
+    if (t >= M3) return;
+    int idx[3] = {(int)(t / ((int64_t)M * M)), (int)((t / M) % M), (int)(t % M)};
+    const int64_t stride = axis == 0 ? (int64_t)M * M : (axis == 1 ? M : 1);
+    const int i = idx[axis];
+    const int64_t base = t - (int64_t)i * stride;
+    auto at = [&](int j) -> double { return in[base + (int64_t)reflect(j, M) * stride]; };
+    double v = at(i) * w0;
+    v += (at(i - 1) + at(i + 1)) * w1;
+    v += (at(i - 2) + at(i + 2)) * w2;
+    if (round_f32) v = (double)(float)v;
+    out[t] = v;
+}
+
+__global__ void threshold_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n, double thr) {
+    const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (t < n) out[t] = in[t] > thr ? 1.0f : 0.0f;
+}
+
+// ---------------------------------------------------------------- CG vector kernels
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__global__ void dot_kernel(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
+                           double* __restrict__ out) {
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) s += (double)a[i] * (double)b[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, s);
+}
+
+__global__ void set_scalars_kernel(double* __restrict__ s, int first, int count, double v) {
+    if (threadIdx.x < count) s[first + threadIdx.x] = v;
+}
+
+// r = b - Ax (or r = b when Ax is null)
+__global__ void residual_kernel(const float* __restrict__ b, const float* __restrict__ ax, float* __restrict__ r,
+                                int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride)
+        r[i] = ax ? b[i] - ax[i] : b[i];
+}
+
+// p = r (first iteration) or p = beta p + r with beta = rho / rho_prev
+__global__ void update_p_kernel(const float* __restrict__ r, float* __restrict__ p, const double* __restrict__ s,
+                                int first, int64_t n) {
+    const float beta = first ? 0.0f : (float)(s[S_RHO] / s[S_RHO_PREV]);
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride)
+        p[i] = first ? r[i] : beta * p[i] + r[i];
+}
+
+// alpha = rho / (p.q); x += alpha p; r -= alpha q; accumulate r.r into s[S_RR_NEW]
+__global__ void update_xr_kernel(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+                                 const float* __restrict__ q, double* __restrict__ s, int64_t n) {
+    const float alpha = (float)(s[S_RHO] / s[S_PQ]);
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) {
+        x[i] = x[i] + alpha * p[i];
+        const float rn = r[i] - alpha * q[i];
+        r[i] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(s + S_RR_NEW, acc);
+}
+
+// rho_prev = rho; rho = rr_new
+__global__ void rotate_scalars_kernel(double* __restrict__ s) {
+    if (threadIdx.x == 0) {
+        s[S_RHO_PREV] = s[S_RHO];
+        s[S_RHO] = s[S_RR_NEW];
+    }
+}
+
+bool valid(cmtip_acsolver s) {
+    if (!s) {
+        set_error("null solver");
+        return false;
+    }
+    return true;
+}
+
+// out = out_scale * S (.) conv(S (.) u) + add_scale * add_src   (add_src may be null)
+int conv_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, const float* add_src,
+               float add_scale, hipStream_t st) {
+    const int M = s->M, L = s->L;
+    const int64_t LL = (int64_t)L * L;
+    float2* t1 = s->tmp1.as<float2>();
+    float2* t2 = s->tmp2.as<float2>();
+    AxisDesc d;
+    // 1: axis 2, lines (i0, i1), input positions [0, M)
+    d = AxisDesc();
+    d.in = u;
+    d.in_real = 1;
+    d.in_su = (int64_t)M * M;
+    d.in_sv = M;
+    d.in_sa = 1;
+    d.ia = M;
+    d.ig1 = 0;
+    d.in_mask = s->support;
+    d.out = t1;
+    d.out_su = (int64_t)M * L;
+    d.out_sv = L;
+    d.out_sa = 1;
+    d.oa = L;
+    d.nu = M;
+    d.nv = M;
+    d.sign = -1;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 2: axis 1, lines (i0, p2)
+    d = AxisDesc();
+    d.in = t1;
+    d.in_su = (int64_t)M * L;
+    d.in_sv = 1;
+    d.in_sa = L;
+    d.ia = M;
+    d.ig1 = 0;
+    d.out = t2;
+    d.out_su = LL;
+    d.out_sv = 1;
+    d.out_sa = L;
+    d.oa = L;
+    d.nu = M;
+    d.nv = L;
+    d.sign = -1;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 3: axis 0 forward, x kernel spectrum, inverse; lines (p1, p2); in place
+    d = AxisDesc();
+    d.in = t2;
+    d.in_su = L;
+    d.in_sv = 1;
+    d.in_sa = LL;
+    d.ia = M;
+    d.ig1 = 0;
+    d.out = t2;
+    d.out_su = L;
+    d.out_sv = 1;
+    d.out_sa = LL;
+    d.oa = M;
+    d.og1 = 0;
+    d.nu = L;
+    d.nv = L;
+    d.sign = -1;
+    d.mid_r = s->Fk.as<const float>();
+    d.mid_su = L;
+    d.mid_sv = 1;
+    d.mid_sa = LL;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 4: inverse axis 1, lines (i0, p2), keep positions [0, M)
+    d = AxisDesc();
+    d.in = t2;
+    d.in_su = LL;
+    d.in_sv = 1;
+    d.in_sa = L;
+    d.ia = L;
+    d.out = t1;
+    d.out_su = (int64_t)M * L;
+    d.out_sv = 1;
+    d.out_sa = L;
+    d.oa = M;
+    d.nu = M;
+    d.nv = L;
+    d.sign = +1;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 5: inverse axis 2, lines (i0, i1), real part, crop, support, scale, axpy
+    d = AxisDesc();
+    d.in = t1;
+    d.in_su = (int64_t)M * L;
+    d.in_sv = L;
+    d.in_sa = 1;
+    d.ia = L;
+    d.out = out;
+    d.out_mode = 1;
+    d.out_su = (int64_t)M * M;
+    d.out_sv = M;
+    d.out_sa = 1;
+    d.oa = M;
+    d.out_scale = out_scale;
+    d.out_mask = s->support;
+    d.add_src = add_src;
+    d.add_scale = add_scale;
+    d.nu = M;
+    d.nv = M;
+    d.sign = +1;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    return CMTIP_OK;
+}
+
+// out = out_scale * S (.) fftshift(ifftn(fftn(ifftshift(S (.) u)) * mask)) + add_scale * add_src.
+// By the shift theorem the ifftshift / fftshift pair cancels once the mask is held in
+// FFT position order (np.fft.ifftshift(F_antisupport), autocorrelation.py:105), so u
+// is transformed in place and the result read back at the same positions.
+int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, const float* add_src,
+              float add_scale, hipStream_t st) {
+    const int M = s->M;
+    const int64_t MM = (int64_t)M * M;
+    float2* t1 = s->tmp1.as<float2>();
+    float2* t2 = s->tmp2.as<float2>();
+    AxisDesc d;
+    // 1: axis 2
+    d = AxisDesc();
+    d.in = u;
+    d.in_real = 1;
+    d.in_su = MM;
+    d.in_sv = M;
+    d.in_sa = 1;
+    d.ia = M;
+    d.in_mask = s->support;
+    d.out = t1;
+    d.out_su = MM;
+    d.out_sv = M;
+    d.out_sa = 1;
+    d.oa = M;
+    d.nu = M;
+    d.nv = M;
+    d.sign = -1;
+    CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
+    // 2: axis 1
+    d = AxisDesc();
+    d.in = t1;
+    d.in_su = MM;
+    d.in_sv = 1;
+    d.in_sa = M;
+    d.ia = M;
+    d.out = t2;
+    d.out_su = MM;
+    d.out_sv = 1;
+    d.out_sa = M;
+    d.oa = M;
+    d.nu = M;
+    d.nv = M;
+    d.sign = -1;
+    CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
+    // 3: axis 0 forward, x mask, inverse (in place)
+    d = AxisDesc();
+    d.in = t2;
+    d.in_su = M;
+    d.in_sv = 1;
+    d.in_sa = MM;
+    d.ia = M;
+    d.out = t2;
+    d.out_su = M;
+    d.out_sv = 1;
+    d.out_sa = MM;
+    d.oa = M;
+    d.nu = M;
+    d.nv = M;
+    d.sign = -1;
+    d.mid_r = s->mask.as<const float>();
+    d.mid_su = M;
+    d.mid_sv = 1;
+    d.mid_sa = MM;
+    CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
+    // 4: inverse axis 1
+    d = AxisDesc();
+    d.in = t2;
+    d.in_su = MM;
+    d.in_sv = 1;
+    d.in_sa = M;
+    d.ia = M;
+    d.out = t1;
+    d.out_su = MM;
+    d.out_sv = 1;
+    d.out_sa = M;
+    d.oa = M;
+    d.nu = M;
+    d.nv = M;
+    d.sign = +1;
+    CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
+    // 5: inverse axis 2, real part, support, scale, axpy
+    d = AxisDesc();
+    d.in = t1;
+    d.in_su = MM;
+    d.in_sv = M;
+    d.in_sa = 1;
+    d.ia = M;
+    d.out = out;
+    d.out_mode = 1;
+    d.out_su = MM;
+    d.out_sv = M;
+    d.out_sa = 1;
+    d.oa = M;
+    d.out_scale = out_scale;
+    d.out_mask = s->support;
+    d.add_src = add_src;
+    d.add_scale = add_scale;
+    d.nu = M;
+    d.nv = M;
+    d.sign = +1;
+    CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
+    return CMTIP_OK;
+}
+
+int matvec(cmtip_acsolver_s* s, const float* u, float* out, double alpha, double rho, double phi,
+           hipStream_t st) {
+    const double L3 = (double)s->L * s->L * s->L;
+    const double M3 = (double)s->M * s->M * s->M;
+    CMTIP_TRY(conv_apply(s, u, out, (float)(alpha / L3), u, (float)rho, st));
+    CMTIP_TRY(reg_apply(s, u, out, (float)(phi / M3), out, 1.0f, st));
+    return CMTIP_OK;
+}
+
+int dot(const float* a, const float* b, int64_t n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(dot_kernel, dim3(nblk(n, 1024)), dim3(kT), 0, st, a, b, n, out);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+int set_scalars(double* s, int first, int count, double v, hipStream_t st) {
+    hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, st, s, first, count, v);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+}  // namespace
+
+extern "C" int cmtip_acsolver_create(int M, int L, cmtip_acsolver* out) {
+    if (!out) {
+        set_error("acsolver_create: null output");
+        return CMTIP_ERR_ARG;
+    }
+    *out = nullptr;
+    if (M < 1 || M > 1024 || L < 2 * M || (L % 2) != 0 || L > 4096) {
+        set_error("acsolver_create: need M >= 1 and an even L >= 2M (M=%d, L=%d)", M, L);
+        return CMTIP_ERR_ARG;
+    }
+    auto* s = new cmtip_acsolver_s();
+    s->M = M;
+    s->L = L;
+    const int64_t M3 = (int64_t)M * M * M;
+    int st = fft1d_init(s->fftL, L);
+    if (st == CMTIP_OK) st = fft1d_init(s->fftM, M);
+    if (st == CMTIP_OK) st = s->Fk.ensure(sizeof(float) * (size_t)L * L * L);
+    if (st == CMTIP_OK) st = s->mask.ensure(sizeof(float) * (size_t)M3);
+    const int64_t t1 = (int64_t)M * M * L > M3 ? (int64_t)M * M * L : M3;
+    if (st == CMTIP_OK) st = s->tmp1.ensure(sizeof(float2) * (size_t)t1);
+    if (st == CMTIP_OK) st = s->tmp2.ensure(sizeof(float2) * (size_t)M * L * L);
+    if (st == CMTIP_OK) st = s->r.ensure(sizeof(float) * (size_t)M3);
+    if (st == CMTIP_OK) st = s->p.ensure(sizeof(float) * (size_t)M3);
+    if (st == CMTIP_OK) st = s->q.ensure(sizeof(float) * (size_t)M3);
+    if (st == CMTIP_OK) st = s->scal.ensure(sizeof(double) * S_COUNT);
+    if (st == CMTIP_OK) {
+        DevBuf arr;
+        st = arr.ensure(sizeof(float) * (size_t)M3);
+        if (st == CMTIP_OK) st = cmtip_gen_F_antisupport(M, arr.as<float>(), nullptr);
+        if (st == CMTIP_OK) st = cmtip_acsolver_set_antisupport(s, arr.as<const float>(), nullptr);
+        if (st == CMTIP_OK && hipDeviceSynchronize() != hipSuccess) {
+            set_error("acsolver_create: antisupport kernels failed");
+            st = CMTIP_ERR_LAUNCH;
+        }
+        arr.release();
+    }
+    if (st != CMTIP_OK) {
+        cmtip_acsolver_destroy(s);
+        return st;
+    }
+    *out = s;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_gen_F_antisupport(int M, float* out, void* stream) {
+    if (M < 1 || !out) {
+        set_error("gen_F_antisupport: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    const int64_t M3 = (int64_t)M * M * M;
+    const double step = M > 1 ? (M_PI - (-M_PI)) / (double)(M - 1) : 0.0;
+    hipLaunchKernelGGL(antisupport_kernel, dim3((unsigned)((M3 + kT - 1) / kT)), dim3(kT), 0, as_stream(stream),
+                       out, M, step);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_acsolver_set_antisupport(cmtip_acsolver s, const float* F_antisupport, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!F_antisupport) {
+        set_error("acsolver_set_antisupport: null mask");
+        return CMTIP_ERR_ARG;
+    }
+    const int M = s->M;
+    const int64_t M3 = (int64_t)M * M * M;
+    hipLaunchKernelGGL(ifftshift3_kernel, dim3((unsigned)((M3 + kT - 1) / kT)), dim3(kT), 0, as_stream(stream),
+                       F_antisupport, s->mask.as<float>(), M);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_acsolver_set_spectrum(cmtip_acsolver s, const float* F_real, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!F_real) {
+        set_error("acsolver_set_spectrum: null spectrum");
+        return CMTIP_ERR_ARG;
+    }
+    const size_t bytes = sizeof(float) * (size_t)s->L * s->L * s->L;
+    CMTIP_HIP(hipMemcpyAsync(s->Fk.p, F_real, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_acsolver_destroy(cmtip_acsolver s) {
+    if (!s) return CMTIP_OK;
+    fft1d_free(s->fftL);
+    fft1d_free(s->fftM);
+    s->Fk.release();
+    s->mask.release();
+    s->tmp1.release();
+    s->tmp2.release();
+    s->r.release();
+    s->p.release();
+    s->q.release();
+    s->scal.release();
+    delete s;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_ac_support(const double* ac_estimate, int M, float* support, void* stream) {
+    return cmtip_ac_support_ex(ac_estimate, M, 0, support, stream);
+}
+
+extern "C" int cmtip_ac_support_ex(const double* ac_estimate, int M, int round_f32, float* support,
+                                   void* stream) {
+    if (!ac_estimate || !support || M < 1) {
+        set_error("ac_support: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(stream);
+    const int64_t M3 = (int64_t)M * M * M;
+    // weights in numpy's order: x = arange(-2, 3); phi = exp(-0.5 / 0.25 * x**2); phi / phi.sum()
+    double phi[5];
+    for (int i = 0; i < 5; ++i) {
+        const double x = (double)(i - 2);
+        phi[i] = exp(-0.5 / 0.25 * (x * x));
+    }
+    double sum = 0.0;
+    for (int i = 0; i < 5; ++i) sum += phi[i];
+    const double w0 = phi[2] / sum, w1 = phi[3] / sum, w2 = phi[4] / sum;
+    DevBuf a, b;
+    CMTIP_TRY(a.ensure(sizeof(double) * (size_t)M3));
+    CMTIP_TRY(b.ensure(sizeof(double) * (size_t)M3));
+    const unsigned g = (unsigned)((M3 + kT - 1) / kT);
+    hipLaunchKernelGGL(gauss_axis_kernel, dim3(g), dim3(kT), 0, st, ac_estimate, a.as<double>(), M, 0, w0, w1, w2,
+                       round_f32);
+    hipLaunchKernelGGL(gauss_axis_kernel, dim3(g), dim3(kT), 0, st, a.as<const double>(), b.as<double>(), M, 1, w0,
+                       w1, w2, round_f32);
+    hipLaunchKernelGGL(gauss_axis_kernel, dim3(g), dim3(kT), 0, st, b.as<const double>(), a.as<double>(), M, 2, w0,
+                       w1, w2, round_f32);
+    hipLaunchKernelGGL(threshold_kernel, dim3(g), dim3(kT), 0, st, a.as<const double>(), support, M3, 1e-12);
+    hipError_t e = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(st);  // scratch buffers are freed below
+    a.release();
+    b.release();
+    if (e != hipSuccess || e2 != hipSuccess) {
+        set_error("ac_support: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        return CMTIP_ERR_LAUNCH;
+    }
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_acsolver_set_support(cmtip_acsolver s, const float* support) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    s->support = support;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_acsolver_set_kernel(cmtip_acsolver s, const float* ugrid_conv, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!ugrid_conv) {
+        set_error("acsolver_set_kernel: null kernel");
+        return CMTIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(stream);
+    const int L = s->L;
+    const int64_t LL = (int64_t)L * L;
+    DevBuf big1, big2;
+    CMTIP_TRY(big1.ensure(sizeof(float2) * (size_t)LL * L));
+    CMTIP_TRY(big2.ensure(sizeof(float2) * (size_t)LL * L));
+    const int h = L / 2;
+    AxisDesc d;
+    // axis 2 with ifftshift
+    d = AxisDesc();
+    d.in = ugrid_conv;
+    d.in_real = 1;
+    d.in_su = LL;
+    d.in_sv = L;
+    d.in_sa = 1;
+    d.ia = L - h;
+    d.ig1 = h;
+    d.ib = h;
+    d.ig2 = 0;
+    d.out = big1.p;
+    d.out_su = LL;
+    d.out_sv = L;
+    d.out_sa = 1;
+    d.oa = L;
+    d.nu = L;
+    d.nv = L;
+    d.sign = -1;
+    int rc = fft_axis_pass(s->fftL, d, st);
+    // axis 1 with ifftshift
+    if (rc == CMTIP_OK) {
+        d = AxisDesc();
+        d.in = big1.p;
+        d.in_su = LL;
+        d.in_sv = 1;
+        d.in_sa = L;
+        d.ia = L - h;
+        d.ig1 = h;
+        d.ib = h;
+        d.ig2 = 0;
+        d.out = big2.p;
+        d.out_su = LL;
+        d.out_sv = 1;
+        d.out_sa = L;
+        d.oa = L;
+        d.nu = L;
+        d.nv = L;
+        d.sign = -1;
+        rc = fft_axis_pass(s->fftL, d, st);
+    }
+    // axis 0 with ifftshift, real part out
+    if (rc == CMTIP_OK) {
+        d = AxisDesc();
+        d.in = big2.p;
+        d.in_su = L;
+        d.in_sv = 1;
+        d.in_sa = LL;
+        d.ia = L - h;
+        d.ig1 = h;
+        d.ib = h;
+        d.ig2 = 0;
+        d.out = s->Fk.p;
+        d.out_mode = 1;
+        d.out_su = L;
+        d.out_sv = 1;
+        d.out_sa = LL;
+        d.oa = L;
+        d.nu = L;
+        d.nv = L;
+        d.sign = -1;
+        rc = fft_axis_pass(s->fftL, d, st);
+    }
+    hipError_t e = hipStreamSynchronize(st);  // scratch buffers are freed below
+    big1.release();
+    big2.release();
+    if (rc != CMTIP_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("acsolver_set_kernel: %s", hipGetErrorString(e));
+        return CMTIP_ERR_HIP;
+    }
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_core_problem_convolution(cmtip_acsolver s, const float* u, float* out, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!u || !out || u == out) {
+        set_error("core_problem_convolution: need distinct u and out");
+        return CMTIP_ERR_ARG;
+    }
+    const double L3 = (double)s->L * s->L * s->L;
+    return conv_apply(s, u, out, (float)(1.0 / L3), nullptr, 0.0f, as_stream(stream));
+}
+
+extern "C" int cmtip_fourier_reg(cmtip_acsolver s, const float* u, float* out, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!u || !out || u == out) {
+        set_error("fourier_reg: need distinct u and out");
+        return CMTIP_ERR_ARG;
+    }
+    const double M3 = (double)s->M * s->M * s->M;
+    return reg_apply(s, u, out, (float)(1.0 / M3), nullptr, 0.0f, as_stream(stream));
+}
+
+extern "C" int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* out, double alpha, double rho,
+                                     double phi, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!u || !out || u == out) {
+        set_error("acsolver_matvec: need distinct u and out");
+        return CMTIP_ERR_ARG;
+    }
+    return matvec(s, u, out, alpha, rho, phi, as_stream(stream));
+}
+
+static int read_scalar(const double* dev, double* host, hipStream_t st) {
+    CMTIP_HIP(hipMemcpyAsync(host, dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    CMTIP_HIP(hipStreamSynchronize(st));
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
+                                 double phi, int maxiter, double rtol, int* iters, int* info, void* stream) {
+    if (!valid(s)) return CMTIP_ERR_ARG;
+    if (!d || !x || maxiter < 0) {
+        set_error("acsolver_cg: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(stream);
+    const int64_t n = (int64_t)s->M * s->M * s->M;
+    double* sc = s->scal.as<double>();
+    float* r = s->r.as<float>();
+    float* p = s->p.as<float>();
+    float* q = s->q.as<float>();
+    const unsigned g = nblk(n, 2048);
+    int count = 0, inf = maxiter;
+
+    // bnrm2 = ||b||; x.any()
+    CMTIP_TRY(set_scalars(sc, 0, S_COUNT, 0.0, st));
+    CMTIP_TRY(dot(d, d, n, sc + S_B2, st));
+    CMTIP_TRY(dot(x, x, n, sc + S_X2, st));
+    double b2 = 0.0, x2 = 0.0;
+    CMTIP_TRY(read_scalar(sc + S_B2, &b2, st));
+    CMTIP_TRY(read_scalar(sc + S_X2, &x2, st));
+    const double bnrm = sqrt(b2);
+    const double atol = rtol * bnrm;
+    if (bnrm == 0.0) {
+        // scipy returns b itself
+        CMTIP_HIP(hipMemcpyAsync(x, d, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        if (iters) *iters = 0;
+        if (info) *info = 0;
+        return CMTIP_OK;
+    }
+    if (x2 > 0.0) {
+        CMTIP_TRY(matvec(s, x, q, alpha, rho, phi, st));
+        hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(kT), 0, st, d, (const float*)q, r, n);
+    } else {
+        hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(kT), 0, st, d, (const float*)nullptr, r, n);
+    }
+    CMTIP_LAUNCH_CHECK();
+    CMTIP_TRY(dot(r, r, n, sc + S_RHO, st));
+    double rr = 0.0;
+    CMTIP_TRY(read_scalar(sc + S_RHO, &rr, st));
+    for (int it = 0; it < maxiter; ++it) {
+        if (sqrt(rr) < atol) {
+            inf = 0;
+            break;
+        }
+        hipLaunchKernelGGL(update_p_kernel, dim3(g), dim3(kT), 0, st, (const float*)r, p, (const double*)sc,
+                           it == 0 ? 1 : 0, n);
+        CMTIP_LAUNCH_CHECK();
+        CMTIP_TRY(matvec(s, p, q, alpha, rho, phi, st));
+        CMTIP_TRY(set_scalars(sc, S_PQ, 2, 0.0, st));
+        CMTIP_TRY(dot(p, q, n, sc + S_PQ, st));
+        hipLaunchKernelGGL(update_xr_kernel, dim3(g), dim3(kT), 0, st, x, r, (const float*)p, (const float*)q, sc,
+                           n);
+        hipLaunchKernelGGL(rotate_scalars_kernel, dim3(1), dim3(64), 0, st, sc);
+        CMTIP_LAUNCH_CHECK();
+        ++count;
+        CMTIP_TRY(read_scalar(sc + S_RHO, &rr, st));
+    }
+    if (iters) *iters = count;
+    if (info) *info = inf;
+    return CMTIP_OK;
+}

@@ -1,0 +1,384 @@
+// Mixed-radix Stockham FFT axis passes (see fft.h for the contract).
+//
+// Stage formulation (Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC 2008): with Ns the product of the
+// radices already applied, butterfly j of a radix-R stage reads x[j + r*n/R],
+// twiddles by exp(sign*2*pi*i*r*(j mod Ns)/(Ns*R)), does an R-point DFT and
+// writes to (j div Ns)*Ns*R + (j mod Ns) + r*Ns.  Radices 4, 2, 3 and 5 have
+// hand-written butterflies; any other prime factor (e.g. 41 or 151, the
+// autocorrelation edge lengths whose Fourier-corner projection needs an exact
+// M-point DFT) runs a direct O(R^2) DFT stage.
+#include <math.h>
+
+#include <vector>
+
+#include "fft.h"
+
+namespace cmtip {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsBudget = 64 * 1024;
+
+struct PassArgs {
+    AxisDesc d;
+    int n, nst, lb, lbshift, ldl, ax_contig;
+    uint32_t nlines;
+    FastDiv fn, fnv;
+    int R[kFftMaxStages], Ns[kFftMaxStages], twoff[kFftMaxStages], omoff[kFftMaxStages];
+    FastDiv fnb[kFftMaxStages], fNs[kFftMaxStages];
+    const float2* tw;
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// sign * i * a
+__device__ __forceinline__ float2 cmuli(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
+
+template <int R>
+__device__ __forceinline__ void butterfly(float2* v, float sg) {
+    if constexpr (R == 2) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const float c = -0.5f, s = 0.86602540378443864676f;
+        float2 a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
+        float2 m = make_float2(v[0].x + c * a.x, v[0].y + c * a.y);
+        float2 t = cmuli(cscale(b, s), sg);
+        v[0] = cadd(v[0], a);
+        v[1] = cadd(m, t);
+        v[2] = csub(m, t);
+    } else if constexpr (R == 4) {
+        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), sg);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else if constexpr (R == 5) {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        float2 m2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        float2 t1 = cmuli(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), sg);
+        float2 t2 = cmuli(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), sg);
+        v[0] = make_float2(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+        v[1] = cadd(m1, t1);
+        v[4] = csub(m1, t1);
+        v[2] = cadd(m2, t2);
+        v[3] = csub(m2, t2);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void radix_stage(const PassArgs& a, int s, const float2* __restrict__ src,
+                                            float2* __restrict__ dst, float sg) {
+    const uint32_t nb = (uint32_t)(a.n / R);
+    const uint32_t Ns = (uint32_t)a.Ns[s];
+    const uint32_t total = (uint32_t)a.lb * nb;
+    const float2* tw = a.tw + a.twoff[s];
+    for (uint32_t b = threadIdx.x; b < total; b += kThreads) {
+        const uint32_t line = fdiv(b, a.fnb[s]);
+        const uint32_t j = b - line * nb;
+        const uint32_t jm = j - fdiv(j, a.fNs[s]) * Ns;
+        const float2* sl = src + line * a.ldl;
+        float2* dl = dst + line * a.ldl;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = sl[j + r * nb];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[jm * (R - 1) + (r - 1)];
+                w.y *= -sg;  // table holds exp(-i...); sg = -1 keeps it, +1 conjugates
+                v[r] = cmul(v[r], w);
+            }
+        }
+        butterfly<R>(v, sg);
+        const uint32_t o = (j - jm) * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dl[o + r * Ns] = v[r];
+    }
+}
+
+__device__ void generic_stage(const PassArgs& a, int s, const float2* __restrict__ src,
+                              float2* __restrict__ dst, float sg) {
+    const int R = a.R[s];
+    const uint32_t nb = (uint32_t)(a.n / R);
+    const uint32_t Ns = (uint32_t)a.Ns[s];
+    const uint32_t n = (uint32_t)a.n;
+    const uint32_t total = (uint32_t)a.lb * n;
+    const float2* tw = a.tw + a.twoff[s];
+    const float2* om = a.tw + a.omoff[s];
+    for (uint32_t b = threadIdx.x; b < total; b += kThreads) {
+        const uint32_t line = fdiv(b, a.fn);
+        const uint32_t rem = b - line * n;
+        const uint32_t r = fdiv(rem, a.fnb[s]);
+        const uint32_t j = rem - r * nb;
+        const uint32_t jm = j - fdiv(j, a.fNs[s]) * Ns;
+        const float2* sl = src + line * a.ldl;
+        float2 acc = make_float2(0.f, 0.f);
+        int e = 0;
+        for (int q = 0; q < R; ++q) {
+            float2 x = sl[j + q * nb];
+            if (q > 0 && Ns > 1) {
+                float2 w = tw[jm * (R - 1) + (q - 1)];
+                w.y *= -sg;
+                x = cmul(x, w);
+            }
+            float2 w = om[e];
+            w.y *= -sg;
+            acc = cadd(acc, cmul(x, w));
+            e += (int)r;
+            if (e >= R) e -= R;
+        }
+        dst[line * a.ldl + (j - jm) * R + jm + r * Ns] = acc;
+    }
+}
+
+// Runs all stages; returns the buffer holding the result.
+__device__ float2* run_stages(const PassArgs& a, float2* b0, float2* b1, float sg) {
+    float2* src = b0;
+    float2* dst = b1;
+    for (int s = 0; s < a.nst; ++s) {
+        switch (a.R[s]) {
+            case 2: radix_stage<2>(a, s, src, dst, sg); break;
+            case 3: radix_stage<3>(a, s, src, dst, sg); break;
+            case 4: radix_stage<4>(a, s, src, dst, sg); break;
+            case 5: radix_stage<5>(a, s, src, dst, sg); break;
+            default: generic_stage(a, s, src, dst, sg); break;
+        }
+        __syncthreads();
+        float2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+__device__ __forceinline__ void split_elem(const PassArgs& a, uint32_t e, uint32_t& line, uint32_t& p) {
+    if (a.ax_contig) {
+        line = fdiv(e, a.fn);
+        p = e - line * (uint32_t)a.n;
+    } else {
+        line = e & (uint32_t)(a.lb - 1);
+        p = e >> a.lbshift;
+    }
+}
+
+template <bool IN_REAL, int OUT_MODE, bool MID>
+__global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
+    extern __shared__ float2 lds[];
+    float2* b0 = lds;
+    float2* b1 = lds + a.lb * a.ldl;
+    const AxisDesc& d = a.d;
+    const uint32_t n = (uint32_t)a.n;
+    const uint32_t line0 = blockIdx.x * (uint32_t)a.lb;
+    const uint32_t tot = (uint32_t)a.lb * n;
+
+    // ---- load (zero padding, input weights and masks folded in)
+    for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
+        uint32_t line, p;
+        split_elem(a, e, line, p);
+        const uint32_t gl = line0 + line;
+        float2 val = make_float2(0.f, 0.f);
+        if (gl < a.nlines) {
+            int g = -1;
+            if ((int)p < d.ia)
+                g = d.ig1 + (int)p;
+            else if ((int)p >= (int)n - d.ib)
+                g = d.ig2 + ((int)p - ((int)n - d.ib));
+            if (g >= 0) {
+                const uint32_t u = fdiv(gl, a.fnv);
+                const uint32_t v = gl - u * (uint32_t)d.nv;
+                const int64_t off = (int64_t)u * d.in_su + (int64_t)v * d.in_sv + (int64_t)g * d.in_sa;
+                if (IN_REAL)
+                    val = make_float2(reinterpret_cast<const float*>(d.in)[off], 0.f);
+                else
+                    val = reinterpret_cast<const float2*>(d.in)[off];
+                float wgt = 1.f;
+                if (d.in_w) wgt *= d.in_w[p];
+                if (d.in_mask) wgt *= d.in_mask[off];
+                val = cscale(val, wgt);
+            }
+        }
+        b0[line * a.ldl + p] = val;
+    }
+    __syncthreads();
+
+    const float sg = (float)d.sign;
+    float2* res = run_stages(a, b0, b1, sg);
+
+    if (MID) {
+        for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
+            uint32_t line, p;
+            split_elem(a, e, line, p);
+            const uint32_t gl = line0 + line;
+            if (gl >= a.nlines) continue;
+            const uint32_t u = fdiv(gl, a.fnv);
+            const uint32_t v = gl - u * (uint32_t)d.nv;
+            const int64_t off = (int64_t)u * d.mid_su + (int64_t)v * d.mid_sv + (int64_t)p * d.mid_sa;
+            float2 x = res[line * a.ldl + p];
+            if (d.mid_c)
+                x = cmul(x, d.mid_c[off]);
+            else
+                x = cscale(x, d.mid_r[off]);
+            res[line * a.ldl + p] = x;
+        }
+        __syncthreads();
+        float2* other = (res == b0) ? b1 : b0;
+        res = run_stages(a, res, other, -sg);
+    }
+
+    // ---- store (cropping, output weights, masks and axpy folded in)
+    for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
+        uint32_t line, p;
+        split_elem(a, e, line, p);
+        const uint32_t gl = line0 + line;
+        if (gl >= a.nlines) continue;
+        int g = -1;
+        if ((int)p < d.oa)
+            g = d.og1 + (int)p;
+        else if ((int)p >= (int)n - d.ob)
+            g = d.og2 + ((int)p - ((int)n - d.ob));
+        if (g < 0) continue;
+        const uint32_t u = fdiv(gl, a.fnv);
+        const uint32_t v = gl - u * (uint32_t)d.nv;
+        const int64_t off = (int64_t)u * d.out_su + (int64_t)v * d.out_sv + (int64_t)g * d.out_sa;
+        float wgt = d.out_scale;
+        if (d.out_w) wgt *= d.out_w[p];
+        if (d.out_mask) wgt *= d.out_mask[off];
+        float2 x = cscale(res[line * a.ldl + p], wgt);
+        if (OUT_MODE == 0) {
+            reinterpret_cast<float2*>(d.out)[off] = x;
+        } else {
+            float y = x.x;
+            if (d.add_src) y += d.add_scale * d.add_src[off];
+            reinterpret_cast<float*>(d.out)[off] = y;
+        }
+    }
+}
+
+void factorize(int n, std::vector<int>& f) {
+    f.clear();
+    while (n % 4 == 0) { f.push_back(4); n /= 4; }
+    while (n % 2 == 0) { f.push_back(2); n /= 2; }
+    while (n % 3 == 0) { f.push_back(3); n /= 3; }
+    while (n % 5 == 0) { f.push_back(5); n /= 5; }
+    for (int p = 7; n > 1; p += 2)
+        while (n % p == 0) { f.push_back(p); n /= p; }
+}
+
+}  // namespace
+
+int fft1d_init(Fft1D& p, int n) {
+    if (n < 1 || n > 8192) {
+        set_error("fft1d_init: unsupported length %d", n);
+        return CMTIP_ERR_ARG;
+    }
+    std::vector<int> f;
+    factorize(n, f);
+    if ((int)f.size() > kFftMaxStages) {
+        set_error("fft1d_init: too many stages for %d", n);
+        return CMTIP_ERR_ARG;
+    }
+    p.n = n;
+    p.nst = (int)f.size();
+    std::vector<float2> tw;
+    int Ns = 1;
+    for (int s = 0; s < p.nst; ++s) {
+        const int R = f[s];
+        p.R[s] = R;
+        p.Ns[s] = Ns;
+        p.twoff[s] = (int)tw.size();
+        for (int jm = 0; jm < Ns; ++jm)
+            for (int r = 1; r < R; ++r) {
+                const double ang = -2.0 * M_PI * (double)jm * (double)r / ((double)Ns * R);
+                tw.push_back(make_float2((float)cos(ang), (float)sin(ang)));
+            }
+        p.omoff[s] = (int)tw.size();
+        if (R > 5)
+            for (int m = 0; m < R; ++m) {
+                const double ang = -2.0 * M_PI * (double)m / (double)R;
+                tw.push_back(make_float2((float)cos(ang), (float)sin(ang)));
+            }
+        Ns *= R;
+    }
+    if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    CMTIP_TRY(p.tw.ensure(tw.size() * sizeof(float2)));
+    CMTIP_HIP(hipMemcpy(p.tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    return CMTIP_OK;
+}
+
+void fft1d_free(Fft1D& p) { p.tw.release(); }
+
+int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
+    PassArgs a;
+    a.d = d;
+    a.n = plan.n;
+    a.nst = plan.nst;
+    const int64_t nlines = (int64_t)d.nu * d.nv;
+    if (nlines <= 0) return CMTIP_OK;
+    if (d.ia + d.ib > plan.n || d.oa + d.ob > plan.n || (d.mid_c && d.mid_r)) {
+        set_error("fft_axis_pass: bad segments (n=%d ia=%d ib=%d oa=%d ob=%d)", plan.n, d.ia, d.ib,
+                  d.oa, d.ob);
+        return CMTIP_ERR_ARG;
+    }
+    if (nlines * (int64_t)d.nv >= ((int64_t)1 << 32) || nlines >= ((int64_t)1 << 31)) {
+        set_error("fft_axis_pass: too many lines");
+        return CMTIP_ERR_ARG;
+    }
+    // lines per workgroup: a power of two that keeps two LDS line buffers under budget
+    const int ldl = plan.n | 1;
+    int lb = 1;
+    while (lb < 64 && 2 * (2 * lb) * ldl * (int)sizeof(float2) <= kLdsBudget) lb *= 2;
+    int lbshift = 0;
+    while ((1 << lbshift) < lb) ++lbshift;
+    a.lb = lb;
+    a.lbshift = lbshift;
+    a.ldl = ldl;
+    a.ax_contig = (d.in_sa == 1 || d.out_sa == 1) ? 1 : 0;
+    a.nlines = (uint32_t)nlines;
+    a.fn = make_fastdiv((uint32_t)plan.n);
+    a.fnv = make_fastdiv((uint32_t)d.nv);
+    for (int s = 0; s < plan.nst; ++s) {
+        a.R[s] = plan.R[s];
+        a.Ns[s] = plan.Ns[s];
+        a.twoff[s] = plan.twoff[s];
+        a.omoff[s] = plan.omoff[s];
+        a.fnb[s] = make_fastdiv((uint32_t)(plan.n / plan.R[s]));
+        a.fNs[s] = make_fastdiv((uint32_t)plan.Ns[s]);
+    }
+    a.tw = plan.tw.as<const float2>();
+    const size_t lds = (size_t)2 * lb * ldl * sizeof(float2);
+    const unsigned grid = (unsigned)((nlines + lb - 1) / lb);
+    const bool mid = d.mid_c || d.mid_r;
+#define CMTIP_AXIS_LAUNCH(IR, OM, MD) \
+    hipLaunchKernelGGL((axis_pass_kernel<IR, OM, MD>), dim3(grid), dim3(kThreads), lds, stream, a)
+    if (d.in_real) {
+        if (d.out_mode == 0) {
+            if (mid) CMTIP_AXIS_LAUNCH(true, 0, true); else CMTIP_AXIS_LAUNCH(true, 0, false);
+        } else {
+            if (mid) CMTIP_AXIS_LAUNCH(true, 1, true); else CMTIP_AXIS_LAUNCH(true, 1, false);
+        }
+    } else {
+        if (d.out_mode == 0) {
+            if (mid) CMTIP_AXIS_LAUNCH(false, 0, true); else CMTIP_AXIS_LAUNCH(false, 0, false);
+        } else {
+            if (mid) CMTIP_AXIS_LAUNCH(false, 1, true); else CMTIP_AXIS_LAUNCH(false, 1, false);
+        }
+    }
+#undef CMTIP_AXIS_LAUNCH
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+}  // namespace cmtip

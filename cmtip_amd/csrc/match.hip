@@ -1,0 +1,582 @@
+// Orientation matching: weighted Euclidean distance between model slices and images,
+// reduced to a per-image argmin (FP32) or top-k followed by FP32 rescoring (BF16).
+//
+// Reference: cmtip/alignment/align.py:113-131 (per-batch std scaling, weights,
+// sklearn euclidean_distances, np.argmin over references).  The distance matrix is
+// never materialised: the cross term <a_r, w^2 b_i> runs on MFMA and the epilogue
+// forms d2 = gamma^2 an[r] - 2 gamma cross + bn[i] and reduces over r in registers,
+// then with one 64-bit atomicMin per (image, workgroup column).  Keys pack
+// (float bits of d2 >= 0) << 32 | r, so the unsigned order is (d2, r) and ties keep
+// the smallest reference index, as np.argmin does.
+#include "common.h"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kThreads = 256;
+constexpr int BM = 128;  // references per workgroup tile
+constexpr int BN = 128;  // images per workgroup tile
+constexpr unsigned long long kEmpty = ~0ull;
+
+__device__ __forceinline__ unsigned long long pack_key(float d2, uint32_t r) {
+    d2 = fmaxf(d2, 0.0f);  // sklearn clamps negative squared distances to 0
+    return ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned long long)r;
+}
+
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
+    const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) {
+    return a < b ? a : b;
+}
+
+// ---------------------------------------------------------------- small reductions
+
+__global__ void weighted_norms_kernel(const float* __restrict__ X, int64_t P, const float* __restrict__ w2,
+                                      float* __restrict__ out) {
+    __shared__ double red[kThreads / 64];
+    const float* row = X + (int64_t)blockIdx.x * P;
+    double s = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+        const double v = (double)row[p];
+        s += (double)w2[p] * v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+        out[blockIdx.x] = (float)t;
+    }
+}
+
+__global__ void moments_kernel(const float* __restrict__ X, int64_t count, double* __restrict__ out2) {
+    double s1 = 0.0, s2 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) {
+        const double v = (double)X[i];
+        s1 += v;
+        s2 += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_down(s1, off, 64);
+        s2 += __shfl_down(s2, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsafeAtomicAdd(out2, s1);
+        unsafeAtomicAdd(out2 + 1, s2);
+    }
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__global__ void prepare_images_kernel(const float* __restrict__ B, int64_t total, int64_t P,
+                                      const float* __restrict__ w2, float* __restrict__ Bw,
+                                      uint16_t* __restrict__ Bw_bf16) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= total) return;
+    const float v = w2[i % P] * B[i];
+    if (Bw) Bw[i] = v;
+    if (Bw_bf16) Bw_bf16[i] = f32_to_bf16(v);
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t count, unsigned long long v) {
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------- FP32 MFMA argmin
+//
+// 256 threads = 4 waves as 2 (refs) x 2 (images); each wave owns a 64 x 64 block
+// computed as 4 x 4 tiles of v_mfma_f32_16x16x4f32.  Operand layouts of that
+// instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and D[i][j] in
+// lane j + 16 (i / 4), register i % 4.  K is staged through double-buffered LDS in
+// slabs of 32.
+
+constexpr int BK32 = 32;
+constexpr int LD32 = BK32 + 1;  // padded row stride (floats)
+
+__device__ __forceinline__ void load_slab_f32(const float* __restrict__ X, int64_t rows, int64_t P,
+                                              int64_t row0, int64_t k0, float4 (&reg)[4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t r = row0 + t / 8 + 32 * i;
+        const int64_t k = k0 + (t % 8) * 4;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < rows) {
+            const float* src = X + r * P + k;
+            if (k + 3 < P && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+                v = *reinterpret_cast<const float4*>(src);
+            } else {
+                if (k < P) v.x = src[0];
+                if (k + 1 < P) v.y = src[1];
+                if (k + 2 < P) v.z = src[2];
+                if (k + 3 < P) v.w = src[3];
+            }
+        }
+        reg[i] = v;
+    }
+}
+
+__device__ __forceinline__ void store_slab_f32(float* __restrict__ S, const float4 (&reg)[4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float* d = S + (t / 8 + 32 * i) * LD32 + (t % 8) * 4;
+        d[0] = reg[i].x;
+        d[1] = reg[i].y;
+        d[2] = reg[i].z;
+        d[3] = reg[i].w;
+    }
+}
+
+// WRITE_DIST: instead of the argmin, write the full matrix D[r, i] = sqrt(max(d2, 0))
+// (float32, row-major n_ref x n_img) as sklearn's euclidean_distances returns it.
+template <bool WRITE_DIST>
+__global__ void __launch_bounds__(kThreads, 2)
+    match_argmin_f32_kernel(const float* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
+                            const float* __restrict__ Bw, int64_t n_img, int64_t P,
+                            const float* __restrict__ an, const float* __restrict__ bn,
+                            const double* __restrict__ gamma, unsigned long long* __restrict__ best,
+                            float* __restrict__ dist, int tiles_m) {
+    __shared__ float As[2][BM * LD32];
+    __shared__ float Bs[2][BN * LD32];
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    float4 ra[4], rb[4];
+    load_slab_f32(A, n_ref, P, m0, 0, ra);
+    load_slab_f32(Bw, n_img, P, n0, 0, rb);
+    const int nk = (int)((P + BK32 - 1) / BK32);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        store_slab_f32(As[buf], ra);
+        store_slab_f32(Bs[buf], rb);
+        __syncthreads();
+        if (kt + 1 < nk) {
+            load_slab_f32(A, n_ref, P, m0, (int64_t)(kt + 1) * BK32, ra);
+            load_slab_f32(Bw, n_img, P, n0, (int64_t)(kt + 1) * BK32, rb);
+        }
+        const float* as = As[buf] + (wm * 64 + li) * LD32 + lk;
+        const float* bs = Bs[buf] + (wn * 64 + li) * LD32 + lk;
+#pragma unroll
+        for (int kk = 0; kk < BK32 / 4; ++kk) {
+            float af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = as[i * 16 * LD32 + kk * 4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = bs[j * 16 * LD32 + kk * 4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    const float g = gamma ? (float)gamma[0] : 1.0f;
+    const float g2 = g * g, m2g = -2.0f * g;
+    if (WRITE_DIST) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t c = n0 + wn * 64 + j * 16 + li;
+            if (c >= n_img) continue;
+            const float bnc = bn[c];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t r = m0 + wm * 64 + i * 16 + lk * 4 + v;
+                    if (r < n_ref) {
+                        const float d2 = fmaf(g2, an[r], fmaf(m2g, acc[i][j][v], bnc));
+                        dist[r * n_img + c] = sqrtf(fmaxf(d2, 0.0f));
+                    }
+                }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t c = n0 + wn * 64 + j * 16 + li;
+        const float bnc = c < n_img ? bn[c] : 0.f;
+        unsigned long long key = kEmpty;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t r = m0 + wm * 64 + i * 16 + lk * 4 + v;
+                if (r < n_ref) {
+                    const float d2 = fmaf(g2, an[r], fmaf(m2g, acc[i][j][v], bnc));
+                    key = umin64(key, pack_key(d2, (uint32_t)r + ref_offset));
+                }
+            }
+        key = umin64(key, shfl_xor_u64(key, 16));
+        key = umin64(key, shfl_xor_u64(key, 32));
+        if (lk == 0 && c < n_img && key != kEmpty) atomicMin(best + c, key);
+    }
+}
+
+// ---------------------------------------------------------------- BF16 MFMA top-k
+//
+// Same tiling with v_mfma_f32_16x16x32_bf16 (gfx950): A[i][k] in lane i + 16 (k / 8),
+// element k % 8 of an 8-wide bf16 vector; B likewise; D as above.  K slabs of 64.
+// The epilogue writes the tile's d2 to LDS transposed, two threads per image scan
+// 64 references each keeping their 8 smallest keys, and insert up to k of them into
+// the image's global list with an atomicMin cascade: atomicMin(slot s, v) leaves the
+// smaller value in the slot and the larger one moves on to slot s + 1, so slot s ends
+// as the (s+1)-th smallest key ever inserted, whatever the interleaving.
+
+constexpr int BK16 = 64;
+constexpr int LD16 = BK16 + 8;  // padded row stride (bf16), keeps 16-byte alignment
+
+__device__ __forceinline__ void load_slab_bf16(const uint16_t* __restrict__ X, int64_t rows, int64_t P,
+                                               int64_t row0, int64_t k0, uint4 (&reg)[4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t r = row0 + t / 8 + 32 * i;
+        const int64_t k = k0 + (t % 8) * 8;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (r < rows) {
+            const uint16_t* src = X + r * P + k;
+            if (k + 7 < P && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+                v = *reinterpret_cast<const uint4*>(src);
+            } else {
+                uint16_t e[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) e[q] = (k + q < P) ? src[q] : (uint16_t)0;
+                v.x = (uint32_t)e[0] | ((uint32_t)e[1] << 16);
+                v.y = (uint32_t)e[2] | ((uint32_t)e[3] << 16);
+                v.z = (uint32_t)e[4] | ((uint32_t)e[5] << 16);
+                v.w = (uint32_t)e[6] | ((uint32_t)e[7] << 16);
+            }
+        }
+        reg[i] = v;
+    }
+}
+
+__device__ __forceinline__ void store_slab_bf16(uint16_t* __restrict__ S, const uint4 (&reg)[4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        *reinterpret_cast<uint4*>(S + (t / 8 + 32 * i) * LD16 + (t % 8) * 8) = reg[i];
+}
+
+constexpr int kTopK = 8;
+
+__device__ __forceinline__ void topk_insert(unsigned long long (&lst)[kTopK], unsigned long long v) {
+    if (v >= lst[kTopK - 1]) return;
+#pragma unroll
+    for (int s = 0; s < kTopK; ++s) {
+        const unsigned long long lo = umin64(lst[s], v);
+        v = lst[s] < v ? v : lst[s];
+        lst[s] = lo;
+    }
+}
+
+__device__ __forceinline__ void global_insert(unsigned long long* slots, int k, unsigned long long v) {
+    for (int s = 0; s < k; ++s) {
+        const unsigned long long old = atomicMin(slots + s, v);
+        if (old == kEmpty) return;
+        v = old > v ? old : v;
+    }
+}
+
+constexpr size_t kBf16Lds = 2 * (size_t)(BM + BN) * LD16 * sizeof(uint16_t);
+static_assert(kBf16Lds >= (size_t)BM * BN * sizeof(float), "epilogue reuses the operand LDS");
+
+__global__ void __launch_bounds__(kThreads, 2)
+    match_topk_bf16_kernel(const uint16_t* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
+                           const uint16_t* __restrict__ Bw, int64_t n_img, int64_t P,
+                           const float* __restrict__ an, const float* __restrict__ bn,
+                           const double* __restrict__ gamma, int k, unsigned long long* __restrict__ cand,
+                           int tiles_m) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kBf16Lds];
+    uint16_t* As0 = reinterpret_cast<uint16_t*>(lds_raw);
+    uint16_t* Bs0 = As0 + 2 * BM * LD16;
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    uint4 ra[4], rb[4];
+    load_slab_bf16(A, n_ref, P, m0, 0, ra);
+    load_slab_bf16(Bw, n_img, P, n0, 0, rb);
+    const int nk = (int)((P + BK16 - 1) / BK16);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        uint16_t* As = As0 + buf * BM * LD16;
+        uint16_t* Bs = Bs0 + buf * BN * LD16;
+        store_slab_bf16(As, ra);
+        store_slab_bf16(Bs, rb);
+        __syncthreads();
+        if (kt + 1 < nk) {
+            load_slab_bf16(A, n_ref, P, m0, (int64_t)(kt + 1) * BK16, ra);
+            load_slab_bf16(Bw, n_img, P, n0, (int64_t)(kt + 1) * BK16, rb);
+        }
+        const uint16_t* as = As + (wm * 64 + li) * LD16 + lk * 8;
+        const uint16_t* bs = Bs + (wn * 64 + li) * LD16 + lk * 8;
+#pragma unroll
+        for (int kk = 0; kk < BK16 / 32; ++kk) {
+            bf16x8 af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8*>(as + i * 16 * LD16 + kk * 32);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const bf16x8*>(bs + j * 16 * LD16 + kk * 32);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // operand LDS is reused below
+
+    // d2 tile, transposed: D[col][row]
+    float* D = reinterpret_cast<float*>(lds_raw);
+    const float g = (float)gamma[0];
+    const float g2 = g * g, m2g = -2.0f * g;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int cl = wn * 64 + j * 16 + li;
+        const int64_t c = n0 + cl;
+        const float bnc = c < n_img ? bn[c] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rl = wm * 64 + i * 16 + lk * 4 + v;
+                const int64_t r = m0 + rl;
+                const float anr = r < n_ref ? an[r] : 0.f;
+                D[cl * BM + rl] = fmaf(g2, anr, fmaf(m2g, acc[i][j][v], bnc));
+            }
+    }
+    __syncthreads();
+    const int cl = threadIdx.x >> 1, half = threadIdx.x & 1;
+    const int64_t c = n0 + cl;
+    if (c < n_img) {
+        unsigned long long lst[kTopK];
+#pragma unroll
+        for (int s = 0; s < kTopK; ++s) lst[s] = kEmpty;
+        for (int q = 0; q < BM / 2; ++q) {
+            const int rl = half * (BM / 2) + q;
+            const int64_t r = m0 + rl;
+            if (r < n_ref) topk_insert(lst, pack_key(D[cl * BM + rl], (uint32_t)r + ref_offset));
+        }
+        unsigned long long* slots = cand + c * k;
+        for (int s = 0; s < k; ++s) {
+            if (lst[s] == kEmpty || lst[s] >= slots[k - 1]) break;
+            global_insert(slots, k, lst[s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP32 rescoring
+// One workgroup per image; each wave takes candidates s = wave, wave + 4, ...
+
+__global__ void __launch_bounds__(kThreads) rescore_kernel(const float* __restrict__ A, int64_t n_ref_total,
+                                                           const float* __restrict__ B, int64_t P,
+                                                           const float* __restrict__ w2,
+                                                           const double* __restrict__ gamma_of_ref, int k,
+                                                           const unsigned long long* __restrict__ cand,
+                                                           unsigned long long* __restrict__ best) {
+    __shared__ unsigned long long red[kThreads / 64];
+    const int64_t i = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float* b = B + i * P;
+    unsigned long long mine = kEmpty;
+    for (int s = wave; s < k; s += kThreads / 64) {
+        const unsigned long long key = cand[i * k + s];
+        if (key == kEmpty) continue;
+        const uint32_t r = (uint32_t)(key & 0xffffffffu);
+        if ((int64_t)r >= n_ref_total) continue;
+        const float g = (float)gamma_of_ref[r];
+        const float* a = A + (int64_t)r * P;
+        float acc = 0.f;
+        for (int64_t p = lane; p < P; p += 64) {
+            const float dv = fmaf(g, a[p], -b[p]);
+            acc = fmaf(w2[p] * dv, dv, acc);
+        }
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        mine = umin64(mine, pack_key(acc, r));
+    }
+    if (lane == 0) red[wave] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = red[0];
+        for (int q = 1; q < kThreads / 64; ++q) m = umin64(m, red[q]);
+        best[i] = m;
+    }
+}
+
+unsigned grid1(int64_t n, int64_t cap = 1 << 20) {
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+using namespace cmtip;
+
+extern "C" int cmtip_weighted_norms(const float* X, int64_t rows, int64_t P, const float* w2, float* out,
+                                    void* stream) {
+    if (rows < 0 || P < 0 || (rows > 0 && (!X || !w2 || !out))) {
+        set_error("weighted_norms: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (rows == 0) return CMTIP_OK;
+    if (rows > 0x7fffffff) {
+        set_error("weighted_norms: too many rows");
+        return CMTIP_ERR_ARG;
+    }
+    hipLaunchKernelGGL(weighted_norms_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P,
+                       w2, out);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_moments(const float* X, int64_t count, double* out2, void* stream) {
+    if (count < 0 || !out2 || (count > 0 && !X)) {
+        set_error("moments: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (count == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(moments_kernel, dim3(grid1(count, 4096)), dim3(kThreads), 0, as_stream(stream), X, count,
+                       out2);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_prepare_images(const float* B, int64_t n_img, int64_t P, const float* w2, float* Bw,
+                                          uint16_t* Bw_bf16, void* stream) {
+    if (n_img < 0 || P < 0 || (n_img * P > 0 && (!B || !w2 || (!Bw && !Bw_bf16)))) {
+        set_error("match_prepare_images: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    const int64_t total = n_img * P;
+    if (total == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(prepare_images_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, as_stream(stream), B, total, P, w2, Bw, Bw_bf16);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_fill_u64(unsigned long long* dst, int64_t count, unsigned long long value, void* stream) {
+    if (count < 0 || (count > 0 && !dst)) {
+        set_error("fill_u64: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (count == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid1(count, 65536)), dim3(kThreads), 0, as_stream(stream), dst, count,
+                       value);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+static int check_match_args(int64_t n_ref, int64_t ref_offset, int64_t n_img, int64_t P, const void* A,
+                            const void* B, const void* an, const void* bn, const void* gamma, const void* out,
+                            int64_t* tiles_m, int64_t* tiles) {
+    if (n_ref < 0 || n_img < 0 || P < 1 || ref_offset < 0 || n_ref + ref_offset > 0xffffffffLL ||
+        (n_ref > 0 && n_img > 0 && (!A || !B || !an || !bn || !gamma || !out))) {
+        set_error("match: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    *tiles_m = (n_ref + BM - 1) / BM;
+    *tiles = *tiles_m * ((n_img + BN - 1) / BN);
+    if (*tiles > 0x7fffffff) {
+        set_error("match: problem too large for one launch");
+        return CMTIP_ERR_ARG;
+    }
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref_offset, const float* Bw,
+                                      int64_t n_img, int64_t P, const float* an, const float* bn,
+                                      const double* gamma, unsigned long long* best, void* stream) {
+    int64_t tiles_m, tiles;
+    CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, gamma, best, &tiles_m, &tiles));
+    if (tiles == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(match_argmin_f32_kernel<false>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
+                       A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best, (float*)nullptr,
+                       (int)tiles_m);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int64_t n_b, int64_t P,
+                                const float* an, const float* bn, float* D, void* stream) {
+    int64_t tiles_m, tiles;
+    const double one = 1.0;
+    CMTIP_TRY(check_match_args(n_a, 0, n_b, P, A, B, an, bn, &one, D, &tiles_m, &tiles));
+    if (tiles == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(match_argmin_f32_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
+                       A, n_a, 0u, B, n_b, P, an, bn, (const double*)nullptr, (unsigned long long*)nullptr, D,
+                       (int)tiles_m);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t ref_offset, const uint16_t* Bw,
+                                     int64_t n_img, int64_t P, const float* an, const float* bn,
+                                     const double* gamma, int k, unsigned long long* cand, void* stream) {
+    if (k < 1 || k > kTopK) {
+        set_error("match_topk_bf16: k must be in [1, %d]", kTopK);
+        return CMTIP_ERR_ARG;
+    }
+    int64_t tiles_m, tiles;
+    CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, gamma, cand, &tiles_m, &tiles));
+    if (tiles == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(match_topk_bf16_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream), A,
+                       n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, (int)tiles_m);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, const float* B, int64_t n_img,
+                                   int64_t P, const float* w2, const double* gamma_of_ref, int k,
+                                   const unsigned long long* cand, unsigned long long* best, void* stream) {
+    if (k < 1 || k > kTopK || n_img < 0 || P < 1 || n_ref_total < 0 ||
+        (n_img > 0 && (!A_all || !B || !w2 || !gamma_of_ref || !cand || !best))) {
+        set_error("match_rescore: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_img == 0) return CMTIP_OK;
+    if (n_img > 0x7fffffff) {
+        set_error("match_rescore: too many images");
+        return CMTIP_ERR_ARG;
+    }
+    hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
+                       n_ref_total, B, P, w2, gamma_of_ref, k, cand, best);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}

@@ -1,0 +1,732 @@
+// 3-D type-1 / type-2 NUFFT plans, rotated-pixel coordinates and fused slicing.
+//
+// Type 2 (forward, cmtip/nufft/forward.py:13-82):
+//   deconvolve + zero-pad + FFT (three LDS-staged axis passes, pruned to the
+//   nonzero mode block) -> fine grid g (n^3) -> ES interpolation at every point.
+// Type 1 (adjoint, cmtip/nufft/adjoint.py:13-86):
+//   ES spreading of every point onto g -> FFT (three passes, pruned to the kept
+//   modes) -> crop + deconvolve + support + scale folded into the last pass.
+//
+// Only the real part of a result is ever needed by the autocorrelation path
+// (callers keep .real, cmtip/alignment/align.py:61-63, adjoint.py:40-43).  Because
+// the interpolation weights are real, Re(interp(g)) = interp(Re g): the real-output
+// type 2 keeps a real fine grid (half the gather bytes), and a real-strength type 1
+// spreads into a real fine grid (half the atomics).
+#include <math.h>
+
+#include <vector>
+
+#include "es.h"
+#include "fft.h"
+
+using namespace cmtip;
+
+struct cmtip_nufft_plan_s {
+    int type = 0, M = 0, iflag = -1, w = 0, n = 0;
+    double beta = 0.0, eps = 0.0;
+    float tc_hi = 0.f, tc_lo = 0.f;  // n / (2 pi) split in two floats
+    Fft1D fft;
+    DevBuf deconv;   // n floats
+    DevBuf fine;     // n^3 float2 (used as n^3 float when real)
+    DevBuf tmpA;     // M * n * n float2
+    DevBuf tmpB;     // M * M * n float2
+    DevBuf rinv;     // 9 doubles per orientation
+    int slice_ready = 0;
+    int64_t npts = 0;
+    const float* x = nullptr;
+    const float* y = nullptr;
+    const float* z = nullptr;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------- device helpers
+
+__device__ __forceinline__ float es_phi(float z, float two_over_w, float beta) {
+    const float u = z * two_over_w;
+    const float s = 1.0f - u * u;
+    return s > 0.0f ? expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+}
+
+// Fine-grid coordinate t = x n / (2 pi) with the constant split in two floats, so the
+// product carries about one rounding.  Non-finite or huge inputs are clamped so every
+// index below stays in range whatever the caller passes.
+__device__ __forceinline__ float fine_coord(float x, float hi, float lo, int n) {
+    float t = fmaf(x, hi, x * lo);
+    const float lim = 4.0f * (float)n;
+    t = fminf(fmaxf(t, -lim), lim);
+    return t;
+}
+
+template <int W>
+__device__ __forceinline__ void kernel_row(float x, float hi, float lo, int n, float beta, int* idx,
+                                           float* wt) {
+    const float t = fine_coord(x, hi, lo, n);
+    const float l0f = ceilf(t - 0.5f * (float)W);
+    const int l0 = (int)l0f;
+    const float d = l0f - t;  // in (-W/2 - 1, -W/2]
+    int base = l0 % n;
+    if (base < 0) base += n;
+    const float tw = 2.0f / (float)W;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        int i = base + a;
+        i = i >= n ? i - n : i;
+        idx[a] = i;
+        wt[a] = es_phi(d + (float)a, tw, beta);
+    }
+}
+
+// ---------------------------------------------------------------- interpolation
+
+template <int W, bool CPLX>
+__global__ void __launch_bounds__(kBlock) interp_kernel(const void* __restrict__ grid, int n, float beta,
+                                                        float hi, float lo, int64_t N,
+                                                        const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z,
+                                                        float2* __restrict__ out_c,
+                                                        float* __restrict__ out_r) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= N) return;
+    int ix[W], iy[W], iz[W];
+    float wx[W], wy[W], wz[W];
+    kernel_row<W>(x[j], hi, lo, n, beta, ix, wx);
+    kernel_row<W>(y[j], hi, lo, n, beta, iy, wy);
+    kernel_row<W>(z[j], hi, lo, n, beta, iz, wz);
+    if (CPLX) {
+        const float2* g = reinterpret_cast<const float2*>(grid);
+        float sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+            float pr = 0.f, pi = 0.f;
+#pragma unroll 1
+            for (int b = 0; b < W; ++b) {
+                const float2* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
+                float qr = 0.f, qi = 0.f;
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    const float2 v = row[iz[c]];
+                    qr = fmaf(wz[c], v.x, qr);
+                    qi = fmaf(wz[c], v.y, qi);
+                }
+                pr = fmaf(wy[b], qr, pr);
+                pi = fmaf(wy[b], qi, pi);
+            }
+            sr = fmaf(wx[a], pr, sr);
+            si = fmaf(wx[a], pi, si);
+        }
+        if (out_c) out_c[j] = make_float2(sr, si);
+        if (out_r) out_r[j] = sr;
+    } else {
+        const float* g = reinterpret_cast<const float*>(grid);
+        float s = 0.f;
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+            float p = 0.f;
+            for (int b = 0; b < W; ++b) {
+                const float* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
+                float q = 0.f;
+#pragma unroll
+                for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
+                p = fmaf(wy[b], q, p);
+            }
+            s = fmaf(wx[a], p, s);
+        }
+        if (out_c) out_c[j] = make_float2(s, 0.f);
+        if (out_r) out_r[j] = s;
+    }
+}
+
+// ---------------------------------------------------------------- spreading
+
+template <int W, bool CPLX>
+__global__ void __launch_bounds__(kBlock) spread_kernel(void* __restrict__ grid, int n, float beta, float hi,
+                                                        float lo, int64_t N,
+                                                        const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z,
+                                                        const void* __restrict__ c) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= N) return;
+    int ix[W], iy[W], iz[W];
+    float wx[W], wy[W], wz[W];
+    kernel_row<W>(x[j], hi, lo, n, beta, ix, wx);
+    kernel_row<W>(y[j], hi, lo, n, beta, iy, wy);
+    kernel_row<W>(z[j], hi, lo, n, beta, iz, wz);
+    if (CPLX) {
+        const float2 v = reinterpret_cast<const float2*>(c)[j];
+        float* g = reinterpret_cast<float*>(grid);
+        for (int a = 0; a < W; ++a)
+            for (int b = 0; b < W; ++b) {
+                const float wab = wx[a] * wy[b];
+                const float vr = wab * v.x, vi = wab * v.y;
+                float* row = g + 2 * (((int64_t)ix[a] * n + iy[b]) * n);
+#pragma unroll
+                for (int cc = 0; cc < W; ++cc) {
+                    unsafeAtomicAdd(row + 2 * iz[cc], wz[cc] * vr);
+                    unsafeAtomicAdd(row + 2 * iz[cc] + 1, wz[cc] * vi);
+                }
+            }
+    } else {
+        const float v = c ? reinterpret_cast<const float*>(c)[j] : 1.0f;
+        float* g = reinterpret_cast<float*>(grid);
+        for (int a = 0; a < W; ++a)
+            for (int b = 0; b < W; ++b) {
+                const float vr = wx[a] * wy[b] * v;
+                float* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
+#pragma unroll
+                for (int cc = 0; cc < W; ++cc) unsafeAtomicAdd(row + iz[cc], wz[cc] * vr);
+            }
+    }
+}
+
+__global__ void fill_f32_kernel(float* __restrict__ p, int64_t count, float v) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t k = i; k < count; k += stride) p[k] = v;
+}
+
+// ---------------------------------------------------------------- rotations
+
+// Inverse of the rotation matrix of a scalar-first quaternion.  The matrix entries
+// follow cmtip/eval/geom.py:25-45 (quaternion products and sums in the quaternion's
+// float32 precision, the 1 - 2(.) and 2(.) steps in float64), and the inverse is the
+// exact adjugate / determinant in float64, as np.linalg.inv on that matrix.
+__device__ void quat_rotinv(const float* q, double* out) {
+#pragma clang fp contract(off)
+    const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const float q01 = q0 * q1, q02 = q0 * q2, q03 = q0 * q3;
+    const float q11 = q1 * q1, q12 = q1 * q2, q13 = q1 * q3;
+    const float q22 = q2 * q2, q23 = q2 * q3, q33 = q3 * q3;
+    double R[9];
+    R[0] = 1.0 - 2.0 * (double)(q22 + q33);
+    R[1] = 2.0 * (double)(q12 - q03);
+    R[2] = 2.0 * (double)(q13 + q02);
+    R[3] = 2.0 * (double)(q12 + q03);
+    R[4] = 1.0 - 2.0 * (double)(q11 + q33);
+    R[5] = 2.0 * (double)(q23 - q01);
+    R[6] = 2.0 * (double)(q13 - q02);
+    R[7] = 2.0 * (double)(q23 + q01);
+    R[8] = 1.0 - 2.0 * (double)(q11 + q22);
+    const double c00 = R[4] * R[8] - R[5] * R[7];
+    const double c01 = R[5] * R[6] - R[3] * R[8];
+    const double c02 = R[3] * R[7] - R[4] * R[6];
+    const double det = R[0] * c00 + R[1] * c01 + R[2] * c02;
+    const double id = 1.0 / det;
+    out[0] = c00 * id;
+    out[1] = (R[2] * R[7] - R[1] * R[8]) * id;
+    out[2] = (R[1] * R[5] - R[2] * R[4]) * id;
+    out[3] = c01 * id;
+    out[4] = (R[0] * R[8] - R[2] * R[6]) * id;
+    out[5] = (R[2] * R[3] - R[0] * R[5]) * id;
+    out[6] = c02 * id;
+    out[7] = (R[1] * R[6] - R[0] * R[7]) * id;
+    out[8] = (R[0] * R[4] - R[1] * R[3]) * id;
+}
+
+__global__ void rotinv_kernel(const float* __restrict__ quats, int64_t n, double* __restrict__ rinv) {
+    const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= n) return;
+    quat_rotinv(quats + 4 * o, rinv + 9 * o);
+}
+
+// x = fl32(fl32(row . s) / ext) * fl32(pi): the reference's einsum in float64, then
+// astype(float32), / reciprocal_extent, * np.pi in float32 (align.py:52-57).
+// ext == 0 returns the unscaled fl32(row . s) (autocorrelation.py:27,210).
+__device__ __forceinline__ float rot_coord(const double* r, double s0, double s1, double s2, float ext) {
+#pragma clang fp contract(off)
+    const double h = r[0] * s0 + r[1] * s1 + r[2] * s2;
+    const float hf = (float)h;
+    return ext == 0.0f ? hf : (hf / ext) * 3.14159265358979323846f;
+}
+
+__global__ void __launch_bounds__(kBlock) rotate_coords_kernel(const double* __restrict__ rinv,
+                                                               const double* __restrict__ s, int64_t P,
+                                                               int64_t total, float ext,
+                                                               float* __restrict__ x,
+                                                               float* __restrict__ y,
+                                                               float* __restrict__ z) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= total) return;
+    const int64_t o = j / P, p = j - o * P;
+    const double* r = rinv + 9 * o;
+    const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
+    x[j] = rot_coord(r, s0, s1, s2, ext);
+    y[j] = rot_coord(r + 3, s0, s1, s2, ext);
+    z[j] = rot_coord(r + 6, s0, s1, s2, ext);
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Fused compute_slices for a real fine grid: coordinates from (rinv, s), real
+// interpolation, float32 (+ optional bfloat16) store and FP64 moments.
+template <int W>
+__global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restrict__ g, int n, float beta,
+                                                            float hi, float lo,
+                                                            const double* __restrict__ rinv,
+                                                            const double* __restrict__ s, int64_t P,
+                                                            int64_t total, float ext,
+                                                            float* __restrict__ out,
+                                                            uint16_t* __restrict__ out_bf16,
+                                                            double* __restrict__ moments) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    float val = 0.f;
+    if (j < total) {
+        const int64_t o = j / P, p = j - o * P;
+        const double* r = rinv + 9 * o;
+        const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
+        int ix[W], iy[W], iz[W];
+        float wx[W], wy[W], wz[W];
+        kernel_row<W>(rot_coord(r, s0, s1, s2, ext), hi, lo, n, beta, ix, wx);
+        kernel_row<W>(rot_coord(r + 3, s0, s1, s2, ext), hi, lo, n, beta, iy, wy);
+        kernel_row<W>(rot_coord(r + 6, s0, s1, s2, ext), hi, lo, n, beta, iz, wz);
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+            float pa = 0.f;
+            for (int b = 0; b < W; ++b) {
+                const float* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
+                float q = 0.f;
+#pragma unroll
+                for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
+                pa = fmaf(wy[b], q, pa);
+            }
+            acc = fmaf(wx[a], pa, acc);
+        }
+        val = acc;
+        out[j] = val;
+        if (out_bf16) out_bf16[j] = f32_to_bf16(val);
+    }
+    if (moments) {
+        double s1v = (double)val, s2v = (double)val * (double)val;
+        // wavefront reduction, then one atomic per wavefront
+        for (int off = 32; off > 0; off >>= 1) {
+            s1v += __shfl_down(s1v, off, 64);
+            s2v += __shfl_down(s2v, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            unsafeAtomicAdd(moments, s1v);
+            unsafeAtomicAdd(moments + 1, s2v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- dispatch helpers
+
+template <template <int> class F, class... Args>
+int dispatch_w(int w, Args... args) {
+    switch (w) {
+#define CMTIP_W(WV) \
+    case WV: return F<WV>::run(args...);
+        CMTIP_W(2) CMTIP_W(3) CMTIP_W(4) CMTIP_W(5) CMTIP_W(6) CMTIP_W(7) CMTIP_W(8) CMTIP_W(9)
+        CMTIP_W(10) CMTIP_W(11) CMTIP_W(12) CMTIP_W(13) CMTIP_W(14) CMTIP_W(15) CMTIP_W(16)
+#undef CMTIP_W
+        default:
+            set_error("unsupported kernel width %d", w);
+            return CMTIP_ERR_ARG;
+    }
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <int W>
+struct InterpLaunch {
+    static int run(const cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, hipStream_t st) {
+        if (p->npts == 0) return CMTIP_OK;
+        if (cplx)
+            hipLaunchKernelGGL((interp_kernel<W, true>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
+                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->z, out_c, out_r);
+        else
+            hipLaunchKernelGGL((interp_kernel<W, false>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
+                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->z, out_c, out_r);
+        CMTIP_LAUNCH_CHECK();
+        return CMTIP_OK;
+    }
+};
+
+template <int W>
+struct SpreadLaunch {
+    static int run(const cmtip_nufft_plan_s* p, bool cplx, const void* c, hipStream_t st) {
+        if (p->npts == 0) return CMTIP_OK;
+        if (cplx)
+            hipLaunchKernelGGL((spread_kernel<W, true>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
+                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->z, c);
+        else
+            hipLaunchKernelGGL((spread_kernel<W, false>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
+                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->z, c);
+        CMTIP_LAUNCH_CHECK();
+        return CMTIP_OK;
+    }
+};
+
+template <int W>
+struct SliceLaunch {
+    static int run(const cmtip_nufft_plan_s* p, const double* s, int64_t P, int64_t total, float ext,
+                   float* out, uint16_t* out_bf16, double* moments, hipStream_t st) {
+        hipLaunchKernelGGL((slice_real_kernel<W>), dim3(blocks_for(total)), dim3(kBlock), 0, st,
+                           p->fine.as<const float>(), p->n, (float)p->beta, p->tc_hi, p->tc_lo,
+                           p->rinv.as<const double>(), s, P, total, ext, out, out_bf16, moments);
+        CMTIP_LAUNCH_CHECK();
+        return CMTIP_OK;
+    }
+};
+
+int fill_zero(void* p, int64_t nfloats, hipStream_t st) {
+    if (nfloats <= 0) return CMTIP_OK;
+    int64_t nb = (nfloats + kBlock - 1) / kBlock;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)nb), dim3(kBlock), 0, st, (float*)p, nfloats, 0.0f);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+// Segments of an n-point line holding M centred modes: LDS positions [0, ceil(M/2))
+// hold k >= 0 (array index k + floor(M/2)); [n - floor(M/2), n) hold k < 0
+// (array index 0 ...).
+void mode_segments_in(AxisDesc& d, int M) {
+    d.ia = M - M / 2;
+    d.ig1 = M / 2;
+    d.ib = M / 2;
+    d.ig2 = 0;
+}
+void mode_segments_out(AxisDesc& d, int M) {
+    d.oa = M - M / 2;
+    d.og1 = M / 2;
+    d.ob = M / 2;
+    d.og2 = 0;
+}
+
+// Uniform grid f (M^3, float or float2) -> fine grid (n^3), real part only when
+// real_out.  Deconvolution weights on every axis; support multiplies f on load.
+int grid_to_fine(cmtip_nufft_plan_s* p, const void* f, bool f_real, const float* support, bool real_out,
+                 hipStream_t st) {
+    const int M = p->M, n = p->n;
+    const float* dw = p->deconv.as<const float>();
+    // pass 1: axis 2 (contiguous), lines (i0, i1)
+    AxisDesc d1;
+    d1.in = f;
+    d1.in_real = f_real ? 1 : 0;
+    d1.in_su = (int64_t)M * M;
+    d1.in_sv = M;
+    d1.in_sa = 1;
+    mode_segments_in(d1, M);
+    d1.in_w = dw;
+    d1.in_mask = support;
+    d1.out = p->tmpB.p;
+    d1.out_su = (int64_t)M * n;
+    d1.out_sv = n;
+    d1.out_sa = 1;
+    d1.oa = n;
+    d1.og1 = 0;
+    d1.nu = M;
+    d1.nv = M;
+    d1.sign = p->iflag;
+    CMTIP_TRY(fft_axis_pass(p->fft, d1, st));
+    // pass 2: axis 1, lines (i0, l2)
+    AxisDesc d2;
+    d2.in = p->tmpB.p;
+    d2.in_su = (int64_t)M * n;
+    d2.in_sv = 1;
+    d2.in_sa = n;
+    mode_segments_in(d2, M);
+    d2.in_w = dw;
+    d2.out = p->tmpA.p;
+    d2.out_su = (int64_t)n * n;
+    d2.out_sv = 1;
+    d2.out_sa = n;
+    d2.oa = n;
+    d2.og1 = 0;
+    d2.nu = M;
+    d2.nv = n;
+    d2.sign = p->iflag;
+    CMTIP_TRY(fft_axis_pass(p->fft, d2, st));
+    // pass 3: axis 0, lines (l1, l2)
+    AxisDesc d3;
+    d3.in = p->tmpA.p;
+    d3.in_su = n;
+    d3.in_sv = 1;
+    d3.in_sa = (int64_t)n * n;
+    mode_segments_in(d3, M);
+    d3.in_w = dw;
+    d3.out = p->fine.p;
+    d3.out_mode = real_out ? 1 : 0;
+    d3.out_su = n;
+    d3.out_sv = 1;
+    d3.out_sa = (int64_t)n * n;
+    d3.oa = n;
+    d3.og1 = 0;
+    d3.nu = n;
+    d3.nv = n;
+    d3.sign = p->iflag;
+    CMTIP_TRY(fft_axis_pass(p->fft, d3, st));
+    return CMTIP_OK;
+}
+
+// Fine grid (n^3, float or float2) -> uniform grid f (M^3): crop, deconvolve,
+// support, scale, real part when real_out.
+int fine_to_grid(cmtip_nufft_plan_s* p, bool fine_real, void* f, bool real_out, const float* support,
+                 float scale, hipStream_t st) {
+    const int M = p->M, n = p->n;
+    const float* dw = p->deconv.as<const float>();
+    // pass A: axis 0, lines (l1, l2)
+    AxisDesc a;
+    a.in = p->fine.p;
+    a.in_real = fine_real ? 1 : 0;
+    a.in_su = n;
+    a.in_sv = 1;
+    a.in_sa = (int64_t)n * n;
+    a.ia = n;
+    a.ig1 = 0;
+    a.out = p->tmpA.p;
+    a.out_su = n;
+    a.out_sv = 1;
+    a.out_sa = (int64_t)n * n;
+    mode_segments_out(a, M);
+    a.out_w = dw;
+    a.nu = n;
+    a.nv = n;
+    a.sign = p->iflag;
+    CMTIP_TRY(fft_axis_pass(p->fft, a, st));
+    // pass B: axis 1, lines (i0, l2)
+    AxisDesc b;
+    b.in = p->tmpA.p;
+    b.in_su = (int64_t)n * n;
+    b.in_sv = 1;
+    b.in_sa = n;
+    b.ia = n;
+    b.ig1 = 0;
+    b.out = p->tmpB.p;
+    b.out_su = (int64_t)M * n;
+    b.out_sv = 1;
+    b.out_sa = n;
+    mode_segments_out(b, M);
+    b.out_w = dw;
+    b.nu = M;
+    b.nv = n;
+    b.sign = p->iflag;
+    CMTIP_TRY(fft_axis_pass(p->fft, b, st));
+    // pass C: axis 2 (contiguous), lines (i0, i1)
+    AxisDesc c;
+    c.in = p->tmpB.p;
+    c.in_su = (int64_t)M * n;
+    c.in_sv = n;
+    c.in_sa = 1;
+    c.ia = n;
+    c.ig1 = 0;
+    c.out = f;
+    c.out_mode = real_out ? 1 : 0;
+    c.out_su = (int64_t)M * M;
+    c.out_sv = M;
+    c.out_sa = 1;
+    mode_segments_out(c, M);
+    c.out_w = dw;
+    c.out_scale = scale;
+    c.out_mask = support;
+    c.nu = M;
+    c.nv = M;
+    c.sign = p->iflag;
+    CMTIP_TRY(fft_axis_pass(p->fft, c, st));
+    return CMTIP_OK;
+}
+
+bool valid_plan(cmtip_nufft_plan p) {
+    if (!p) {
+        set_error("null plan");
+        return false;
+    }
+    return true;
+}
+
+int ensure_rinv(cmtip_nufft_plan_s* p, const float* quats, int64_t n_orient, hipStream_t st) {
+    CMTIP_TRY(p->rinv.ensure((size_t)(n_orient > 0 ? n_orient : 1) * 9 * sizeof(double)));
+    if (n_orient > 0) {
+        hipLaunchKernelGGL(rotinv_kernel, dim3(blocks_for(n_orient)), dim3(kBlock), 0, st, quats, n_orient,
+                           p->rinv.as<double>());
+        CMTIP_LAUNCH_CHECK();
+    }
+    return CMTIP_OK;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+
+extern "C" int cmtip_nufft_makeplan(int type, int n_modes, int iflag, double eps, cmtip_nufft_plan* out) {
+    if (!out) {
+        set_error("makeplan: null output");
+        return CMTIP_ERR_ARG;
+    }
+    *out = nullptr;
+    if ((type != 1 && type != 2) || n_modes < 1 || n_modes > 1024 || (iflag != 1 && iflag != -1)) {
+        set_error("makeplan: bad arguments (type=%d, n_modes=%d, iflag=%d)", type, n_modes, iflag);
+        return CMTIP_ERR_ARG;
+    }
+    auto* p = new cmtip_nufft_plan_s();
+    p->type = type;
+    p->M = n_modes;
+    p->iflag = iflag;
+    p->eps = eps;
+    EsParams es = es_params(eps);
+    p->w = es.w;
+    p->beta = es.beta;
+    p->n = fine_size(n_modes, es.w);
+    const double tc = (double)p->n / (2.0 * M_PI);
+    p->tc_hi = (float)tc;
+    p->tc_lo = (float)(tc - (double)p->tc_hi);
+    const int n = p->n, M = n_modes;
+    int st = fft1d_init(p->fft, n);
+    if (st == CMTIP_OK) st = p->deconv.ensure(sizeof(float) * n);
+    if (st == CMTIP_OK) {
+        std::vector<float> dw = deconv_weights(M, n, p->w, p->beta);
+        if (hipMemcpy(p->deconv.p, dw.data(), sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("makeplan: hipMemcpy failed");
+            st = CMTIP_ERR_HIP;
+        }
+    }
+    if (st == CMTIP_OK) st = p->fine.ensure(sizeof(float2) * (size_t)n * n * n);
+    if (st == CMTIP_OK) st = p->tmpA.ensure(sizeof(float2) * (size_t)M * n * n);
+    if (st == CMTIP_OK) st = p->tmpB.ensure(sizeof(float2) * (size_t)M * M * n);
+    if (st != CMTIP_OK) {
+        cmtip_nufft_destroy(p);
+        return st;
+    }
+    *out = p;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_nufft_info(cmtip_nufft_plan p, int* w, int* n_fine, double* beta) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (w) *w = p->w;
+    if (n_fine) *n_fine = p->n;
+    if (beta) *beta = p->beta;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y,
+                                  const float* z, void* stream) {
+    (void)stream;
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (npts < 0 || (npts > 0 && (!x || !y || !z))) {
+        set_error("setpts: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    p->npts = npts;
+    p->x = x;
+    p->y = y;
+    p->z = z;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int flags, const float* support,
+                                   float scale, void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    const bool grid_real = flags & CMTIP_GRID_REAL;
+    const bool pts_ones = (flags & CMTIP_PTS_ONES) && p->type == 1;
+    const bool pts_real = (flags & CMTIP_PTS_REAL) || pts_ones;
+    if (pts_ones) c = nullptr;
+    if (!f || (p->npts > 0 && !c && !pts_ones)) {
+        set_error("execute: null data pointer");
+        return CMTIP_ERR_ARG;
+    }
+    if (p->npts > 0 && !p->x) {
+        set_error("execute: setpts not called");
+        return CMTIP_ERR_STATE;
+    }
+    if (p->type == 2) {
+        p->slice_ready = 0;
+        CMTIP_TRY(grid_to_fine(p, f, grid_real, support, pts_real, st));
+        return dispatch_w<InterpLaunch>(p->w, (const cmtip_nufft_plan_s*)p, !pts_real,
+                                        pts_real ? (float2*)nullptr : (float2*)c,
+                                        pts_real ? (float*)c : (float*)nullptr, st);
+    }
+    // type 1
+    p->slice_ready = 0;
+    const int64_t n3 = (int64_t)p->n * p->n * p->n;
+    CMTIP_TRY(fill_zero(p->fine.p, pts_real ? n3 : 2 * n3, st));
+    CMTIP_TRY(dispatch_w<SpreadLaunch>(p->w, (const cmtip_nufft_plan_s*)p, !pts_real, (const void*)c, st));
+    return fine_to_grid(p, pts_real, f, grid_real, support, scale, st);
+}
+
+extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
+    if (!p) return CMTIP_OK;
+    fft1d_free(p->fft);
+    p->deconv.release();
+    p->fine.release();
+    p->tmpA.release();
+    p->tmpB.release();
+    p->rinv.release();
+    delete p;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_rotate_coords(const float* quats, int64_t n_orient, const double* s, int64_t P,
+                                   float recip_extent, float* x, float* y, float* z, void* stream) {
+    if (n_orient < 0 || P < 0 || (n_orient > 0 && P > 0 && (!quats || !s || !x || !y || !z))) {
+        set_error("rotate_coords: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_orient == 0 || P == 0) return CMTIP_OK;
+    hipStream_t st = as_stream(stream);
+    DevBuf rinv;
+    CMTIP_TRY(rinv.ensure((size_t)n_orient * 9 * sizeof(double)));
+    hipLaunchKernelGGL(rotinv_kernel, dim3(blocks_for(n_orient)), dim3(kBlock), 0, st, quats, n_orient,
+                       rinv.as<double>());
+    hipError_t e = hipGetLastError();
+    const int64_t total = n_orient * P;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rotate_coords_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, st,
+                           rinv.as<const double>(), s, P, total, recip_extent, x, y, z);
+        e = hipGetLastError();
+    }
+    // the scratch matrix buffer must outlive the kernels
+    hipError_t e2 = hipStreamSynchronize(st);
+    rinv.release();
+    if (e != hipSuccess || e2 != hipSuccess) {
+        set_error("rotate_coords: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        return CMTIP_ERR_LAUNCH;
+    }
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_slice_prepare(cmtip_nufft_plan p, const float* ac, const float* support, void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (p->type != 2 || !ac) {
+        set_error("slice_prepare: needs a type-2 plan and an ac grid");
+        return CMTIP_ERR_ARG;
+    }
+    CMTIP_TRY(grid_to_fine(p, ac, true, support, true, as_stream(stream)));
+    p->slice_ready = 1;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t n_orient, const double* s,
+                                int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
+                                double* moments, void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (!p->slice_ready) {
+        set_error("slice_real: cmtip_slice_prepare not called since the last execute");
+        return CMTIP_ERR_STATE;
+    }
+    if (n_orient < 0 || P < 0 || (n_orient > 0 && P > 0 && (!quats || !s || !out))) {
+        set_error("slice_real: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_orient == 0 || P == 0) return CMTIP_OK;
+    hipStream_t st = as_stream(stream);
+    CMTIP_TRY(ensure_rinv(p, quats, n_orient, st));
+    return dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, P, n_orient * P, recip_extent,
+                                   out, out_bf16, moments, st);
+}

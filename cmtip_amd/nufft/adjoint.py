@@ -1,0 +1,59 @@
+"""Adjoint (type 1) NUFFT: nonuniform points -> grid, isign +1, divided by M^3.
+
+Mirrors cmtip/nufft/adjoint.py:13-86.  Both names run the HIP path.
+"""
+import numpy as np
+import torch
+
+from .. import _arrays
+from .._lib import CMTIP_GRID_REAL, CMTIP_PTS_ONES, CMTIP_PTS_REAL
+from .plan import DEFAULT_EPS, get_plan
+
+
+def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False):
+    assert _arrays.max_abs(H_, K_, L_) < 3 * np.pi  # adjoint.py:29, 66
+    as_torch = _arrays.is_torch(nuvect, H_)
+    x = _arrays.to_dev(H_, torch.float32).reshape(-1)
+    y = _arrays.to_dev(K_, torch.float32).reshape(-1)
+    z = _arrays.to_dev(L_, torch.float32).reshape(-1)
+    flags = 0
+    c = None
+    if ones:
+        flags |= CMTIP_PTS_ONES
+    else:
+        cplx = torch.is_complex(nuvect) if isinstance(nuvect, torch.Tensor) else np.iscomplexobj(nuvect)
+        if cplx:
+            c = _arrays.to_dev(nuvect, torch.complex64).reshape(-1)
+        else:
+            c = _arrays.to_dev(nuvect, torch.float32).reshape(-1)
+            flags |= CMTIP_PTS_REAL
+        assert c.numel() == x.numel()
+    M = int(M)
+    if use_recip_sym:
+        f = torch.empty((M, M, M), dtype=torch.float32, device=x.device)
+        flags |= CMTIP_GRID_REAL
+    else:
+        f = torch.empty((M, M, M), dtype=torch.complex64, device=x.device)
+    sup = None if support is None else _arrays.to_dev(support, torch.float32)
+    plan = get_plan(1, M, +1, eps)
+    plan.setpts(x, y, z)
+    plan.execute(c, f, flags, sup, float(1.0 / M ** 3) if scale is None else float(scale))
+    return _arrays.out(f, as_torch)
+
+
+def adjoint_cpu(nuvect, H_, K_, L_, M, use_recip_sym=True, support=None, *, eps=DEFAULT_EPS):
+    """Type-1 NUFFT: f_k = sum_j c_j exp(+i k.x_j), times support, real part when
+    use_recip_sym, divided by M^3.  Signature of cmtip/nufft/adjoint.py:13; runs on the
+    GPU (the finufft call at adjoint.py:33 is replaced)."""
+    return _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps)
+
+
+def adjoint_gpu(nuvect, H_, K_, L_, M, use_recip_sym=True, support=None, *, eps=DEFAULT_EPS):
+    """Type-1 NUFFT; signature of cmtip/nufft/adjoint.py:46."""
+    return _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps)
+
+
+def adjoint_ones(H_, K_, L_, M, *, scale=None, eps=DEFAULT_EPS):
+    """Real type-1 of unit strengths (the kernel of autocorrelation.py:143-147) without
+    materialising the ones; scale defaults to 1/M^3 as adjoint_* divides."""
+    return _adjoint(None, H_, K_, L_, M, True, None, eps, scale=scale, ones=True)

@@ -1,0 +1,219 @@
+/*
+ * cmtip_amd C ABI: the hot path of cartesian MTIP reconstruction (apeck12/cmtip) as
+ * hand-written HIP kernels for MI355X (gfx950).
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t passed as
+ * void*; none allocates host-visible state beyond the plan/solver objects, none
+ * synchronises the device unless its comment says so, and every one returns an int
+ * status (CMTIP_OK on success).  No exception crosses the ABI.  After a failure,
+ * cmtip_last_error() returns a message for the calling thread.
+ *
+ * Array conventions (the reference's CPU path, cmtip/nufft/forward.py:40 and
+ * cmtip/nufft/adjoint.py:33, is normative):
+ *   - Uniform grids are (M, M, M) row-major (last axis fastest).  Array index i along
+ *     an axis holds Fourier mode k = i - floor(M/2).
+ *   - Point coordinate x pairs with array axis 0, y with axis 1, z with axis 2.
+ *   - Type 2 ("forward"):  c_j = sum_k f_k exp(-i k.x_j)       (iflag -1)
+ *     Type 1 ("adjoint"):  f_k = sum_j c_j exp(+i k.x_j)       (iflag +1)
+ *   - Coordinates must satisfy |x| < 3*pi (forward.py:27, adjoint.py:29).
+ *   - Complex arrays are interleaved float pairs (numpy complex64 / torch.complex64).
+ *
+ * Plans and solvers are not thread-safe; use one per stream.
+ */
+#ifndef CMTIP_AMD_H
+#define CMTIP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define CMTIP_OK 0
+#define CMTIP_ERR_ARG 1     /* bad size, pointer or flag combination */
+#define CMTIP_ERR_ALLOC 2   /* device allocation failed */
+#define CMTIP_ERR_HIP 3     /* a HIP runtime call failed */
+#define CMTIP_ERR_LAUNCH 4  /* a kernel launch failed */
+#define CMTIP_ERR_RANGE 5   /* a nonuniform point lies outside [-3pi, 3pi) */
+#define CMTIP_ERR_STATE 6   /* call order violated (e.g. execute before setpts) */
+
+/* Message for the last failure on the calling thread ("" if none). */
+const char* cmtip_last_error(void);
+/* ABI version: major*10000 + minor*100 + patch. */
+int cmtip_version(void);
+
+/* ---------------------------------------------------------------- NUFFT plans
+ * Replace the cufinufft v1 plan object used by the reference GPU path
+ *   cufinufft(type, n_modes, n_trans=1, isign, eps, dtype=float32, gpu_method=1)
+ *   -> set_pts(...) -> execute(c, f)                (cmtip/nufft/forward.py:77-81,
+ *                                                    cmtip/nufft/adjoint.py:73-76)
+ * and finufft.nufft3d2 / finufft.nufft3d1 on the CPU path (forward.py:40,
+ * adjoint.py:33).  FP32 arithmetic; the exponential-of-semicircle kernel width w
+ * follows from eps (w = ceil(log10(1/eps)) + 1, 2 <= w <= 16, upsampling 2).
+ */
+typedef struct cmtip_nufft_plan_s* cmtip_nufft_plan;
+
+/* type: 1 or 2.  n_modes: M (cubic).  iflag: sign of the exponent (+1 / -1).
+ * The plan owns its fine grid and FFT scratch.  Allocates device memory. */
+int cmtip_nufft_makeplan(int type, int n_modes, int iflag, double eps, cmtip_nufft_plan* plan);
+
+/* Kernel width, fine-grid edge and ES shape parameter of a plan (any pointer may be NULL). */
+int cmtip_nufft_info(cmtip_nufft_plan plan, int* w, int* n_fine, double* beta);
+
+/* Register npts nonuniform points (device float arrays, caller-owned, kept by
+ * reference until the next setpts).  Replaces cufinufft set_pts (forward.py:78,
+ * adjoint.py:74).  Type-1 plans also sort the points into spreading bins here. */
+int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, const float* y,
+                       const float* z, void* stream);
+
+/* execute flags */
+#define CMTIP_GRID_REAL 1   /* the uniform grid f is float32 (real); else complex64 */
+#define CMTIP_PTS_REAL 2    /* the point values c are float32 (real); else complex64 */
+#define CMTIP_PTS_ONES 4    /* type 1 only: every strength is 1 (c is ignored), as the
+                               all-ones kernel input of autocorrelation.py:143,146 */
+
+/* Type 2: c = T2(support (.) f); the support (float32 M^3, may be NULL) multiplies f
+ * on load (forward.py:34-35); with CMTIP_PTS_REAL only Re(c) is written.
+ * Type 1: f = scale * support (.) T1(c); with CMTIP_GRID_REAL only Re(f) is written;
+ * CMTIP_PTS_REAL means the strengths c are real.  The reference's adjoint is
+ * scale = 1/M^3 with the support applied after the transform (adjoint.py:35-43).
+ * Replaces cufinufft execute (forward.py:79, adjoint.py:75). */
+int cmtip_nufft_execute(cmtip_nufft_plan plan, void* c, void* f, int flags, const float* support,
+                        float scale, void* stream);
+
+int cmtip_nufft_destroy(cmtip_nufft_plan plan);
+
+/* ---------------------------------------------------------------- rotated q-vectors
+ * Coordinates of every pixel under every orientation, orientation-major
+ * (j = o*P + p):  x_j = fl32(fl32((R(q_o)^-1 s_p)_0) / fl32(recip_extent)) * fl32(pi),
+ * likewise y (row 1) and z (row 2).  R(q) is the scalar-first quaternion rotation of
+ * cmtip/eval/geom.py:37-45; its inverse is taken exactly (adjugate / determinant, FP64)
+ * as np.linalg.inv does in cmtip/alignment/align.py:51 and
+ * cmtip/autocorrelation/autocorrelation.py:24.  The rotated vector is computed in FP64
+ * and rounded once, as the reference's einsum + astype(float32) does (align.py:52-57,
+ * autocorrelation.py:27,133-135,210).
+ * quats: n_orient x 4 float32; s: 3 x P float64 (pixel_position_reciprocal).
+ * recip_extent = 0 writes the unscaled fl32(R^-1 s) (gen_nonuniform_positions). */
+int cmtip_rotate_coords(const float* quats, int64_t n_orient, const double* s, int64_t P,
+                        float recip_extent, float* x, float* y, float* z, void* stream);
+
+/* ---------------------------------------------------------------- fused slicing
+ * compute_slices (cmtip/alignment/align.py:39-65) for a real autocorrelation, in two
+ * steps.  cmtip_slice_prepare loads ac (float32 M^3) into the type-2 plan's fine grid
+ * (deconvolution, zero padding and three FFT passes; the grid kept is Re(FFT), which
+ * is exactly what Re(c) needs for a real ac).  cmtip_slice_real then evaluates
+ * Re(T2(ac)) at the rotated pixel coordinates, generating them in-kernel from
+ * (quats, s, recip_extent) exactly as cmtip_rotate_coords does, and writes
+ * out[o*P + p] (float32).  out_bf16 (optional, uint16 bfloat16 bits, round to
+ * nearest even) receives the same values.  moments (optional, 2 doubles, accumulated,
+ * not cleared) receives sum(out) and sum(out^2) for the reference's per-batch std
+ * scaling (align.py:122-124). */
+int cmtip_slice_prepare(cmtip_nufft_plan plan, const float* ac, const float* support, void* stream);
+int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient, const double* s,
+                     int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
+                     double* moments, void* stream);
+
+/* ---------------------------------------------------------------- matching
+ * Orientation matching (cmtip/alignment/align.py:113-131): for every image i, the
+ * reference r minimising the weighted Euclidean distance
+ *     d2[r,i] = sum_p w_p^2 (gamma a[r,p] - b[i,p])^2
+ *             = gamma^2 an[r] - 2 gamma <a_r, b'_i> + bn[i]
+ * (sklearn euclidean_distances, align.py:35, on model_slices*gamma*w and slices*w).
+ * b' = w^2 (.) b is prepared by cmtip_match_prepare_images; an, bn by
+ * cmtip_weighted_norms.  best[i] packs (float bits of max(d2,0)) << 32 | r and is
+ * reduced with atomicMin, so ties keep the smallest reference index (np.argmin,
+ * align.py:131).  Callers initialise best to all ones bits before the first batch;
+ * ref_offset is added to r (the batch start, align.py:116). */
+int cmtip_weighted_norms(const float* X, int64_t rows, int64_t P, const float* w2, float* out,
+                         void* stream);
+int cmtip_moments(const float* X, int64_t count, double* out2, void* stream);
+int cmtip_match_prepare_images(const float* B, int64_t n_img, int64_t P, const float* w2,
+                               float* Bw, uint16_t* Bw_bf16, void* stream);
+int cmtip_fill_u64(unsigned long long* dst, int64_t count, unsigned long long value, void* stream);
+
+/* FP32 path: cross term on MFMA (v_mfma_f32_16x16x4f32), argmin fused in the epilogue. */
+int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref_offset, const float* Bw,
+                           int64_t n_img, int64_t P, const float* an, const float* bn,
+                           const double* gamma, unsigned long long* best, void* stream);
+
+/* calc_eudist (align.py:27-36): D[r, i] = sqrt(max(an[r] - 2 <a_r, b_i> + bn[i], 0)) as
+ * float32 (n_a x n_b, row-major), with an, bn the plain squared row norms
+ * (cmtip_weighted_norms with w2 = 1).  Same MFMA kernel, full-matrix epilogue. */
+int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int64_t n_b, int64_t P,
+                     const float* an, const float* bn, float* D, void* stream);
+
+/* BF16 path: cross term in bfloat16 on MFMA with FP32 accumulation; keeps the k
+ * (1 <= k <= 8) smallest approximate distances per image in cand (n_img x k packed
+ * keys as above, initialised to all ones bits before the first batch), then
+ * cmtip_match_rescore recomputes each candidate's distance in FP32 from the float32
+ * slices (d2 = sum_p w_p^2 (gamma a - b)^2 accumulated in FP32) and writes the argmin
+ * to best.  gamma_of_ref maps a reference index to its batch gamma (n_ref_total
+ * doubles). */
+int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t ref_offset,
+                          const uint16_t* Bw, int64_t n_img, int64_t P, const float* an,
+                          const float* bn, const double* gamma, int k,
+                          unsigned long long* cand, void* stream);
+int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, const float* B, int64_t n_img,
+                        int64_t P, const float* w2, const double* gamma_of_ref, int k,
+                        const unsigned long long* cand, unsigned long long* best, void* stream);
+
+/* ---------------------------------------------------------------- autocorrelation solve
+ * The W x = d system of cmtip/autocorrelation/autocorrelation.py:113-173 and its scipy
+ * CG solve (:239).  A solver holds FFT plans for the M^3 grid and the L^3 Toeplitz
+ * embedding (L even, L >= 2M; the reference uses L = 2M, :141), the kernel spectrum,
+ * the support and the CG work vectors.  Vectors are float32 M^3; CG scalars are FP64. */
+typedef struct cmtip_acsolver_s* cmtip_acsolver;
+
+int cmtip_acsolver_create(int M, int L, cmtip_acsolver* out);
+int cmtip_acsolver_destroy(cmtip_acsolver s);
+
+/* support = (gaussian_filter(ac_estimate, 0.5) > 1e-12) as float32 (autocorrelation.py:218-219;
+ * scipy.ndimage defaults: mode 'reflect', truncate 4, i.e. a 5-tap kernel), FP64 math.
+ * ac_estimate: float64 M^3 on the device. */
+int cmtip_ac_support(const double* ac_estimate, int M, float* support, void* stream);
+/* As cmtip_ac_support; round_f32 = 1 rounds the filtered volume to float32 after each
+ * axis, as scipy does when ac_estimate is a float32 array. */
+int cmtip_ac_support_ex(const double* ac_estimate, int M, int round_f32, float* support, void* stream);
+
+/* Keep the support used by the operators (float32 M^3, device, caller-owned). */
+int cmtip_acsolver_set_support(cmtip_acsolver s, const float* support);
+
+/* F_ugrid_conv_ = fftn(ifftshift(ugrid_conv)) (autocorrelation.py:148) from the real
+ * type-1 of ones on the L^3 grid (float32 L^3).  The spectrum of a real kernel that is
+ * even modulo L is real; its real part is kept. */
+int cmtip_acsolver_set_kernel(cmtip_acsolver s, const float* ugrid_conv, void* stream);
+/* Install a spectrum given directly (real part, float32 L^3 in FFT position order, as
+ * the F_ugrid_conv_ argument of core_problem_convolution, autocorrelation.py:32). */
+int cmtip_acsolver_set_spectrum(cmtip_acsolver s, const float* F_real, void* stream);
+
+/* gen_F_antisupport (autocorrelation.py:66-85) as float32 0/1 in array order (M^3). */
+int cmtip_gen_F_antisupport(int M, float* out, void* stream);
+/* Install the antisupport used by fourier_reg (float32 M^3, array order; the solver
+ * keeps it ifftshifted, autocorrelation.py:105).  acsolver_create installs
+ * gen_F_antisupport(M) itself. */
+int cmtip_acsolver_set_antisupport(cmtip_acsolver s, const float* F_antisupport, void* stream);
+
+/* core_problem_convolution (autocorrelation.py:32-63): out = S (.) crop(ifft(F . fft(pad(S (.) u)))). */
+int cmtip_core_problem_convolution(cmtip_acsolver s, const float* u, float* out, void* stream);
+
+/* fourier_reg (autocorrelation.py:88-110) with the antisupport of gen_F_antisupport (:66-85). */
+int cmtip_fourier_reg(cmtip_acsolver s, const float* u, float* out, void* stream);
+
+/* W_matvec (autocorrelation.py:150-155): out = alpha*conv(u) + rho*u + phi*reg(u). */
+int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* out, double alpha, double rho,
+                          double phi, void* stream);
+
+/* scipy.sparse.linalg.cg (scipy/sparse/linalg/_isolve/iterative.py:305-422) on W with
+ * rtol, atol = 0, the given maxiter, x0 = x (overwritten by the solution).  Returns
+ * the number of completed iterations (the reference's callback count,
+ * autocorrelation.py:228-246) in *iters and scipy's info in *info.  Synchronises the
+ * stream once per iteration to read the FP64 residual norm. */
+int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
+                      double phi, int maxiter, double rtol, int* iters, int* info, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMTIP_AMD_H */

@@ -1,0 +1,123 @@
+"""Autocorrelation solve (autocorrelation.py:12-248) against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+from scipy.ndimage import gaussian_filter
+
+from cmtip_amd import autocorrelation as A
+from cmtip_amd import synth
+from oracle import ac as OAC
+from oracle import geom
+from oracle import nufft as O
+from tests.gpu_util import dev, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("M", [5, 41, 81])
+def test_antisupport_exact(M):
+    say(f"gen_F_antisupport M={M}")
+    assert np.array_equal(A.gen_F_antisupport(M), OAC.gen_F_antisupport(M))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_support_exact(dtype):
+    M = 21
+    ac = synth.gaussian_ac(M, 20, seed=2).astype(np.float64)
+    # hard zeros outside a ball so the filtered volume crosses the 1e-12 threshold
+    c = np.arange(M) - M // 2
+    r = np.sqrt(c[:, None, None] ** 2 + c[None, :, None] ** 2 + c[None, None, :] ** 2)
+    ac[r > 6] = 0.0
+    ac = ac.astype(dtype)
+    say(f"support dtype={dtype.__name__}")
+    got = A.autocorrelation._support_of(ac, M).cpu().numpy()
+    want = (gaussian_filter(ac, 0.5) > 1e-12).astype(np.float32)
+    assert np.array_equal(got, want)
+
+
+def _geometry(M=11, n_side=16, n_img=60, seed=0):
+    s, ext = synth.detector(n_side)
+    q = synth.random_quaternions(n_img, seed=seed)
+    acm = synth.gaussian_ac(M, 20, seed=seed + 1)
+    x, y, z = geom.rotated_coords(q, s, ext)
+    return s, ext, q, acm, x, y, z
+
+
+@pytest.mark.parametrize("M", [7, 11])
+def test_core_problem_convolution(M):
+    s, ext, q, acm, x, y, z = _geometry(M)
+    F = OAC.kernel_spectrum(x, y, z, M, eps=1e-12)
+    rng = np.random.default_rng(1)
+    S = (rng.random((M,) * 3) > 0.2).astype(np.float32)
+    u = rng.standard_normal(M ** 3).astype(np.float32)
+    say(f"core_problem_convolution M={M}")
+    got = A.core_problem_convolution(u, M, F, 2 * M, S)
+    want = OAC.core_problem_convolution(u.astype(np.float64), M, F, 2 * M, S)
+    assert rel_l2(got, want) < 1e-5
+
+
+@pytest.mark.parametrize("M", [7, 8, 11])
+def test_fourier_reg(M):
+    rng = np.random.default_rng(M)
+    S = (rng.random((M,) * 3) > 0.2).astype(np.float32)
+    u = rng.standard_normal(M ** 3).astype(np.float32)
+    anti = OAC.gen_F_antisupport(M)
+    say(f"fourier_reg M={M}")
+    got = A.fourier_reg(u, S, anti, M, True)
+    want = OAC.fourier_reg(u.astype(np.float64), S, anti, M)
+    assert rel_l2(got, want) < 1e-5
+
+
+def test_setup_linops_vs_oracle():
+    M = 11
+    s, ext, q, acm, x, y, z = _geometry(M)
+    data = O.nufft2(acm.astype(np.float64), x, y, z, eps=1e-12).real
+    N = data.size
+    sup = np.ones((M,) * 3, np.float32)
+    x0 = np.zeros(M ** 3)
+    rl = M ** 3 / N / 1000
+    say("setup_linops M=11")
+    W, d = A.setup_linops(torch.as_tensor(x), torch.as_tensor(y), torch.as_tensor(z), data.astype(np.float32), sup,
+                          np.ones(N, np.float32), x0, M, M ** 3, N, ext, 1, rl, 1e3, scaled=True)
+    d_want = OAC.rhs(x, y, z, data, M, sup, x0, 1, rl)
+    assert rel_l2(d, d_want) < 1e-5
+    F = OAC.kernel_spectrum(x, y, z, M)
+    anti = OAC.gen_F_antisupport(M)
+    u = np.random.default_rng(0).standard_normal(M ** 3)
+    want = OAC.core_problem_convolution(u, M, F, 2 * M, sup) + rl * u + 1e3 * OAC.fourier_reg(u, sup, anti, M)
+    assert rel_l2(W.matvec(u.astype(np.float32)), want) < 1e-5
+
+
+@pytest.mark.parametrize("with_estimate", [False, True])
+def test_solve_ac_vs_oracle(with_estimate):
+    M = 11
+    s, ext, q, acm, x, y, z = _geometry(M, n_img=80)
+    data = O.nufft2(acm.astype(np.float64), x, y, z, eps=1e-12).real.reshape(len(q), 1, -1).astype(np.float32)
+    est = acm.astype(np.float64) if with_estimate else None
+    say(f"solve_ac M=11 estimate={with_estimate}")
+    got, info = A.solve_ac(0, s, ext, data, M, orientations=q,
+                           ac_estimate=None if est is None else est.copy(), return_info=True)
+    want, its, sup, d, F = OAC.solve(q, s, ext, data, M, ac_estimate=est)
+    assert got.dtype == np.float64 and got.shape == (M, M, M)
+    # the GPU solution solves the oracle's system to the requested tolerance (FP32 slack)
+    anti = OAC.gen_F_antisupport(M)
+    rl = M ** 3 / data.size / 1000
+
+    def W(u):
+        return OAC.core_problem_convolution(u, M, F, 2 * M, sup) + rl * u + 1e3 * OAC.fourier_reg(u, sup, anti, M)
+    res = np.linalg.norm(W(got.ravel()) - d) / np.linalg.norm(d)
+    assert res < 5e-5
+    assert abs(info["iterations"] - its) <= max(2, its // 5)
+    assert rel_l2(got, want) < 1e-3
+
+
+def test_cg_iterates_track_oracle():
+    M = 11
+    s, ext, q, acm, x, y, z = _geometry(M, n_img=80)
+    data = O.nufft2(acm.astype(np.float64), x, y, z, eps=1e-12).real.reshape(len(q), 1, -1).astype(np.float32)
+    for k in (1, 3, 6):
+        say(f"solve_ac maxiter={k}")
+        got, info = A.solve_ac(0, s, ext, data, M, orientations=q, maxiter=k, return_info=True)
+        want, its, _, _, _ = OAC.solve(q, s, ext, data, M, maxiter=k)
+        assert info["iterations"] == its == k
+        assert rel_l2(got, want) < 1e-4

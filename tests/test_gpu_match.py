@@ -1,0 +1,110 @@
+"""Orientation matching (align.py:68-266) against the FP64 oracle: sklearn distances,
+per-batch std scaling, argmin.  Assignments must be identical for every image whose
+best-to-second-best squared-distance gap exceeds 1e-4 of the best (the band covers
+the 1e-5 FP32 slice error and the FP32 cross-term rounding)."""
+import numpy as np
+import pytest
+import torch
+
+from cmtip_amd import alignment, synth
+from oracle import align as OA
+from oracle import nufft as O
+from oracle import geom
+from tests.gpu_util import dev, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+BAND = 1e-4
+
+
+def _dataset(M=21, n_side=32, n_ref=300, n_img=64, photons=10.0, seed=0):
+    s, ext = synth.detector(n_side)
+    ac = synth.gaussian_ac(M, 40, seed=seed)
+    ref = synth.random_quaternions(n_ref, seed=seed + 1)
+    q_img = synth.random_quaternions(n_img, seed=seed + 2)
+    x, y, z = geom.rotated_coords(q_img, s, ext)
+    clean = O.nufft2(ac.astype(np.float64), x, y, z, eps=1e-12).real.reshape(n_img, -1)
+    imgs = synth.poisson_images(clean, photons, seed=seed + 3) if photons else clean.astype(np.float32)
+    return s, ext, ac, ref, q_img, imgs
+
+
+def test_calc_eudist_vs_sklearn():
+    from sklearn.metrics.pairwise import euclidean_distances
+    rng = np.random.default_rng(0)
+    A = rng.random((150, 777)).astype(np.float32)
+    B = rng.random((70, 777)).astype(np.float32)
+    say("calc_eudist 150x70x777")
+    D = alignment.calc_eudist(A, B)
+    want = euclidean_distances(A.astype(np.float64), B.astype(np.float64))
+    assert D.shape == (150, 70) and D.dtype == np.float32
+    assert np.abs(D - want).max() < 1e-5 * want.max()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("nbatches,order", [(1, 0), (3, 0), (2, -2)])
+def test_match_vs_oracle(precision, nbatches, order):
+    s, ext, ac, ref, _, imgs = _dataset()
+    say(f"match precision={precision} nbatches={nbatches} order={order}")
+    q, idx = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), nbatches=nbatches, order=order,
+                                         ref_orientations=ref, precision=precision, return_index=True)
+    want, D = OA.match(ref, s, ext, imgs, ac, nbatches=nbatches, order=order)
+    ok = OA.tie_band_ok(D, rel=BAND)
+    assert ok.mean() > 0.5, "too few images outside the tie band for a meaningful check"
+    assert np.array_equal(np.asarray(idx)[ok], want[ok])
+    np.testing.assert_array_equal(q, ref[np.asarray(idx)])
+    assert q.dtype == np.float32 and q.shape == (imgs.shape[0], 4)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_true_orientations_recovered(precision):
+    # align.py:109-111: the true quaternions are planted in the reference set
+    s, ext, ac, ref, q_img, imgs = _dataset(n_img=24, photons=0)
+    np.random.seed(7)
+    say(f"true-orientation recovery {precision}")
+    q = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), nbatches=2, ref_orientations=ref,
+                                     true_orientations=q_img, precision=precision)
+    np.testing.assert_array_equal(q, q_img)
+
+
+def test_match_torch_inputs_stay_on_device():
+    s, ext, ac, ref, _, imgs = _dataset(n_img=16)
+    d = dev()
+    say("match with device tensors")
+    q = alignment.match_orientations(0, s, ext, torch.from_numpy(imgs).to(d), torch.from_numpy(ac).to(d), len(ref),
+                                     ref_orientations=ref)
+    assert isinstance(q, torch.Tensor) and q.is_cuda
+
+
+def test_match_mult_assigns_conformations():
+    M, n_side = 21, 32
+    s, ext = synth.detector(n_side)
+    acs = [synth.gaussian_ac(M, 40, seed=10), synth.gaussian_ac(M, 40, seed=20)]
+    ref = synth.random_quaternions(200, seed=3)
+    q_img = synth.random_quaternions(20, seed=4)
+    x, y, z = geom.rotated_coords(q_img, s, ext)
+    conf = np.array([0] * 10 + [1] * 10)
+    imgs = np.stack([O.nufft2(acs[c].astype(np.float64), x[i * n_side ** 2:(i + 1) * n_side ** 2],
+                              y[i * n_side ** 2:(i + 1) * n_side ** 2], z[i * n_side ** 2:(i + 1) * n_side ** 2],
+                              eps=1e-12).real for i, c in enumerate(conf)]).astype(np.float32)
+    say("match_orientations_mult 2 volumes")
+    q, got_conf = alignment.match_orientations_mult(0, s, ext, imgs, acs, len(ref), ref_orientations=ref,
+                                                    true_orientations=q_img)
+    np.testing.assert_array_equal(got_conf, conf)
+    np.testing.assert_array_equal(q, q_img)
+
+
+def test_match_batch_wrapper():
+    s, ext, ac, ref, _, imgs = _dataset(n_img=50)
+    say("match_orientations_batch")
+    q = alignment.match_orientations_batch(0, s, ext, imgs, ac, len(ref), batch_size=20, ref_orientations=ref)
+    q1 = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), ref_orientations=ref)
+    assert q.shape == (50, 4) and q.dtype == np.float64
+    np.testing.assert_array_equal(q, q1.astype(np.float64))
+
+
+def test_generate_weights():
+    s, _ = synth.detector(16)
+    for order in (0, -2, 1):
+        w = alignment.generate_weights(s, order)
+        want = OA.generate_weights(s, order)
+        assert w.shape == (1, 256)
+        assert rel_l2(w, want) < 1e-6

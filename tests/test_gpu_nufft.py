@@ -1,0 +1,137 @@
+"""GPU type-2 / type-1 NUFFT against the FP64 oracle (exact sums), the reference's
+uniform-grid identity (cmtip/tests/test_nufft.py:40-96) and the adjoint identity.
+
+Tolerance: relative L2 <= 1e-5 at eps = 1e-6 (FP32 arithmetic, kernel width 7)."""
+import numpy as np
+import pytest
+import torch
+
+from cmtip_amd import nufft
+from oracle import nufft as O
+from tests.gpu_util import dev, points, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.mark.parametrize("M", [7, 8, 21])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_type2_vs_exact(M, cplx):
+    rng = np.random.default_rng(M + 100 * cplx)
+    N = 4000
+    f = rng.standard_normal((M, M, M)).astype(np.float32)
+    if cplx:
+        f = (f + 1j * rng.standard_normal((M, M, M))).astype(np.complex64)
+    x, y, z = points(rng, N)
+    say(f"type2 M={M} cplx={cplx} N={N}")
+    got = nufft.forward_gpu(f, x, y, z, None, use_recip_sym=not cplx)
+    want = O.nudft2(f.astype(np.complex128), x, y, z)
+    assert got.dtype == np.complex64 and got.shape == (N,)
+    assert rel_l2(got, want) < TOL
+    if not cplx:
+        say(f"type2 real-out M={M}")
+        re = nufft.forward_real(f, x, y, z)
+        assert re.dtype == np.float32
+        assert rel_l2(re, want.real) < TOL
+
+
+def test_forward_support_mutates_grid():
+    rng = np.random.default_rng(5)
+    M = 9
+    f = rng.standard_normal((M, M, M)).astype(np.float32)
+    sup = (rng.random((M, M, M)) > 0.5).astype(np.float32)
+    x, y, z = points(rng, 500)
+    f0 = f.copy()
+    say("type2 with support")
+    got = nufft.forward_cpu(f, x, y, z, sup, True)
+    np.testing.assert_array_equal(f, f0 * sup)  # in place, as forward.py:35
+    assert rel_l2(got, O.nudft2(f0 * sup, x, y, z)) < TOL
+
+
+@pytest.mark.parametrize("M", [7, 8, 21])
+@pytest.mark.parametrize("kind", ["complex", "real", "ones"])
+def test_type1_vs_exact(M, kind):
+    rng = np.random.default_rng(M + 7)
+    N = 4000
+    x, y, z = points(rng, N)
+    sup = (rng.random((M, M, M)) > 0.3).astype(np.float32)
+    if kind == "complex":
+        c = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    elif kind == "real":
+        c = rng.standard_normal(N).astype(np.float32)
+    else:
+        c = np.ones(N, np.float32)
+    want = O.nudft1(c.astype(np.complex128), x, y, z, M)
+    say(f"type1 M={M} kind={kind}")
+    if kind == "ones":
+        got = nufft.adjoint_ones(x, y, z, M)
+        assert rel_l2(got, want.real / M ** 3) < TOL
+        return
+    got = nufft.adjoint_gpu(c, x, y, z, M, use_recip_sym=True, support=sup)
+    assert got.dtype == np.float32 and got.shape == (M, M, M)
+    assert rel_l2(got, (want * sup).real / M ** 3) < TOL
+    got_c = nufft.adjoint_cpu(c, x, y, z, M, use_recip_sym=False)
+    assert got_c.dtype == np.complex64
+    assert rel_l2(got_c, want / M ** 3) < TOL
+
+
+def test_uniform_grid_identity():
+    # cmtip/tests/test_nufft.py:31-49: forward on the uniform grid equals the centred DFT
+    M = 21
+    rng = np.random.default_rng(0)
+    ls = np.linspace(-1, 1, M + 1)
+    ls = (ls[:-1] + ls[1:]) / 2
+    H, K, L = (np.pi * g.ravel().astype(np.float32) for g in np.meshgrid(ls, ls, ls, indexing="ij"))
+    ac = np.abs(rng.standard_normal((M, M, M))).astype(np.float32)
+    calc_np = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(ac)).real)
+    say("uniform grid forward/adjoint")
+    calc = nufft.forward_gpu(ac, H, K, L).real
+    assert np.mean(np.abs(calc_np.ravel() - calc)) < 1e-4 * np.abs(calc_np).mean()
+    ivol = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(ac))).real.astype(np.float32)
+    back = nufft.adjoint_gpu(ivol.ravel(), H, K, L, M)
+    want = np.real(np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(ivol))))
+    assert rel_l2(back, want) < TOL
+
+
+def test_adjoint_identity_large():
+    # size-independent property at a million points: <A f, c> = <f, A^H c>
+    M, N = 41, 1_000_000
+    rng = np.random.default_rng(11)
+    d = dev()
+    f = torch.randn((M, M, M), device=d)
+    c = torch.complex(torch.randn(N, device=d), torch.randn(N, device=d))
+    x, y, z = (torch.empty(N, device=d).uniform_(-np.pi, np.pi) for _ in range(3))
+    say("adjoint identity M=41 N=1e6")
+    Af = nufft.forward_gpu(f, x, y, z)
+    Ahc = nufft.adjoint_gpu(c, x, y, z, M, use_recip_sym=False) * M ** 3
+    lhs = torch.vdot(Af.to(torch.complex128), c.to(torch.complex128))
+    rhs = torch.vdot(f.to(torch.complex128).reshape(-1), Ahc.to(torch.complex128).reshape(-1))
+    assert float(abs(lhs - rhs) / abs(lhs)) < 1e-5
+
+
+@pytest.mark.parametrize("eps,tol", [(1e-4, 2e-4), (1e-5, 2e-5), (1e-6, 1e-5)])
+def test_eps_controls_error(eps, tol):
+    M, N = 16, 3000
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((M, M, M)).astype(np.float32)
+    x, y, z = points(rng, N)
+    say(f"type2 eps={eps}")
+    got = nufft.forward_gpu(f, x, y, z, eps=eps)
+    assert rel_l2(got, O.nudft2(f, x, y, z)) < tol
+
+
+def test_empty_points():
+    M = 8
+    f = np.ones((M, M, M), np.float32)
+    e = np.zeros(0, np.float32)
+    say("empty point set")
+    assert nufft.forward_gpu(f, e, e, e).shape == (0,)
+    assert np.all(nufft.adjoint_gpu(np.zeros(0, np.complex64), e, e, e, M) == 0)
+
+
+def test_out_of_range_asserts():
+    M = 8
+    f = np.ones((M, M, M), np.float32)
+    x = np.array([3 * np.pi + 0.1], np.float32)
+    with pytest.raises(AssertionError):
+        nufft.forward_gpu(f, x, x, x)

@@ -1,0 +1,59 @@
+"""Rotated pixel coordinates and fused slicing (compute_slices, align.py:39-65)
+against the oracle's float32 coordinates (bit-exact) and FP64 slices (rel L2 <= 1e-5)."""
+import numpy as np
+import pytest
+import torch
+
+from cmtip_amd import alignment, autocorrelation, synth
+from oracle import align as OA
+from oracle import geom
+from tests.gpu_util import dev, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rotate_coords_bit_exact():
+    s, ext = synth.detector(32)
+    q = synth.random_quaternions(200, seed=4)
+    say("rotate_coords 200 x 1024")
+    H, K, L = autocorrelation.gen_nonuniform_positions(q, s, recip_extent=ext)
+    x, y, z = geom.rotated_coords(q, s, ext)
+    for got, want in ((H, x), (K, y), (L, z)):
+        got = np.asarray(got).ravel()
+        ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+        assert ulp.max() <= 1 and (ulp > 0).mean() < 1e-3
+    Hu, Ku, Lu = autocorrelation.gen_nonuniform_positions(q, s)
+    Hw, Kw, Lw = geom.rotated_vectors(q, s)
+    assert np.asarray(Hu).shape == (200, 1, 1024)
+    assert np.abs(np.asarray(Hu) - Hw).max() <= np.abs(Hw).max() * 2 ** -23
+
+
+@pytest.mark.parametrize("M,n_side", [(15, 16), (21, 32)])
+def test_compute_slices_vs_oracle(M, n_side):
+    s, ext = synth.detector(n_side)
+    ac = synth.gaussian_ac(M, 30, seed=M)
+    q = synth.random_quaternions(40, seed=M + 1)
+    say(f"compute_slices M={M} P={n_side ** 2}")
+    got = alignment.compute_slices(q, s, ext, ac)
+    want = OA.compute_slices(q, s, ext, ac, exact=True)
+    assert got.shape == (40 * n_side * n_side,) and got.dtype == np.float32
+    assert rel_l2(got, want) < 1e-5
+
+
+def test_slice_bf16_and_moments():
+    M, n_side = 21, 32
+    s, ext = synth.detector(n_side)
+    ac = synth.gaussian_ac(M, 30, seed=1)
+    q = synth.random_quaternions(64, seed=2)
+    d = dev()
+    sl = alignment.Slicer(ac, s, ext)
+    out = torch.empty((64, n_side * n_side), device=d)
+    b16 = torch.empty((64, n_side * n_side), device=d, dtype=torch.int16)
+    mom = torch.zeros(2, dtype=torch.float64, device=d)
+    say("slice with bf16 + moments")
+    sl.slice(q, out=out, out_bf16=b16, moments=mom)
+    ref_b16 = out.to(torch.bfloat16).view(torch.int16)
+    assert torch.equal(b16, ref_b16)
+    o64 = out.double()
+    assert abs(float(mom[0]) - float(o64.sum())) <= 1e-9 * float(o64.abs().sum())
+    assert abs(float(mom[1]) - float((o64 * o64).sum())) <= 1e-9 * float((o64 * o64).sum())
