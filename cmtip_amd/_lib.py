@@ -43,6 +43,9 @@ SIGNATURES = {
     "cmtip_weighted_norms": [_vp, _i64, _i64, _vp, _vp, _vp],
     "cmtip_moments": [_vp, _i64, _vp, _vp],
     "cmtip_match_prepare_images": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
+    "cmtip_column_mean": [_vp, _i64, _i64, _vp, _vp],
+    "cmtip_center_rows": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "cmtip_match_prepare_images_centered": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_fill_u64": [_vp, _i64, _u64, _vp],
     "cmtip_match_argmin_f32": [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_eudist_f32": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp],

@@ -111,25 +111,34 @@ def reference_orientations(n_ref_orientations, seed=0):
 
 
 class _Matcher:
-    """State shared by one alignment call: images, weights and per-image best keys."""
+    """State shared by one alignment call: images, weights and per-image best keys.
+
+    Model slices and images are centred on the mean image c before the cross term
+    (d2 is unchanged; the FP32 / BF16 expanded form then does not cancel)."""
 
     def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk):
         self.P = pixel_position_reciprocal.shape[-1]
         B = _arrays.to_dev(slices_, torch.float32).reshape(-1, self.P)
         self.B = B
         self.n_img = B.shape[0]
+        L = _lib.lib()
         w = generate_weights(_arrays.to_dev(pixel_position_reciprocal, torch.float32), order=order)
         self.w2 = (w.reshape(-1) * w.reshape(-1)).contiguous()
         self.precision = precision
         self.k = int(topk)
         dev = B.device
+        self.c = torch.empty(self.P, dtype=torch.float32, device=dev)
+        check(L.cmtip_column_mean(ptr(B), self.n_img, self.P, ptr(self.c), stream()), "cmtip_column_mean")
         self.Bw = torch.empty_like(B) if precision == "fp32" else None
         self.Bw16 = torch.empty(B.shape, dtype=torch.int16, device=dev) if precision == "bf16" else None
-        check(_lib.lib().cmtip_match_prepare_images(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.Bw),
-                                                    ptr(self.Bw16), stream()), "cmtip_match_prepare_images")
-        self.bn = _norms(B, self.w2)
+        check(L.cmtip_match_prepare_images_centered(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.c),
+                                                    ptr(self.Bw), ptr(self.Bw16), stream()),
+              "cmtip_match_prepare_images_centered")
+        self.bn = torch.empty(self.n_img, dtype=torch.float32, device=dev)
+        check(L.cmtip_center_rows(ptr(B), self.n_img, self.P, None, ptr(self.c), ptr(self.w2), None, None,
+                                  ptr(self.bn), stream()), "cmtip_center_rows")
         mom = torch.zeros(2, dtype=torch.float64, device=dev)
-        check(_lib.lib().cmtip_moments(ptr(B), B.numel(), ptr(mom), stream()), "cmtip_moments")
+        check(L.cmtip_moments(ptr(B), B.numel(), ptr(mom), stream()), "cmtip_moments")
         self.std_data = _std_from_moments(mom, B.numel())
 
     def run(self, slicer, ref, nbatches, verbose=False):
@@ -139,6 +148,7 @@ class _Matcher:
         dev = self.B.device
         L = _lib.lib()
         best = torch.full((self.n_img,), EMPTY_KEY, dtype=torch.int64, device=dev)
+        an = torch.empty(quot, dtype=torch.float32, device=dev)
         if self.precision == "bf16":
             cand = torch.full((self.n_img, self.k), EMPTY_KEY, dtype=torch.int64, device=dev)
             A_all = torch.empty((n_ref, self.P), dtype=torch.float32, device=dev)
@@ -150,20 +160,23 @@ class _Matcher:
             start, end = nb * quot, (nb + 1) * quot
             mom = torch.zeros(2, dtype=torch.float64, device=dev)
             Ab = A_all[start:end] if self.precision == "bf16" else A
-            slicer.slice(ref[start:end], out=Ab, out_bf16=A16 if self.precision == "bf16" else None, moments=mom)
+            slicer.slice(ref[start:end], out=Ab, moments=mom)
             # data/model std ratio (align.py:122-124), rounded to float32 as numpy keeps it
             gamma = (self.std_data / _std_from_moments(mom, Ab.numel())).float().double().reshape(1)
             if verbose:
                 print(f"Data/Model std ratio: {float(gamma):.7g}.", flush=True)
-            an = _norms(Ab, self.w2)
             if self.precision == "bf16":
                 gamma_ref[start:end] = gamma
+                check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), None,
+                                          ptr(A16), ptr(an), stream()), "cmtip_center_rows")
                 check(L.cmtip_match_topk_bf16(ptr(A16), quot, start, ptr(self.Bw16), self.n_img, self.P, ptr(an),
-                                              ptr(self.bn), ptr(gamma), self.k, ptr(cand), stream()),
+                                              ptr(self.bn), None, self.k, ptr(cand), stream()),
                       "cmtip_match_topk_bf16")
             else:
+                check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), ptr(Ab),
+                                          None, ptr(an), stream()), "cmtip_center_rows")
                 check(L.cmtip_match_argmin_f32(ptr(Ab), quot, start, ptr(self.Bw), self.n_img, self.P, ptr(an),
-                                               ptr(self.bn), ptr(gamma), ptr(best), stream()),
+                                               ptr(self.bn), None, ptr(best), stream()),
                       "cmtip_match_argmin_f32")
         if self.precision == "bf16":
             check(L.cmtip_match_rescore(ptr(A_all), n_ref, ptr(self.B), self.n_img, self.P, ptr(self.w2),

@@ -82,13 +82,48 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 }
 
 __global__ void prepare_images_kernel(const float* __restrict__ B, int64_t total, int64_t P,
-                                      const float* __restrict__ w2, float* __restrict__ Bw,
-                                      uint16_t* __restrict__ Bw_bf16) {
+                                      const float* __restrict__ w2, const float* __restrict__ c,
+                                      float* __restrict__ Bw, uint16_t* __restrict__ Bw_bf16) {
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= total) return;
-    const float v = w2[i % P] * B[i];
+    const int64_t p = i % P;
+    const float v = w2[p] * (c ? B[i] - c[p] : B[i]);
     if (Bw) Bw[i] = v;
     if (Bw_bf16) Bw_bf16[i] = f32_to_bf16(v);
+}
+
+// c[p] = mean over rows of X[:, p] (FP64 accumulation)
+__global__ void column_mean_kernel(const float* __restrict__ X, int64_t rows, int64_t P, float* __restrict__ c) {
+    const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= P) return;
+    double s = 0.0;
+    for (int64_t r = 0; r < rows; ++r) s += (double)X[r * P + p];
+    c[p] = (float)(s / (double)(rows > 0 ? rows : 1));
+}
+
+// Per row: y = gamma x - c; norms[row] = sum w2 y^2 (FP64 accumulation); y is written
+// as float32 (Y, may alias X) and/or bfloat16 (Y16) when those are given.
+__global__ void center_rows_kernel(const float* X, int64_t P, const double* __restrict__ gamma,
+                                   const float* __restrict__ c, const float* __restrict__ w2, float* Y,
+                                   uint16_t* __restrict__ Y16, float* __restrict__ norms) {
+    __shared__ double red[kThreads / 64];
+    const float g = gamma ? (float)gamma[0] : 1.0f;
+    const int64_t base = (int64_t)blockIdx.x * P;
+    double s = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+        const float y = c ? fmaf(g, X[base + p], -c[p]) : g * X[base + p];
+        s += (double)w2[p] * (double)y * (double)y;
+        if (Y) Y[base + p] = y;
+        if (Y16) Y16[base + p] = f32_to_bf16(y);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+        norms[blockIdx.x] = (float)t;
+    }
 }
 
 __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t count, unsigned long long v) {
@@ -362,7 +397,7 @@ __global__ void __launch_bounds__(kThreads, 2)
 
     // d2 tile, transposed: D[col][row]
     float* D = reinterpret_cast<float*>(lds_raw);
-    const float g = (float)gamma[0];
+    const float g = gamma ? (float)gamma[0] : 1.0f;
     const float g2 = g * g, m2g = -2.0f * g;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -477,8 +512,38 @@ extern "C" int cmtip_moments(const float* X, int64_t count, double* out2, void* 
     return CMTIP_OK;
 }
 
+extern "C" int cmtip_column_mean(const float* X, int64_t rows, int64_t P, float* c, void* stream) {
+    if (rows < 0 || P < 0 || (P > 0 && (!c || (rows > 0 && !X)))) {
+        set_error("column_mean: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (P == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(column_mean_kernel, dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), X, rows, P, c);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_center_rows(const float* X, int64_t rows, int64_t P, const double* gamma, const float* c,
+                                 const float* w2, float* Y, uint16_t* Y16, float* norms, void* stream) {
+    if (rows < 0 || P < 1 || (rows > 0 && (!X || !w2 || !norms)) || rows > 0x7fffffff) {
+        set_error("center_rows: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (rows == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(center_rows_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P, gamma,
+                       c, w2, Y, Y16, norms);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
 extern "C" int cmtip_match_prepare_images(const float* B, int64_t n_img, int64_t P, const float* w2, float* Bw,
                                           uint16_t* Bw_bf16, void* stream) {
+    return cmtip_match_prepare_images_centered(B, n_img, P, w2, nullptr, Bw, Bw_bf16, stream);
+}
+
+extern "C" int cmtip_match_prepare_images_centered(const float* B, int64_t n_img, int64_t P, const float* w2,
+                                                   const float* c, float* Bw, uint16_t* Bw_bf16, void* stream) {
     if (n_img < 0 || P < 0 || (n_img * P > 0 && (!B || !w2 || (!Bw && !Bw_bf16)))) {
         set_error("match_prepare_images: bad arguments");
         return CMTIP_ERR_ARG;
@@ -486,7 +551,7 @@ extern "C" int cmtip_match_prepare_images(const float* B, int64_t n_img, int64_t
     const int64_t total = n_img * P;
     if (total == 0) return CMTIP_OK;
     hipLaunchKernelGGL(prepare_images_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads),
-                       0, as_stream(stream), B, total, P, w2, Bw, Bw_bf16);
+                       0, as_stream(stream), B, total, P, w2, c, Bw, Bw_bf16);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -504,10 +569,10 @@ extern "C" int cmtip_fill_u64(unsigned long long* dst, int64_t count, unsigned l
 }
 
 static int check_match_args(int64_t n_ref, int64_t ref_offset, int64_t n_img, int64_t P, const void* A,
-                            const void* B, const void* an, const void* bn, const void* gamma, const void* out,
-                            int64_t* tiles_m, int64_t* tiles) {
+                            const void* B, const void* an, const void* bn, const void* out, int64_t* tiles_m,
+                            int64_t* tiles) {
     if (n_ref < 0 || n_img < 0 || P < 1 || ref_offset < 0 || n_ref + ref_offset > 0xffffffffLL ||
-        (n_ref > 0 && n_img > 0 && (!A || !B || !an || !bn || !gamma || !out))) {
+        (n_ref > 0 && n_img > 0 && (!A || !B || !an || !bn || !out))) {
         set_error("match: bad arguments");
         return CMTIP_ERR_ARG;
     }
@@ -524,7 +589,7 @@ extern "C" int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref
                                       int64_t n_img, int64_t P, const float* an, const float* bn,
                                       const double* gamma, unsigned long long* best, void* stream) {
     int64_t tiles_m, tiles;
-    CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, gamma, best, &tiles_m, &tiles));
+    CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, best, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
     hipLaunchKernelGGL(match_argmin_f32_kernel<false>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
                        A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best, (float*)nullptr,
@@ -536,8 +601,7 @@ extern "C" int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref
 extern "C" int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int64_t n_b, int64_t P,
                                 const float* an, const float* bn, float* D, void* stream) {
     int64_t tiles_m, tiles;
-    const double one = 1.0;
-    CMTIP_TRY(check_match_args(n_a, 0, n_b, P, A, B, an, bn, &one, D, &tiles_m, &tiles));
+    CMTIP_TRY(check_match_args(n_a, 0, n_b, P, A, B, an, bn, D, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
     hipLaunchKernelGGL(match_argmin_f32_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
                        A, n_a, 0u, B, n_b, P, an, bn, (const double*)nullptr, (unsigned long long*)nullptr, D,
@@ -554,7 +618,7 @@ extern "C" int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t r
         return CMTIP_ERR_ARG;
     }
     int64_t tiles_m, tiles;
-    CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, gamma, cand, &tiles_m, &tiles));
+    CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, cand, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
     hipLaunchKernelGGL(match_topk_bf16_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream), A,
                        n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, (int)tiles_m);

@@ -66,7 +66,7 @@ __device__ __forceinline__ void kernel_row(float x, float hi, float lo, int n, f
     const float t = fine_coord(x, hi, lo, n);
     const float l0f = ceilf(t - 0.5f * (float)W);
     const int l0 = (int)l0f;
-    const float d = l0f - t;  // in (-W/2 - 1, -W/2]
+    const float d = l0f - t;  // in [-W/2, -W/2 + 1)
     int base = l0 % n;
     if (base < 0) base += n;
     const float tw = 2.0f / (float)W;

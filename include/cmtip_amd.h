@@ -118,13 +118,27 @@ int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient
  * Orientation matching (cmtip/alignment/align.py:113-131): for every image i, the
  * reference r minimising the weighted Euclidean distance
  *     d2[r,i] = sum_p w_p^2 (gamma a[r,p] - b[i,p])^2
- *             = gamma^2 an[r] - 2 gamma <a_r, b'_i> + bn[i]
  * (sklearn euclidean_distances, align.py:35, on model_slices*gamma*w and slices*w).
- * b' = w^2 (.) b is prepared by cmtip_match_prepare_images; an, bn by
- * cmtip_weighted_norms.  best[i] packs (float bits of max(d2,0)) << 32 | r and is
- * reduced with atomicMin, so ties keep the smallest reference index (np.argmin,
- * align.py:131).  Callers initialise best to all ones bits before the first batch;
- * ref_offset is added to r (the batch start, align.py:116). */
+ * Both operands are first centred on the same per-pixel vector c (the mean image,
+ * cmtip_column_mean), which leaves every d2 unchanged but removes the common
+ * intensity profile, so the expanded form
+ *     d2 = an[r] - 2 <a'_r, b'_i> + bn[i],  a' = gamma a - c,  b' = w^2 (.) (b - c),
+ *     an = sum w^2 (gamma a - c)^2,  bn = sum w^2 (b - c)^2
+ * does not cancel catastrophically in FP32 / BF16 when the images have a large
+ * dynamic range.  a' and an come from cmtip_center_rows, b' from
+ * cmtip_match_prepare_images_centered and bn from cmtip_center_rows with gamma = 1.
+ * best[i] packs (float bits of max(d2,0)) << 32 | r and is reduced with atomicMin,
+ * so ties keep the smallest reference index (np.argmin, align.py:131).  Callers
+ * initialise best to all ones bits before the first batch; ref_offset is added to r
+ * (the batch start, align.py:116).  gamma may be NULL (1). */
+int cmtip_column_mean(const float* X, int64_t rows, int64_t P, float* c, void* stream);
+/* y = gamma x - c per row (c may be NULL), norms = sum_p w2 y^2; y written to Y (float32,
+ * may alias X) and/or Y16 (bfloat16 bits) when given. */
+int cmtip_center_rows(const float* X, int64_t rows, int64_t P, const double* gamma, const float* c,
+                      const float* w2, float* Y, uint16_t* Y16, float* norms, void* stream);
+/* Bw = w2 (.) (B - c) as float32 and/or bfloat16 (c may be NULL). */
+int cmtip_match_prepare_images_centered(const float* B, int64_t n_img, int64_t P, const float* w2,
+                                        const float* c, float* Bw, uint16_t* Bw_bf16, void* stream);
 int cmtip_weighted_norms(const float* X, int64_t rows, int64_t P, const float* w2, float* out,
                          void* stream);
 int cmtip_moments(const float* X, int64_t count, double* out2, void* stream);

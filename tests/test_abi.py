@@ -44,3 +44,28 @@ def test_product_refuses_without_gpu():
     import pytest
     with pytest.raises(RuntimeError):
         _lib.lib()
+
+
+def test_argument_errors_need_no_gpu():
+    """Bad arguments are rejected with CMTIP_ERR_ARG before any device work, and the
+    message is kept for cmtip_last_error; valid-looking null-gamma calls pass the checks
+    (they stop at the first HIP call only where a GPU is missing)."""
+    import ctypes
+    from cmtip_amd import _lib
+    L = _lib.load()
+    ERR_ARG = 1
+    h = ctypes.c_void_p()
+    assert L.cmtip_nufft_makeplan(3, 10, -1, 1e-6, ctypes.byref(h)) == ERR_ARG
+    assert b"bad arguments" in L.cmtip_last_error()
+    assert L.cmtip_nufft_makeplan(2, 0, -1, 1e-6, ctypes.byref(h)) == ERR_ARG
+    assert L.cmtip_nufft_makeplan(2, 10, 0, 1e-6, ctypes.byref(h)) == ERR_ARG
+    assert L.cmtip_nufft_setpts(None, 1, None, None, None, None) == ERR_ARG
+    assert L.cmtip_acsolver_create(10, 15, ctypes.byref(h)) == ERR_ARG  # odd L
+    assert L.cmtip_acsolver_create(10, 18, ctypes.byref(h)) == ERR_ARG  # L < 2M
+    assert L.cmtip_match_topk_bf16(None, 10, 0, None, 10, 16, None, None, None, 9, None, None) == ERR_ARG  # k > 8
+    assert L.cmtip_match_argmin_f32(None, 10, 0, None, 10, 16, None, None, None, None, None) == ERR_ARG
+    assert L.cmtip_match_argmin_f32(None, 10, -1, None, 10, 16, None, None, None, None, None) == ERR_ARG
+    # zero-sized problems are no-ops that succeed without touching the device
+    assert L.cmtip_match_argmin_f32(None, 0, 0, None, 0, 16, None, None, None, None, None) == 0
+    assert L.cmtip_rotate_coords(None, 0, None, 0, 1.0, None, None, None, None) == 0
+    assert L.cmtip_center_rows(None, 0, 16, None, None, None, None, None, None, None) == 0

@@ -93,12 +93,71 @@ def test_match_mult_assigns_conformations():
 
 
 def test_match_batch_wrapper():
+    # each image batch is aligned on its own, with its own data std (align.py:249-265)
     s, ext, ac, ref, _, imgs = _dataset(n_img=50)
     say("match_orientations_batch")
     q = alignment.match_orientations_batch(0, s, ext, imgs, ac, len(ref), batch_size=20, ref_orientations=ref)
-    q1 = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), ref_orientations=ref)
     assert q.shape == (50, 4) and q.dtype == np.float64
-    np.testing.assert_array_equal(q, q1.astype(np.float64))
+    for start, end in ((0, 20), (20, 50)):
+        qb = alignment.match_orientations(0, s, ext, imgs[start:end], ac, len(ref), ref_orientations=ref)
+        np.testing.assert_array_equal(q[start:end], qb.astype(np.float64))
+
+
+def _kernel_case(n_ref, n_img, P, seed):
+    d = dev()
+    g = torch.Generator(device=d).manual_seed(seed)
+    A = torch.rand((n_ref, P), device=d, generator=g) * 50
+    B = torch.poisson(torch.rand((n_img, P), device=d, generator=g) * 20, generator=g)
+    w2 = torch.full((P,), 1.0 / P ** 2, device=d)
+    gamma = torch.tensor([0.37], dtype=torch.float64, device=d)
+    A64, B64, w64 = A.double(), B.double(), w2.double()
+    D = ((gamma ** 2) * (A64 * A64 * w64).sum(1)[:, None] - 2 * gamma * (A64 * w64) @ B64.T
+         + (B64 * B64 * w64).sum(1)[None, :])
+    return A, B, w2, gamma, D
+
+
+def _gap_ok(D, rel):
+    s = torch.sort(D, dim=0).values
+    return ((s[1] - s[0]) > rel * s[0].abs()).cpu().numpy()
+
+
+@pytest.mark.parametrize("n_ref,n_img,P", [(700, 300, 4104), (10000, 5000, 16384)])
+def test_match_kernels_vs_fp64(n_ref, n_img, P):
+    from cmtip_amd._lib import check, lib, ptr, stream
+    A, B, w2, gamma, D = _kernel_case(n_ref, n_img, P, 0)
+    want = torch.argmin(D, dim=0).cpu().numpy()
+    L = lib()
+    d = dev()
+    Bw = torch.empty_like(B)
+    Bw16 = torch.empty(B.shape, dtype=torch.int16, device=d)
+    check(L.cmtip_match_prepare_images(ptr(B), n_img, P, ptr(w2), ptr(Bw), ptr(Bw16), stream()))
+    an = torch.empty(n_ref, device=d)
+    bn = torch.empty(n_img, device=d)
+    check(L.cmtip_weighted_norms(ptr(A), n_ref, P, ptr(w2), ptr(an), stream()))
+    check(L.cmtip_weighted_norms(ptr(B), n_img, P, ptr(w2), ptr(bn), stream()))
+    say(f"fp32 argmin kernel {n_ref}x{n_img}x{P}")
+    best = torch.full((n_img,), -1, dtype=torch.int64, device=d)
+    check(L.cmtip_match_argmin_f32(ptr(A), n_ref, 0, ptr(Bw), n_img, P, ptr(an), ptr(bn), ptr(gamma), ptr(best),
+                                   stream()))
+    got = (best & 0xFFFFFFFF).cpu().numpy()
+    ok = _gap_ok(D, 1e-5)
+    assert ok.mean() > 0.5
+    assert np.array_equal(got[ok], want[ok]), f"fp32 mismatches: {(got[ok] != want[ok]).sum()} of {ok.sum()}"
+    say(f"bf16 top-8 kernel {n_ref}x{n_img}x{P}")
+    A16 = A.to(torch.bfloat16).view(torch.int16)
+    cand = torch.full((n_img, 8), -1, dtype=torch.int64, device=d)
+    check(L.cmtip_match_topk_bf16(ptr(A16), n_ref, 0, ptr(Bw16), n_img, P, ptr(an), ptr(bn), ptr(gamma), 8,
+                                  ptr(cand), stream()))
+    gref = gamma.expand(n_ref).contiguous()
+    best2 = torch.full((n_img,), -1, dtype=torch.int64, device=d)
+    check(L.cmtip_match_rescore(ptr(A), n_ref, ptr(B), n_img, P, ptr(w2), ptr(gref), 8, ptr(cand), ptr(best2),
+                                stream()))
+    got2 = (best2 & 0xFFFFFFFF).cpu().numpy()
+    ok2 = _gap_ok(D, 1e-4)
+    assert np.array_equal(got2[ok2], want[ok2]), f"bf16 mismatches: {(got2[ok2] != want[ok2]).sum()} of {ok2.sum()}"
+    # candidate lists are sorted and hold distinct references
+    c = cand.cpu().numpy().view(np.uint64)
+    assert np.all(np.diff(c.astype(np.float64), axis=1) >= 0)
 
 
 def test_generate_weights():
@@ -108,3 +167,32 @@ def test_generate_weights():
         want = OA.generate_weights(s, order)
         assert w.shape == (1, 256)
         assert rel_l2(w, want) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_realistic_dynamic_range_vs_fp64(precision):
+    # images with the benchmark's statistics (M=81 autocorrelation, 128x128 pixels,
+    # Poisson at 10 photons/pixel: central pixels far brighter than the edges)
+    d = dev()
+    M, side, n_ref, n_img = 81, 128, 2000, 256
+    s, ext = synth.detector(side)
+    ac = torch.from_numpy(synth.gaussian_ac(M, 1000, seed=0)).to(d)
+    ref = synth.random_quaternions(n_ref, seed=0)
+    q_img = synth.random_quaternions(n_img, seed=1)
+    sl = alignment.Slicer(ac, s, ext)
+    clean = sl.slice(q_img)
+    torch.manual_seed(2)
+    imgs = torch.poisson(clean.clamp(min=0) * (10.0 / float(clean.clamp(min=0).mean())))
+    say(f"realistic matching {precision}")
+    q, idx = alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref, precision=precision,
+                                         return_index=True)
+    A = sl.slice(ref).double()
+    B = imgs.double()
+    w2 = 1.0 / (side * side) ** 2
+    gamma = float(B.std(unbiased=False) / A.std(unbiased=False))
+    D = w2 * ((gamma * A) ** 2).sum(1)[:, None] - 2 * w2 * gamma * A @ B.T + w2 * (B ** 2).sum(1)[None, :]
+    want = torch.argmin(D, dim=0).cpu().numpy()
+    ok = _gap_ok(D, 1e-5)
+    got = idx.cpu().numpy()
+    assert ok.mean() > 0.9
+    assert np.array_equal(got[ok], want[ok]), f"{(got[ok] != want[ok]).sum()} mismatches of {ok.sum()}"
