@@ -39,11 +39,11 @@ SIGNATURES = {
     "cmtip_nufft_destroy": [_vp],
     "cmtip_rotate_coords": [_vp, _i64, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "cmtip_slice_prepare": [_vp, _vp, _vp, _vp],
-    "cmtip_slice_real": [_vp, _vp, _i64, _vp, _i64, _f, _vp, _vp, _vp, _vp],
+    "cmtip_slice_real": [_vp, _vp, _i64, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "cmtip_weighted_norms": [_vp, _i64, _i64, _vp, _vp, _vp],
     "cmtip_moments": [_vp, _i64, _vp, _vp],
     "cmtip_match_prepare_images": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
-    "cmtip_column_mean": [_vp, _i64, _i64, _vp, _vp],
+    "cmtip_image_stats": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_center_rows": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cmtip_match_prepare_images_centered": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_fill_u64": [_vp, _i64, _u64, _vp],

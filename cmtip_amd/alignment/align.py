@@ -67,20 +67,56 @@ def _ext32(reciprocal_extent):
     return float(np.float32(reciprocal_extent))
 
 
+def _spread_bits(v):
+    v = (v | (v << 8)) & 0x00FF00FF
+    v = (v | (v << 4)) & 0x0F0F0F0F
+    v = (v | (v << 2)) & 0x33333333
+    return (v | (v << 1)) & 0x55555555
+
+
+def pixel_order(s):
+    """Pixels in Morton order of their panel coordinates: runs of 64 consecutive
+    entries are compact 8 x 8 blocks of a square panel, so the slicing kernel can stage
+    their fine-grid footprint in LDS.  s: (3, P) device tensor of reciprocal vectors.
+
+    Pixels on the Ewald sphere |s + z/lambda| = 1/lambda have 1/lambda =
+    -|s|^2 / (2 s_z), and (s_x, s_y) / (s_z + 1/lambda) is the gnomonic (flat-panel)
+    position, linear in the pixel index of a planar detector; other geometries fall
+    back to (s_x, s_y)."""
+    P = s.shape[1]
+    side = max(1, int(np.ceil(np.sqrt(P))))
+    sd = s.double()
+    s2 = (sd * sd).sum(0)
+    sel = sd[2].abs() > 1e-12 * s2.max().sqrt().clamp(min=1e-300)
+    xy = sd[:2]
+    if bool(sel.any()):
+        k = float((-s2[sel] / (2 * sd[2][sel])).median())
+        den = sd[2] + k
+        if k > 0 and bool((den > 0).all()):
+            xy = sd[:2] / den
+    lo = xy.min(dim=1, keepdim=True).values
+    span = (xy.max(dim=1, keepdim=True).values - lo).clamp(min=1e-300)
+    q = torch.floor((xy - lo) / span * (side - 1) + 0.5).long().clamp(0, 0xFFFF)
+    code = _spread_bits(q[0]) | (_spread_bits(q[1]) << 1)
+    return torch.argsort(code, stable=True).to(torch.int32).contiguous()
+
+
 class Slicer:
     """Slices of one autocorrelation at many orientations (compute_slices, align.py:39-65).
 
     The fine grid is prepared once per autocorrelation; ``slice`` then writes
-    Re(T2(ac)) at the rotated pixels of a batch of quaternions.
+    Re(T2(ac)) at the rotated pixels of a batch of quaternions.  Pixels are processed
+    in ``pixel_order`` (the output layout does not change).
     """
 
-    def __init__(self, ac, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS):
+    def __init__(self, ac, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS, tile_pixels=True):
         self.ac = _arrays.to_dev(ac, torch.float32)
         M = self.ac.shape[0]
         self.plan = get_plan(2, M, -1, eps)
         self.s = _s64(pixel_position_reciprocal)
         self.P = self.s.shape[1]
         self.ext = _ext32(reciprocal_extent)
+        self.order = pixel_order(self.s) if tile_pixels else None
         self.plan.slice_prepare(self.ac)
 
     def slice(self, quats, out=None, out_bf16=None, moments=None):
@@ -88,7 +124,7 @@ class Slicer:
         if out is None:
             out = torch.empty((q.shape[0], self.P), dtype=torch.float32, device=q.device)
         assert out.is_contiguous() and out.numel() == q.shape[0] * self.P
-        self.plan.slice_real(q, self.s, self.P, self.ext, out, out_bf16, moments)
+        self.plan.slice_real(q, self.s, self.P, self.ext, out, out_bf16, moments, order=self.order)
         return out
 
 
@@ -128,7 +164,10 @@ class _Matcher:
         self.k = int(topk)
         dev = B.device
         self.c = torch.empty(self.P, dtype=torch.float32, device=dev)
-        check(L.cmtip_column_mean(ptr(B), self.n_img, self.P, ptr(self.c), stream()), "cmtip_column_mean")
+        colsum = torch.empty(2 * self.P, dtype=torch.float64, device=dev)
+        mom = torch.empty(2, dtype=torch.float64, device=dev)
+        check(L.cmtip_image_stats(ptr(B), self.n_img, self.P, ptr(colsum), ptr(self.c), ptr(mom), stream()),
+              "cmtip_image_stats")
         self.Bw = torch.empty_like(B) if precision == "fp32" else None
         self.Bw16 = torch.empty(B.shape, dtype=torch.int16, device=dev) if precision == "bf16" else None
         check(L.cmtip_match_prepare_images_centered(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.c),
@@ -137,8 +176,6 @@ class _Matcher:
         self.bn = torch.empty(self.n_img, dtype=torch.float32, device=dev)
         check(L.cmtip_center_rows(ptr(B), self.n_img, self.P, None, ptr(self.c), ptr(self.w2), None, None,
                                   ptr(self.bn), stream()), "cmtip_center_rows")
-        mom = torch.zeros(2, dtype=torch.float64, device=dev)
-        check(L.cmtip_moments(ptr(B), B.numel(), ptr(mom), stream()), "cmtip_moments")
         self.std_data = _std_from_moments(mom, B.numel())
 
     def run(self, slicer, ref, nbatches, verbose=False):

@@ -92,13 +92,72 @@ __global__ void prepare_images_kernel(const float* __restrict__ B, int64_t total
     if (Bw_bf16) Bw_bf16[i] = f32_to_bf16(v);
 }
 
-// c[p] = mean over rows of X[:, p] (FP64 accumulation)
-__global__ void column_mean_kernel(const float* __restrict__ X, int64_t rows, int64_t P, float* __restrict__ c) {
-    const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (p >= P) return;
-    double s = 0.0;
-    for (int64_t r = 0; r < rows; ++r) s += (double)X[r * P + p];
-    c[p] = (float)(s / (double)(rows > 0 ? rows : 1));
+// Column sums of X (rows x P) in FP64: block (pixel group of 64, kStatWaves waves)
+// splits the rows over its waves; the per-wave partials are combined in LDS in a
+// fixed order, so the result does not depend on scheduling.  colsum[p] = sum_r X[r,p],
+// colsum[P + p] = sum_r X[r,p]^2.
+constexpr int kStatWaves = 16;
+
+__global__ void __launch_bounds__(64 * kStatWaves) column_sums_kernel(const float* __restrict__ X, int64_t rows,
+                                                                      int64_t P, double* __restrict__ colsum) {
+    __shared__ double red[2][kStatWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+    double s1 = 0.0, s2 = 0.0;
+    if (p < P) {
+        int64_t r = wave;
+        for (; r + 3 * kStatWaves < rows; r += 4 * kStatWaves) {
+            const float a = X[r * P + p], b = X[(r + kStatWaves) * P + p];
+            const float c = X[(r + 2 * kStatWaves) * P + p], d = X[(r + 3 * kStatWaves) * P + p];
+            s1 += ((double)a + (double)b) + ((double)c + (double)d);
+            s2 += ((double)a * a + (double)b * b) + ((double)c * c + (double)d * d);
+        }
+        for (; r < rows; r += kStatWaves) {
+            const double v = (double)X[r * P + p];
+            s1 += v;
+            s2 += v * v;
+        }
+    }
+    red[0][wave][lane] = s1;
+    red[1][wave][lane] = s2;
+    __syncthreads();
+    if (wave == 0 && p < P) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int w = 0; w < kStatWaves; ++w) {
+            t1 += red[0][w][lane];
+            t2 += red[1][w][lane];
+        }
+        colsum[p] = t1;
+        colsum[P + p] = t2;
+    }
+}
+
+// c = colsum / rows (float32) and mom = (sum of all, sum of squares) in a fixed order
+__global__ void finish_stats_kernel(const double* __restrict__ colsum, int64_t rows, int64_t P,
+                                    float* __restrict__ c, double* __restrict__ mom) {
+    __shared__ double red[2][kThreads];
+    double s1 = 0.0, s2 = 0.0;
+    const double inv = 1.0 / (double)(rows > 0 ? rows : 1);
+    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+        const double a = colsum[p];
+        s1 += a;
+        s2 += colsum[P + p];
+        if (c) c[p] = (float)(a * inv);
+    }
+    red[0][threadIdx.x] = s1;
+    red[1][threadIdx.x] = s2;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && mom) {
+        mom[0] = red[0][0];
+        mom[1] = red[1][0];
+    }
 }
 
 // Per row: y = gamma x - c; norms[row] = sum w2 y^2 (FP64 accumulation); y is written
@@ -512,14 +571,16 @@ extern "C" int cmtip_moments(const float* X, int64_t count, double* out2, void* 
     return CMTIP_OK;
 }
 
-extern "C" int cmtip_column_mean(const float* X, int64_t rows, int64_t P, float* c, void* stream) {
-    if (rows < 0 || P < 0 || (P > 0 && (!c || (rows > 0 && !X)))) {
-        set_error("column_mean: bad arguments");
+extern "C" int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double* colsum, float* c, double* mom,
+                                 void* stream) {
+    if (rows < 0 || P < 1 || !colsum || (rows > 0 && !X)) {
+        set_error("image_stats: bad arguments");
         return CMTIP_ERR_ARG;
     }
-    if (P == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(column_mean_kernel, dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                       as_stream(stream), X, rows, P, c);
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64 * kStatWaves), 0, st, X, rows, P,
+                       colsum);
+    hipLaunchKernelGGL(finish_stats_kernel, dim3(1), dim3(kThreads), 0, st, (const double*)colsum, rows, P, c, mom);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

@@ -25,12 +25,14 @@ struct cmtip_nufft_plan_s {
     int type = 0, M = 0, iflag = -1, w = 0, n = 0;
     double beta = 0.0, eps = 0.0;
     float tc_hi = 0.f, tc_lo = 0.f;  // n / (2 pi) split in two floats
+    float beta_l2e = 0.f;            // beta * log2(e), the kernels' exponent scale
     Fft1D fft;
     DevBuf deconv;   // n floats
     DevBuf fine;     // n^3 float2 (used as n^3 float when real)
     DevBuf tmpA;     // M * n * n float2
     DevBuf tmpB;     // M * M * n float2
     DevBuf rinv;     // 9 doubles per orientation
+    DevBuf part;     // kMomentSlots x 2 doubles: striped partial moments
     int slice_ready = 0;
     int64_t npts = 0;
     const float* x = nullptr;
@@ -41,13 +43,18 @@ struct cmtip_nufft_plan_s {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMomentSlots = 1024;  // striped accumulators for the slice moments
 
 // ---------------------------------------------------------------- device helpers
 
-__device__ __forceinline__ float es_phi(float z, float two_over_w, float beta) {
+// phi(z) = exp(beta (sqrt(1 - (2z/w)^2) - 1)) with the hardware square root and
+// base-2 exponential (v_sqrt_f32, v_exp_f32; about 1 ulp each).  beta_l2e = beta log2(e).
+__device__ __forceinline__ float es_phi(float z, float two_over_w, float beta_l2e) {
     const float u = z * two_over_w;
-    const float s = 1.0f - u * u;
-    return s > 0.0f ? expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+    const float s = fmaf(-u, u, 1.0f);
+    const float r = __builtin_amdgcn_sqrtf(fmaxf(s, 0.0f));
+    const float e = __builtin_amdgcn_exp2f(beta_l2e * (r - 1.0f));
+    return s > 0.0f ? e : 0.0f;
 }
 
 // Fine-grid coordinate t = x n / (2 pi) with the constant split in two floats, so the
@@ -183,6 +190,28 @@ __global__ void __launch_bounds__(kBlock) spread_kernel(void* __restrict__ grid,
     }
 }
 
+// moments[0..1] += sum of the kMomentSlots striped partials; one wave
+__global__ void finish_moments_kernel(const double* __restrict__ part, double* __restrict__ moments) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < kMomentSlots; i += 64) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        moments[0] += a;
+        moments[1] += b;
+    }
+}
+
+__global__ void fill_f64_kernel(double* __restrict__ p, int64_t count, double v) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
 __global__ void fill_f32_kernel(float* __restrict__ p, int64_t count, float v) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -266,55 +295,185 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
-// Fused compute_slices for a real fine grid: coordinates from (rinv, s), real
-// interpolation, float32 (+ optional bfloat16) store and FP64 moments.
+// ---------------------------------------------------------------- fused slicing
+//
+// Each wave takes one tile of 64 pixels of one orientation.  With the pixels in a
+// spatially compact order (8 x 8 detector blocks, see cmtip_slice_real's order
+// argument), the tile's rotated points occupy a small box of the fine grid: the wave
+// stages that box (its lanes' kernel footprints, at most kBoxFloats floats) in LDS
+// with coalesced loads along z, then every lane interpolates from LDS.  Tiles whose
+// box does not fit (rare: very tilted, strongly curved or scattered pixel sets) read
+// the fine grid directly.  Workgroups are renumbered so that consecutive tiles of an
+// orientation run on the same XCD and share its L2.
+
+constexpr int kBoxFloats = 3328;  // per wave: 52 KB per 256-thread workgroup, three per CU
+constexpr int kBoxGroups = 6;   // (y, z) pairs of an x plane: at most 64 * kBoxGroups
+constexpr int kBoxPlanes = 4;   // x planes loaded per step (loads in flight per lane)
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Workgroup b runs on XCD b % 8; give each XCD a contiguous run of logical blocks.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t x = b % 8u, q = nb / 8u, r = nb % 8u;
+    return x * q + (x < r ? x : r) + b / 8u;
+}
+
+template <int W>
+__device__ __forceinline__ void row_weights(float t, int l0, float beta, float* wt) {
+    const float d = (float)l0 - t;
+    const float tw = 2.0f / (float)W;
+#pragma unroll
+    for (int a = 0; a < W; ++a) wt[a] = es_phi(d + (float)a, tw, beta);
+}
+
+__device__ __forceinline__ int first_node(float t, int W) { return (int)ceilf(t - 0.5f * (float)W); }
+
+__device__ __forceinline__ int wrap_n(int l, int n) {
+    int b = l % n;
+    return b < 0 ? b + n : b;
+}
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restrict__ g, int n, float beta,
                                                             float hi, float lo,
                                                             const double* __restrict__ rinv,
-                                                            const double* __restrict__ s, int64_t P,
-                                                            int64_t total, float ext,
-                                                            float* __restrict__ out,
+                                                            const double* __restrict__ s,
+                                                            const int* __restrict__ order, int64_t P,
+                                                            int64_t tiles_per_orient, int64_t total_tiles,
+                                                            float ext, float* __restrict__ out,
                                                             uint16_t* __restrict__ out_bf16,
                                                             double* __restrict__ moments) {
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    float val = 0.f;
-    if (j < total) {
-        const int64_t o = j / P, p = j - o * P;
-        const double* r = rinv + 9 * o;
-        const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
-        int ix[W], iy[W], iz[W];
-        float wx[W], wy[W], wz[W];
-        kernel_row<W>(rot_coord(r, s0, s1, s2, ext), hi, lo, n, beta, ix, wx);
-        kernel_row<W>(rot_coord(r + 3, s0, s1, s2, ext), hi, lo, n, beta, iy, wy);
-        kernel_row<W>(rot_coord(r + 6, s0, s1, s2, ext), hi, lo, n, beta, iz, wz);
-        float acc = 0.f;
+    __shared__ float box_all[kBlock / 64][kBoxFloats];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* box = box_all[wave];
+    const int64_t tile = (int64_t)xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave;
+    const bool tile_ok = tile < total_tiles;
+    const int64_t o = tile_ok ? tile / tiles_per_orient : 0;
+    const int64_t slot = (tile_ok ? tile - o * tiles_per_orient : 0) * 64 + lane;
+    const bool active = tile_ok && slot < P;
+    const int64_t p = order ? (int64_t)order[active ? slot : 0] : (active ? slot : 0);
+    const double* r = rinv + 9 * o;
+    const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
+    const float tx = fine_coord(rot_coord(r, s0, s1, s2, ext), hi, lo, n);
+    const float ty = fine_coord(rot_coord(r + 3, s0, s1, s2, ext), hi, lo, n);
+    const float tz = fine_coord(rot_coord(r + 6, s0, s1, s2, ext), hi, lo, n);
+    const int lx = first_node(tx, W), ly = first_node(ty, W), lz = first_node(tz, W);
+    // box of the active lanes' footprints (wave-uniform values held in scalar registers)
+    const int big = 1 << 29;
+    const int mnx = __builtin_amdgcn_readfirstlane(wave_min_i(active ? lx : big));
+    const int mny = __builtin_amdgcn_readfirstlane(wave_min_i(active ? ly : big));
+    const int mnz = __builtin_amdgcn_readfirstlane(wave_min_i(active ? lz : big));
+    const int ex = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lx : -big) + W - mnx);
+    const int ey = __builtin_amdgcn_readfirstlane(wave_max_i(active ? ly : -big) + W - mny);
+    const int ez = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lz : -big) + W - mnz);
+    const bool any = mnx < big;
+    const bool use_lds = any && ex <= n && ey <= n && ez <= n && ey * ez <= 64 * kBoxGroups &&
+                         ex * ey * ez <= kBoxFloats;
+    if (use_lds) {
+        // lane q of group gi owns the (y, z) pair q = gi * 64 + lane of the box's x planes:
+        // its fine-grid and LDS offsets are fixed, so each x plane costs one scalar offset
+        const int bx = wrap_n(mnx, n), by = wrap_n(mny, n), bz = wrap_n(mnz, n);
+        const int eyz = ey * ez;
+        const float inv_z = 1.0f / (float)ez;
+        int goff[kBoxGroups];
 #pragma unroll
-        for (int a = 0; a < W; ++a) {
-            float pa = 0.f;
-            for (int b = 0; b < W; ++b) {
-                const float* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
-                float q = 0.f;
+        for (int gi = 0; gi < kBoxGroups; ++gi) {
+            const int q = gi * 64 + lane;
+            const int j = (int)(((float)q + 0.5f) * inv_z);
+            const int k = q - j * ez;
+            int gy = by + j, gz = bz + k;
+            gy = gy >= n ? gy - n : gy;
+            gz = gz >= n ? gz - n : gz;
+            goff[gi] = q < eyz ? gy * n + gz : -1;
+        }
+        const int nn = n * n;
+        for (int i0 = 0; i0 < ex; i0 += kBoxPlanes) {
+            float v[kBoxPlanes][kBoxGroups];
 #pragma unroll
-                for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
-                pa = fmaf(wy[b], q, pa);
+            for (int di = 0; di < kBoxPlanes; ++di) {
+                int gx = bx + i0 + di;
+                gx = gx >= n ? gx - n : gx;
+                const float* plane = g + (uint32_t)(gx * nn);
+#pragma unroll
+                for (int gi = 0; gi < kBoxGroups; ++gi)
+                    v[di][gi] = (i0 + di < ex && goff[gi] >= 0) ? plane[goff[gi]] : 0.f;
             }
-            acc = fmaf(wx[a], pa, acc);
+#pragma unroll
+            for (int di = 0; di < kBoxPlanes; ++di)
+#pragma unroll
+                for (int gi = 0; gi < kBoxGroups; ++gi)
+                    if (i0 + di < ex && goff[gi] >= 0) box[(i0 + di) * eyz + gi * 64 + lane] = v[di][gi];
+        }
+    }
+    __syncthreads();
+    float val = 0.f;
+    if (active) {
+        float wx[W], wy[W], wz[W];
+        row_weights<W>(tx, lx, beta, wx);
+        row_weights<W>(ty, ly, beta, wy);
+        row_weights<W>(tz, lz, beta, wz);
+        float acc = 0.f;
+        if (use_lds) {
+            const int eyz = ey * ez;
+            const float* base = box + ((lx - mnx) * ey + (ly - mny)) * ez + (lz - mnz);
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                float pa = 0.f;
+#pragma unroll
+                for (int b = 0; b < W; ++b) {
+                    const float* row = base + a * eyz + b * ez;
+                    float q = 0.f;
+#pragma unroll
+                    for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[c], q);
+                    pa = fmaf(wy[b], q, pa);
+                }
+                acc = fmaf(wx[a], pa, acc);
+            }
+        } else {
+            int ix[W], iy[W], iz[W];
+            const int bx = wrap_n(lx, n), by = wrap_n(ly, n), bz = wrap_n(lz, n);
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                ix[a] = bx + a >= n ? bx + a - n : bx + a;
+                iy[a] = by + a >= n ? by + a - n : by + a;
+                iz[a] = bz + a >= n ? bz + a - n : bz + a;
+            }
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                float pa = 0.f;
+                for (int b = 0; b < W; ++b) {
+                    const float* row = g + (uint32_t)((ix[a] * n + iy[b]) * n);
+                    float q = 0.f;
+#pragma unroll
+                    for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
+                    pa = fmaf(wy[b], q, pa);
+                }
+                acc = fmaf(wx[a], pa, acc);
+            }
         }
         val = acc;
+        const int64_t j = o * P + p;
         out[j] = val;
         if (out_bf16) out_bf16[j] = f32_to_bf16(val);
     }
     if (moments) {
         double s1v = (double)val, s2v = (double)val * (double)val;
-        // wavefront reduction, then one atomic per wavefront
         for (int off = 32; off > 0; off >>= 1) {
             s1v += __shfl_down(s1v, off, 64);
             s2v += __shfl_down(s2v, off, 64);
         }
-        if ((threadIdx.x & 63) == 0) {
-            unsafeAtomicAdd(moments, s1v);
-            unsafeAtomicAdd(moments + 1, s2v);
+        if (lane == 0 && tile_ok) {
+            double* slot = moments + 2 * (blockIdx.x % kMomentSlots);
+            unsafeAtomicAdd(slot, s1v);
+            unsafeAtomicAdd(slot + 1, s2v);
         }
     }
 }
@@ -343,11 +502,11 @@ struct InterpLaunch {
         if (p->npts == 0) return CMTIP_OK;
         if (cplx)
             hipLaunchKernelGGL((interp_kernel<W, true>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
                                p->z, out_c, out_r);
         else
             hipLaunchKernelGGL((interp_kernel<W, false>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
                                p->z, out_c, out_r);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
@@ -360,11 +519,11 @@ struct SpreadLaunch {
         if (p->npts == 0) return CMTIP_OK;
         if (cplx)
             hipLaunchKernelGGL((spread_kernel<W, true>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
                                p->z, c);
         else
             hipLaunchKernelGGL((spread_kernel<W, false>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, (float)p->beta, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
+                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
                                p->z, c);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
@@ -373,11 +532,18 @@ struct SpreadLaunch {
 
 template <int W>
 struct SliceLaunch {
-    static int run(const cmtip_nufft_plan_s* p, const double* s, int64_t P, int64_t total, float ext,
-                   float* out, uint16_t* out_bf16, double* moments, hipStream_t st) {
-        hipLaunchKernelGGL((slice_real_kernel<W>), dim3(blocks_for(total)), dim3(kBlock), 0, st,
-                           p->fine.as<const float>(), p->n, (float)p->beta, p->tc_hi, p->tc_lo,
-                           p->rinv.as<const double>(), s, P, total, ext, out, out_bf16, moments);
+    static int run(const cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient,
+                   float ext, float* out, uint16_t* out_bf16, double* moments, hipStream_t st) {
+        const int64_t tpo = (P + 63) / 64;
+        const int64_t tiles = tpo * n_orient;
+        const int64_t blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
+        if (blocks > 0x7fffffffLL) {
+            set_error("slice_real: too many orientations for one launch");
+            return CMTIP_ERR_ARG;
+        }
+        hipLaunchKernelGGL((slice_real_kernel<W>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                           p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo,
+                           p->rinv.as<const double>(), s, order, P, tpo, tiles, ext, out, out_bf16, moments);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -581,7 +747,14 @@ extern "C" int cmtip_nufft_makeplan(int type, int n_modes, int iflag, double eps
     EsParams es = es_params(eps);
     p->w = es.w;
     p->beta = es.beta;
+    p->beta_l2e = (float)(es.beta / M_LN2);
     p->n = fine_size(n_modes, es.w);
+    if ((int64_t)p->n * p->n * p->n >= ((int64_t)1 << 31)) {
+        // kernels index the fine grid with 32-bit offsets
+        set_error("makeplan: fine grid %d^3 too large (n_modes=%d)", p->n, n_modes);
+        delete p;
+        return CMTIP_ERR_ARG;
+    }
     const double tc = (double)p->n / (2.0 * M_PI);
     p->tc_hi = (float)tc;
     p->tc_lo = (float)(tc - (double)p->tc_hi);
@@ -668,6 +841,7 @@ extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     p->tmpA.release();
     p->tmpB.release();
     p->rinv.release();
+    p->part.release();
     delete p;
     return CMTIP_OK;
 }
@@ -713,7 +887,7 @@ extern "C" int cmtip_slice_prepare(cmtip_nufft_plan p, const float* ac, const fl
 }
 
 extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t n_orient, const double* s,
-                                int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
+                                const int* order, int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
                                 double* moments, void* stream) {
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
     if (!p->slice_ready) {
@@ -727,6 +901,19 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
     if (n_orient == 0 || P == 0) return CMTIP_OK;
     hipStream_t st = as_stream(stream);
     CMTIP_TRY(ensure_rinv(p, quats, n_orient, st));
-    return dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, P, n_orient * P, recip_extent,
-                                   out, out_bf16, moments, st);
+    double* part = nullptr;
+    if (moments) {
+        CMTIP_TRY(p->part.ensure(sizeof(double) * 2 * kMomentSlots));
+        part = p->part.as<double>();
+        hipLaunchKernelGGL(fill_f64_kernel, dim3(2 * kMomentSlots / kBlock), dim3(kBlock), 0, st, part,
+                           (int64_t)2 * kMomentSlots, 0.0);
+        CMTIP_LAUNCH_CHECK();
+    }
+    CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P, n_orient, recip_extent, out,
+                                      out_bf16, part, st));
+    if (moments) {
+        hipLaunchKernelGGL(finish_moments_kernel, dim3(1), dim3(64), 0, st, (const double*)part, moments);
+        CMTIP_LAUNCH_CHECK();
+    }
+    return CMTIP_OK;
 }

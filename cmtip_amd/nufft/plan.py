@@ -47,10 +47,11 @@ class NufftPlan:
     def slice_prepare(self, ac, support=None):
         check(_lib.lib().cmtip_slice_prepare(self._h, ptr(ac), ptr(support), stream()), "cmtip_slice_prepare")
 
-    def slice_real(self, quats, s, P, recip_extent, out, out_bf16=None, moments=None):
+    def slice_real(self, quats, s, P, recip_extent, out, out_bf16=None, moments=None, order=None):
         n_orient = quats.shape[0]
-        check(_lib.lib().cmtip_slice_real(self._h, ptr(quats), n_orient, ptr(s), int(P), float(recip_extent),
-                                          ptr(out), ptr(out_bf16), ptr(moments), stream()), "cmtip_slice_real")
+        check(_lib.lib().cmtip_slice_real(self._h, ptr(quats), n_orient, ptr(s), ptr(order), int(P),
+                                          float(recip_extent), ptr(out), ptr(out_bf16), ptr(moments), stream()),
+              "cmtip_slice_real")
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -105,13 +105,17 @@ int cmtip_rotate_coords(const float* quats, int64_t n_orient, const double* s, i
  * is exactly what Re(c) needs for a real ac).  cmtip_slice_real then evaluates
  * Re(T2(ac)) at the rotated pixel coordinates, generating them in-kernel from
  * (quats, s, recip_extent) exactly as cmtip_rotate_coords does, and writes
- * out[o*P + p] (float32).  out_bf16 (optional, uint16 bfloat16 bits, round to
+ * out[o*P + p] (float32).  order (optional, P int32, a permutation of the pixels)
+ * is the order in which pixels are processed, 64 at a time: pixels that are close on
+ * the detector should be close in it (8 x 8 blocks of a square panel), which lets each
+ * group of 64 stage its fine-grid footprint in LDS.  NULL processes pixels in index
+ * order.  The result does not depend on it.  out_bf16 (optional, uint16 bfloat16 bits, round to
  * nearest even) receives the same values.  moments (optional, 2 doubles, accumulated,
  * not cleared) receives sum(out) and sum(out^2) for the reference's per-batch std
  * scaling (align.py:122-124). */
 int cmtip_slice_prepare(cmtip_nufft_plan plan, const float* ac, const float* support, void* stream);
 int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient, const double* s,
-                     int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
+                     const int* order, int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
                      double* moments, void* stream);
 
 /* ---------------------------------------------------------------- matching
@@ -120,7 +124,7 @@ int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient
  *     d2[r,i] = sum_p w_p^2 (gamma a[r,p] - b[i,p])^2
  * (sklearn euclidean_distances, align.py:35, on model_slices*gamma*w and slices*w).
  * Both operands are first centred on the same per-pixel vector c (the mean image,
- * cmtip_column_mean), which leaves every d2 unchanged but removes the common
+ * cmtip_image_stats), which leaves every d2 unchanged but removes the common
  * intensity profile, so the expanded form
  *     d2 = an[r] - 2 <a'_r, b'_i> + bn[i],  a' = gamma a - c,  b' = w^2 (.) (b - c),
  *     an = sum w^2 (gamma a - c)^2,  bn = sum w^2 (b - c)^2
@@ -131,7 +135,12 @@ int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient
  * so ties keep the smallest reference index (np.argmin, align.py:131).  Callers
  * initialise best to all ones bits before the first batch; ref_offset is added to r
  * (the batch start, align.py:116).  gamma may be NULL (1). */
-int cmtip_column_mean(const float* X, int64_t rows, int64_t P, float* c, void* stream);
+/* Per-pixel sums of the images (colsum, 2P doubles of workspace: sums then sums of
+ * squares), their mean image c (float32, may be NULL) and mom = (sum, sum of squares)
+ * over all entries (2 doubles, may be NULL; for slices_.std(), align.py:122), all in
+ * FP64 with a fixed summation order. */
+int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double* colsum, float* c, double* mom,
+                      void* stream);
 /* y = gamma x - c per row (c may be NULL), norms = sum_p w2 y^2; y written to Y (float32,
  * may alias X) and/or Y16 (bfloat16 bits) when given. */
 int cmtip_center_rows(const float* X, int64_t rows, int64_t P, const double* gamma, const float* c,

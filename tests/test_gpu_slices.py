@@ -40,6 +40,38 @@ def test_compute_slices_vs_oracle(M, n_side):
     assert rel_l2(got, want) < 1e-5
 
 
+@pytest.mark.parametrize("tile_pixels", [True, False])
+def test_slicer_paths_agree(tile_pixels):
+    # tiled (LDS box) and untiled orders give the same slices; scattered pixels force
+    # the direct-gather fallback
+    M = 41
+    ac = synth.gaussian_ac(M, 100, seed=3)
+    s, ext = synth.detector(64)
+    q = synth.random_quaternions(30, seed=9)
+    say(f"slicer tile_pixels={tile_pixels}")
+    sl = alignment.Slicer(ac, s, ext, tile_pixels=tile_pixels)
+    got = sl.slice(q).cpu().numpy().ravel()
+    want = OA.compute_slices(q, s, ext, ac, eps=1e-12)
+    assert rel_l2(got, want) < 1e-5
+    rng = np.random.default_rng(0)
+    srand = (rng.standard_normal((3, 1, 3000)) * ext / 3).astype(np.float32)
+    srand = np.clip(srand, -ext / 2, ext / 2)
+    say("slicer scattered pixels")
+    sl2 = alignment.Slicer(ac, srand, ext, tile_pixels=tile_pixels)
+    got2 = sl2.slice(q).cpu().numpy().ravel()
+    want2 = OA.compute_slices(q, srand, ext, ac, eps=1e-12)
+    assert rel_l2(got2, want2) < 1e-5
+
+
+def test_pixel_order_blocks():
+    s, _ = synth.detector(128)
+    order = alignment.align.pixel_order(torch.from_numpy(s.reshape(3, -1)).to(dev())).cpu().numpy()
+    assert sorted(order.tolist()) == list(range(128 * 128))
+    blk = order[:64]
+    r, c = blk // 128, blk % 128
+    assert r.max() - r.min() == 7 and c.max() - c.min() == 7
+
+
 def test_slice_bf16_and_moments():
     M, n_side = 21, 32
     s, ext = synth.detector(n_side)
