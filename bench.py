@@ -160,7 +160,7 @@ def main():
     N = n_ref * P
     n_fine = sl.plan.n_fine
     if slice_ms >= gemm_ms:
-        out_b = 4 + (2 if args.precision == "bf16" else 0)
+        out_b = 4
         bytes_launch = out_b * N + 24 * P + 16 * n_ref + 4 * n_fine ** 3
         achieved = bytes_launch / (slice_ms * 1e-3) / 1e9
         roof = {"kernel": "slice_real_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -14,6 +14,7 @@
 // spreads into a real fine grid (half the atomics).
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "es.h"
@@ -38,6 +39,19 @@ struct cmtip_nufft_plan_s {
     const float* x = nullptr;
     const float* y = nullptr;
     const float* z = nullptr;
+    // spatial bins of the points (set by setpts): counting sort into bins of bs^3 fine
+    // cells, bin-sorted coordinate copies, and the chunk list worked by the kernels
+    int bs = 0, nbx = 0;
+    int64_t nbins = 0;
+    DevBuf bin_of;    // N int32
+    DevBuf count;     // nbins int32 (also the scatter cursors)
+    DevBuf offset;    // nbins + 1 int32
+    DevBuf nchunk;    // nbins + 1 int32: chunk counts, then their exclusive scan
+    DevBuf perm;      // N int32: sorted position -> original index
+    DevBuf xs, ys, zs;  // N float: coordinates in sorted order
+    DevBuf chunks;    // max_chunks x int4 (bin, start, end, 0)
+    DevBuf ctrl;      // int32: [0] number of chunks, [1] work counter
+    int64_t max_chunks = 0;
 };
 
 namespace {
@@ -67,126 +81,345 @@ __device__ __forceinline__ float fine_coord(float x, float hi, float lo, int n) 
     return t;
 }
 
-template <int W>
-__device__ __forceinline__ void kernel_row(float x, float hi, float lo, int n, float beta, int* idx,
-                                           float* wt) {
+// ---------------------------------------------------------------- binning
+//
+// setpts sorts the points by the fine-grid bin holding the first node of their kernel
+// footprint (l0 mod n, per axis): bins are bs^3 cells, so every footprint of a bin's
+// points lies in its subgrid of (bs + W - 1)^3 cells.  Bins are cut into chunks of at
+// most kChunk points; each chunk is one unit of work for the spread (type 1) and
+// interpolation (type 2) kernels, which stage the chunk's subgrid in LDS.
+
+constexpr int kChunk = 4096;
+constexpr int kSpreadBlock = 256;
+
+// Subgrids are 16^3 cells for w <= 12 and 20^3 above (bin edge = subgrid edge - w + 1).
+__host__ __device__ inline int bin_size_for(int w) { return w <= 12 ? 17 - w : 21 - w; }
+
+__device__ __forceinline__ int first_node_wrapped(float x, float hi, float lo, int n, int W, float* t_out) {
     const float t = fine_coord(x, hi, lo, n);
-    const float l0f = ceilf(t - 0.5f * (float)W);
-    const int l0 = (int)l0f;
-    const float d = l0f - t;  // in [-W/2, -W/2 + 1)
-    int base = l0 % n;
-    if (base < 0) base += n;
+    *t_out = t;
+    int l0 = (int)ceilf(t - 0.5f * (float)W);
+    l0 %= n;
+    return l0 < 0 ? l0 + n : l0;
+}
+
+// Wave-aggregated counter update: lanes of a wave holding the same bin issue one
+// atomicAdd for the group and get consecutive slots (points that are neighbours in
+// the input, e.g. adjacent pixels, mostly share a bin).  Returns the lane's slot.
+__device__ __forceinline__ int wave_bin_add(int* __restrict__ counter, int b, bool valid) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long active = __ballot(valid);
+    int slot = 0;
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int lb = __shfl(b, leader, 64);
+        const unsigned long long m = __ballot(valid && b == lb) & active;
+        int old = 0;
+        if (lane == leader) old = atomicAdd(counter + lb, __popcll(m));
+        old = __shfl(old, leader, 64);
+        if ((m >> lane) & 1ull) slot = old + __popcll(m & ((1ull << lane) - 1ull));
+        active &= ~m;
+    }
+    return slot;
+}
+
+__global__ void bin_count_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                 const float* __restrict__ z, int64_t N, int n, int W, int bs, int nbx, float hi,
+                                 float lo, int* __restrict__ bin_of, int* __restrict__ count) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = j < N;
+    int b = 0;
+    if (valid) {
+        float t;
+        const int bx = first_node_wrapped(x[j], hi, lo, n, W, &t) / bs;
+        const int by = first_node_wrapped(y[j], hi, lo, n, W, &t) / bs;
+        const int bz = first_node_wrapped(z[j], hi, lo, n, W, &t) / bs;
+        b = (bx * nbx + by) * nbx + bz;
+        bin_of[j] = b;
+    }
+    wave_bin_add(count, b, valid);
+}
+
+// One workgroup: offset = exclusive scan of count, nchunk = exclusive scan of
+// ceil(count / kChunk); ctrl[0] = total chunks.  Counts are then cleared for use as
+// scatter cursors.
+__global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count, int64_t nbins,
+                                                        int* __restrict__ offset, int* __restrict__ nchunk,
+                                                        int* __restrict__ ctrl) {
+    __shared__ int sa[1024], sb[1024];
+    __shared__ int carry_a, carry_b;
+    if (threadIdx.x == 0) {
+        carry_a = 0;
+        carry_b = 0;
+    }
+    __syncthreads();
+    for (int64_t base = 0; base < nbins; base += 1024) {
+        const int64_t i = base + threadIdx.x;
+        const int c = i < nbins ? count[i] : 0;
+        int va = c, vb = (c + kChunk - 1) / kChunk;
+        sa[threadIdx.x] = va;
+        sb[threadIdx.x] = vb;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int ta = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0;
+            const int tb = threadIdx.x >= (unsigned)off ? sb[threadIdx.x - off] : 0;
+            __syncthreads();
+            sa[threadIdx.x] += ta;
+            sb[threadIdx.x] += tb;
+            __syncthreads();
+        }
+        if (i < nbins) {
+            offset[i] = carry_a + sa[threadIdx.x] - va;
+            nchunk[i] = carry_b + sb[threadIdx.x] - vb;
+            count[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) {
+            carry_a += sa[1023];
+            carry_b += sb[1023];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        offset[nbins] = carry_a;
+        nchunk[nbins] = carry_b;
+        ctrl[0] = carry_b;
+    }
+}
+
+__global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                   const float* __restrict__ z, int64_t N, const int* __restrict__ bin_of,
+                                   const int* __restrict__ offset, int* __restrict__ cursor, int* __restrict__ perm,
+                                   float* __restrict__ xs, float* __restrict__ ys, float* __restrict__ zs) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = j < N;
+    const int b = valid ? bin_of[j] : 0;
+    const int slot = wave_bin_add(cursor, b, valid);
+    if (!valid) return;
+    const int pos = offset[b] + slot;
+    perm[pos] = (int)j;
+    xs[pos] = x[j];
+    ys[pos] = y[j];
+    zs[pos] = z[j];
+}
+
+__global__ void chunk_fill_kernel(const int* __restrict__ offset, const int* __restrict__ nchunk, int64_t nbins,
+                                  int4* __restrict__ chunks) {
+    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nbins) return;
+    const int start = offset[b], end = offset[b + 1];
+    int c = nchunk[b];
+    for (int s0 = start; s0 < end; s0 += kChunk, ++c) chunks[c] = make_int4((int)b, s0, min(end, s0 + kChunk), 0);
+}
+
+// ---------------------------------------------------------------- binned spreading / interpolation
+//
+// Persistent workgroups take chunks from a device work counter until none are left.
+// Spreading: zero the LDS subgrid, add every point's w^3 contributions with LDS
+// atomics, then add the nonzero subgrid cells to the fine grid with one global atomic
+// each.  Interpolation: load the subgrid from the fine grid, then every point gathers
+// its w^3 taps from LDS and is written to its original position.
+
+template <bool CPLX>
+struct Cell;
+template <>
+struct Cell<false> {
+    typedef float T;
+};
+template <>
+struct Cell<true> {
+    typedef float2 T;
+};
+
+template <int W>
+__device__ __forceinline__ void local_weights(float t, int l0w, int origin, float beta_l2e, int* loc, float* wt) {
+    // l0w: wrapped first node; t: fine coordinate (unwrapped); origin: bin origin cell
+    const float l0 = ceilf(t - 0.5f * (float)W);
+    const float d = l0 - t;
+    *loc = l0w - origin;
     const float tw = 2.0f / (float)W;
 #pragma unroll
-    for (int a = 0; a < W; ++a) {
-        int i = base + a;
-        i = i >= n ? i - n : i;
-        idx[a] = i;
-        wt[a] = es_phi(d + (float)a, tw, beta);
+    for (int a = 0; a < W; ++a) wt[a] = es_phi(d + (float)a, tw, beta_l2e);
+}
+
+// Spreading accumulates in LDS in 64-bit fixed point: integer LDS atomics are an
+// order of magnitude faster than float ones on this hardware, and integer sums are
+// exact and independent of the order of the adds.  Every contribution (an FP32
+// product) is scaled by 2^sc and rounded to an integer; sc is chosen from max|c| so
+// that a cell's sum over a chunk of at most kChunk points (weights <= 1) stays below
+// 2^62.  The chunk's cells are then added to the fine grid as floats.
+__device__ __forceinline__ int spread_scale_exp(float cmax) {
+    if (!(cmax > 0.0f)) return 0;
+    int e;
+    frexpf(cmax, &e);  // cmax < 2^e
+    return 62 - 13 - e;  // kChunk = 2^12 points, one bit of margin
+}
+
+__global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsigned* __restrict__ out_bits) {
+    float m = 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) m = fmaxf(m, fabsf(c[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(m));  // order of non-negative floats
+}
+
+template <int W, bool CPLX>
+__global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
+    typename Cell<CPLX>::T* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx,
+    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
+    const int* __restrict__ perm, const typename Cell<CPLX>::T* __restrict__ c, const unsigned* __restrict__ cmax_bits,
+    const int4* __restrict__ chunks, int* __restrict__ ctrl) {
+    typedef typename Cell<CPLX>::T T;
+    constexpr int NC = CPLX ? 2 : 1;  // integer accumulators per cell
+    extern __shared__ unsigned long long acc[];
+    // LDS layout (x * S + y) * SP + z with an odd row pitch SP = S + 1 (bank spread)
+    const int S = bs + W - 1, SP = S + 1, SSP = S * SP, S3 = S * S * S, L3 = S * S * SP;
+    __shared__ int chunk_id;
+    const int total = ctrl[0];
+    const int sc = spread_scale_exp(c ? __uint_as_float(*cmax_bits) : 1.0f);
+    const double down = ldexp(1.0, -sc);
+    for (;;) {
+        if (threadIdx.x == 0) chunk_id = atomicAdd(ctrl + 1, 1);
+        __syncthreads();
+        const int cid = chunk_id;
+        if (cid >= total) break;
+        const int4 ch = chunks[cid];
+        const int b = ch.x;
+        const int ox = (b / (nbx * nbx)) * bs, oy = ((b / nbx) % nbx) * bs, oz = (b % nbx) * bs;
+        for (int i = threadIdx.x; i < NC * L3; i += kSpreadBlock) acc[i] = 0ull;
+        __syncthreads();
+        for (int k = ch.y + threadIdx.x; k < ch.z; k += kSpreadBlock) {
+            float tx, ty, tz;
+            const int lxw = first_node_wrapped(xs[k], hi, lo, n, W, &tx);
+            const int lyw = first_node_wrapped(ys[k], hi, lo, n, W, &ty);
+            const int lzw = first_node_wrapped(zs[k], hi, lo, n, W, &tz);
+            int lx, ly, lz;
+            float wx[W], wy[W], wz[W];
+            local_weights<W>(tx, lxw, ox, beta_l2e, &lx, wx);
+            local_weights<W>(ty, lyw, oy, beta_l2e, &ly, wy);
+            local_weights<W>(tz, lzw, oz, beta_l2e, &lz, wz);
+            // lx, ly, lz lie in [0, bs) by construction of the bins; never index outside LDS
+            if ((unsigned)lx >= (unsigned)bs || (unsigned)ly >= (unsigned)bs || (unsigned)lz >= (unsigned)bs) continue;
+            float vr, vi = 0.f;
+            if (CPLX) {
+                const float2 cv = reinterpret_cast<const float2*>(c)[perm[k]];
+                vr = ldexpf(cv.x, sc);
+                vi = ldexpf(cv.y, sc);
+            } else {
+                vr = ldexpf(c ? reinterpret_cast<const float*>(c)[perm[k]] : 1.0f, sc);
+            }
+            unsigned long long* base = acc + NC * ((lx * S + ly) * SP + lz);
+#pragma unroll
+            for (int a = 0; a < W; ++a)
+#pragma unroll
+                for (int bb = 0; bb < W; ++bb) {
+                    const float wab = wx[a] * wy[bb];
+                    unsigned long long* row = base + NC * (a * SSP + bb * SP);
+                    const float pr = wab * vr;
+                    const float pi = wab * vi;
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) {
+                        atomicAdd(row + NC * cc, (unsigned long long)__float2ll_rn(wz[cc] * pr));
+                        if (CPLX) atomicAdd(row + NC * cc + 1, (unsigned long long)__float2ll_rn(wz[cc] * pi));
+                    }
+                }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < S3; i += kSpreadBlock) {
+            const int ix = i / (S * S), iy = (i / S) % S, iz = i % S;
+            const int li = (ix * S + iy) * SP + iz;
+            int gx = ox + ix, gy = oy + iy, gz = oz + iz;
+            gx = gx >= n ? gx - n : gx;
+            gy = gy >= n ? gy - n : gy;
+            gz = gz >= n ? gz - n : gz;
+            const uint32_t off = (uint32_t)((gx * n + gy) * n + gz);
+            float* dst = reinterpret_cast<float*>(grid) + (size_t)NC * off;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const long long v = (long long)acc[NC * li + q];
+                if (v != 0) unsafeAtomicAdd(dst + q, (float)((double)v * down));
+            }
+        }
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------- interpolation
-
 template <int W, bool CPLX>
-__global__ void __launch_bounds__(kBlock) interp_kernel(const void* __restrict__ grid, int n, float beta,
-                                                        float hi, float lo, int64_t N,
-                                                        const float* __restrict__ x,
-                                                        const float* __restrict__ y,
-                                                        const float* __restrict__ z,
-                                                        float2* __restrict__ out_c,
-                                                        float* __restrict__ out_r) {
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j >= N) return;
-    int ix[W], iy[W], iz[W];
-    float wx[W], wy[W], wz[W];
-    kernel_row<W>(x[j], hi, lo, n, beta, ix, wx);
-    kernel_row<W>(y[j], hi, lo, n, beta, iy, wy);
-    kernel_row<W>(z[j], hi, lo, n, beta, iz, wz);
-    if (CPLX) {
-        const float2* g = reinterpret_cast<const float2*>(grid);
-        float sr = 0.f, si = 0.f;
-#pragma unroll
-        for (int a = 0; a < W; ++a) {
-            float pr = 0.f, pi = 0.f;
-#pragma unroll 1
-            for (int b = 0; b < W; ++b) {
-                const float2* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
-                float qr = 0.f, qi = 0.f;
-#pragma unroll
-                for (int c = 0; c < W; ++c) {
-                    const float2 v = row[iz[c]];
-                    qr = fmaf(wz[c], v.x, qr);
-                    qi = fmaf(wz[c], v.y, qi);
-                }
-                pr = fmaf(wy[b], qr, pr);
-                pi = fmaf(wy[b], qi, pi);
-            }
-            sr = fmaf(wx[a], pr, sr);
-            si = fmaf(wx[a], pi, si);
+__global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
+    const typename Cell<CPLX>::T* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx,
+    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
+    const int* __restrict__ perm, float2* __restrict__ out_c, float* __restrict__ out_r,
+    const int4* __restrict__ chunks, int* __restrict__ ctrl) {
+    typedef typename Cell<CPLX>::T T;
+    extern __shared__ float sub_raw[];
+    T* sub = reinterpret_cast<T*>(sub_raw);
+    // LDS layout (x * S + y) * SP + z with an odd row pitch SP = S + 1 (bank spread)
+    const int S = bs + W - 1, SP = S + 1, SSP = S * SP, S3 = S * S * S;
+    __shared__ int chunk_id;
+    const int total = ctrl[0];
+    for (;;) {
+        if (threadIdx.x == 0) chunk_id = atomicAdd(ctrl + 1, 1);
+        __syncthreads();
+        const int cid = chunk_id;
+        if (cid >= total) break;
+        const int4 ch = chunks[cid];
+        const int b = ch.x;
+        const int ox = (b / (nbx * nbx)) * bs, oy = ((b / nbx) % nbx) * bs, oz = (b % nbx) * bs;
+        for (int i = threadIdx.x; i < S3; i += kSpreadBlock) {
+            const int ix = i / (S * S), iy = (i / S) % S, iz = i % S;
+            int gx = ox + ix, gy = oy + iy, gz = oz + iz;
+            gx = gx >= n ? gx - n : gx;
+            gy = gy >= n ? gy - n : gy;
+            gz = gz >= n ? gz - n : gz;
+            sub[(ix * S + iy) * SP + iz] = grid[(uint32_t)((gx * n + gy) * n + gz)];
         }
-        if (out_c) out_c[j] = make_float2(sr, si);
-        if (out_r) out_r[j] = sr;
-    } else {
-        const float* g = reinterpret_cast<const float*>(grid);
-        float s = 0.f;
-#pragma unroll
-        for (int a = 0; a < W; ++a) {
-            float p = 0.f;
-            for (int b = 0; b < W; ++b) {
-                const float* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
-                float q = 0.f;
-#pragma unroll
-                for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
-                p = fmaf(wy[b], q, p);
+        __syncthreads();
+        for (int k = ch.y + threadIdx.x; k < ch.z; k += kSpreadBlock) {
+            float tx, ty, tz;
+            const int lxw = first_node_wrapped(xs[k], hi, lo, n, W, &tx);
+            const int lyw = first_node_wrapped(ys[k], hi, lo, n, W, &ty);
+            const int lzw = first_node_wrapped(zs[k], hi, lo, n, W, &tz);
+            int lx, ly, lz;
+            float wx[W], wy[W], wz[W];
+            local_weights<W>(tx, lxw, ox, beta_l2e, &lx, wx);
+            local_weights<W>(ty, lyw, oy, beta_l2e, &ly, wy);
+            local_weights<W>(tz, lzw, oz, beta_l2e, &lz, wz);
+            // lx, ly, lz lie in [0, bs) by construction of the bins; never index outside LDS
+            if ((unsigned)lx >= (unsigned)bs || (unsigned)ly >= (unsigned)bs || (unsigned)lz >= (unsigned)bs) {
+                lx = 0;
+                ly = 0;
+                lz = 0;
             }
-            s = fmaf(wx[a], p, s);
-        }
-        if (out_c) out_c[j] = make_float2(s, 0.f);
-        if (out_r) out_r[j] = s;
-    }
-}
-
-// ---------------------------------------------------------------- spreading
-
-template <int W, bool CPLX>
-__global__ void __launch_bounds__(kBlock) spread_kernel(void* __restrict__ grid, int n, float beta, float hi,
-                                                        float lo, int64_t N,
-                                                        const float* __restrict__ x,
-                                                        const float* __restrict__ y,
-                                                        const float* __restrict__ z,
-                                                        const void* __restrict__ c) {
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j >= N) return;
-    int ix[W], iy[W], iz[W];
-    float wx[W], wy[W], wz[W];
-    kernel_row<W>(x[j], hi, lo, n, beta, ix, wx);
-    kernel_row<W>(y[j], hi, lo, n, beta, iy, wy);
-    kernel_row<W>(z[j], hi, lo, n, beta, iz, wz);
-    if (CPLX) {
-        const float2 v = reinterpret_cast<const float2*>(c)[j];
-        float* g = reinterpret_cast<float*>(grid);
-        for (int a = 0; a < W; ++a)
-            for (int b = 0; b < W; ++b) {
-                const float wab = wx[a] * wy[b];
-                const float vr = wab * v.x, vi = wab * v.y;
-                float* row = g + 2 * (((int64_t)ix[a] * n + iy[b]) * n);
+            const T* base = sub + (lx * S + ly) * SP + lz;
+            float sr = 0.f, si = 0.f;
 #pragma unroll
-                for (int cc = 0; cc < W; ++cc) {
-                    unsafeAtomicAdd(row + 2 * iz[cc], wz[cc] * vr);
-                    unsafeAtomicAdd(row + 2 * iz[cc] + 1, wz[cc] * vi);
+            for (int a = 0; a < W; ++a) {
+                float pr = 0.f, pi = 0.f;
+#pragma unroll
+                for (int bb = 0; bb < W; ++bb) {
+                    const T* row = base + a * SSP + bb * SP;
+                    float qr = 0.f, qi = 0.f;
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) {
+                        if (CPLX) {
+                            const float2 v = *reinterpret_cast<const float2*>(row + cc);
+                            qr = fmaf(wz[cc], v.x, qr);
+                            qi = fmaf(wz[cc], v.y, qi);
+                        } else {
+                            qr = fmaf(wz[cc], *reinterpret_cast<const float*>(row + cc), qr);
+                        }
+                    }
+                    pr = fmaf(wy[bb], qr, pr);
+                    if (CPLX) pi = fmaf(wy[bb], qi, pi);
                 }
+                sr = fmaf(wx[a], pr, sr);
+                if (CPLX) si = fmaf(wx[a], pi, si);
             }
-    } else {
-        const float v = c ? reinterpret_cast<const float*>(c)[j] : 1.0f;
-        float* g = reinterpret_cast<float*>(grid);
-        for (int a = 0; a < W; ++a)
-            for (int b = 0; b < W; ++b) {
-                const float vr = wx[a] * wy[b] * v;
-                float* row = g + ((int64_t)ix[a] * n + iy[b]) * n;
-#pragma unroll
-                for (int cc = 0; cc < W; ++cc) unsafeAtomicAdd(row + iz[cc], wz[cc] * vr);
-            }
+            const int j = perm[k];
+            if (out_c) out_c[j] = make_float2(sr, si);
+            if (out_r) out_r[j] = sr;
+        }
+        __syncthreads();
     }
 }
 
@@ -210,6 +443,12 @@ __global__ void finish_moments_kernel(const double* __restrict__ part, double* _
 __global__ void fill_f64_kernel(double* __restrict__ p, int64_t count, double v) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < count) p[i] = v;
+}
+
+__global__ void fill_i32_kernel(int* __restrict__ p, int64_t count, int v) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t k = i; k < count; k += stride) p[k] = v;
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ p, int64_t count, float v) {
@@ -496,18 +735,42 @@ int dispatch_w(int w, Args... args) {
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+int cu_count() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+    }
+    return cus;
+}
+
+int reset_work_counter(cmtip_nufft_plan_s* p, hipStream_t st) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->ctrl.as<int>() + 1, (int64_t)1, 0);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
 template <int W>
 struct InterpLaunch {
-    static int run(const cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, hipStream_t st) {
+    static int run(cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, hipStream_t st) {
         if (p->npts == 0) return CMTIP_OK;
+        CMTIP_TRY(reset_work_counter(p, st));
+        const int S = p->bs + W - 1;
+        const size_t lds = (size_t)S * S * (S + 1) * (cplx ? sizeof(float2) : sizeof(float));
+        const unsigned grid = (unsigned)(cu_count() * (cplx ? 2 : 4));
+        const int4* ch = p->chunks.as<const int4>();
         if (cplx)
-            hipLaunchKernelGGL((interp_kernel<W, true>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
-                               p->z, out_c, out_r);
+            hipLaunchKernelGGL((interp_binned_kernel<W, true>), dim3(grid), dim3(kSpreadBlock), lds, st,
+                               p->fine.as<const float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
+                               p->perm.as<const int>(), out_c, out_r, ch, p->ctrl.as<int>());
         else
-            hipLaunchKernelGGL((interp_kernel<W, false>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
-                               p->z, out_c, out_r);
+            hipLaunchKernelGGL((interp_binned_kernel<W, false>), dim3(grid), dim3(kSpreadBlock), lds, st,
+                               p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
+                               p->perm.as<const int>(), out_c, out_r, ch, p->ctrl.as<int>());
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -515,20 +778,70 @@ struct InterpLaunch {
 
 template <int W>
 struct SpreadLaunch {
-    static int run(const cmtip_nufft_plan_s* p, bool cplx, const void* c, hipStream_t st) {
+    static int run(cmtip_nufft_plan_s* p, bool cplx, const void* c, hipStream_t st) {
         if (p->npts == 0) return CMTIP_OK;
+        CMTIP_TRY(reset_work_counter(p, st));
+        unsigned* cmax = reinterpret_cast<unsigned*>(p->ctrl.as<int>() + 2);
+        if (c) {
+            hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int*>(cmax),
+                               (int64_t)1, 0);
+            const int64_t cnt = (cplx ? 2 : 1) * p->npts;
+            hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)std::min<int64_t>(4096, (cnt + kBlock - 1) / kBlock)),
+                               dim3(kBlock), 0, st, reinterpret_cast<const float*>(c), cnt, cmax);
+        }
+        const int S = p->bs + W - 1;
+        const size_t lds = (size_t)S * S * (S + 1) * (cplx ? 2 : 1) * sizeof(unsigned long long);
+        const unsigned grid = (unsigned)(cu_count() * (lds > 40000 ? 2 : 4));
+        const int4* ch = p->chunks.as<const int4>();
         if (cplx)
-            hipLaunchKernelGGL((spread_kernel<W, true>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
-                               p->z, c);
+            hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadBlock), lds, st,
+                               p->fine.as<float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
+                               p->perm.as<const int>(), reinterpret_cast<const float2*>(c), cmax, ch,
+                               p->ctrl.as<int>());
         else
-            hipLaunchKernelGGL((spread_kernel<W, false>), dim3(blocks_for(p->npts)), dim3(kBlock), 0, st,
-                               p->fine.p, p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->npts, p->x, p->y,
-                               p->z, c);
+            hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadBlock), lds, st,
+                               p->fine.as<float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
+                               p->perm.as<const int>(), reinterpret_cast<const float*>(c), cmax, ch,
+                               p->ctrl.as<int>());
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
 };
+
+// Sort the registered points into spatial bins (counting sort) and build the chunk list.
+int bin_points(cmtip_nufft_plan_s* p, hipStream_t st) {
+    const int64_t N = p->npts;
+    p->bs = bin_size_for(p->w);
+    p->nbx = (p->n + p->bs - 1) / p->bs;
+    p->nbins = (int64_t)p->nbx * p->nbx * p->nbx;
+    p->max_chunks = p->nbins + N / kChunk + 1;
+    CMTIP_TRY(p->bin_of.ensure(sizeof(int) * (size_t)N));
+    CMTIP_TRY(p->count.ensure(sizeof(int) * (size_t)p->nbins));
+    CMTIP_TRY(p->offset.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
+    CMTIP_TRY(p->nchunk.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
+    CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)N));
+    CMTIP_TRY(p->xs.ensure(sizeof(float) * (size_t)N));
+    CMTIP_TRY(p->ys.ensure(sizeof(float) * (size_t)N));
+    CMTIP_TRY(p->zs.ensure(sizeof(float) * (size_t)N));
+    CMTIP_TRY(p->chunks.ensure(sizeof(int4) * (size_t)p->max_chunks));
+    CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
+    const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
+    const unsigned gp = blocks_for(N);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), p->nbins, 0);
+    hipLaunchKernelGGL(bin_count_kernel, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N, p->n, p->w, p->bs,
+                       p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins,
+                       p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
+                       p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(),
+                       p->xs.as<float>(), p->ys.as<float>(), p->zs.as<float>());
+    hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
+                       p->nchunk.as<const int>(), p->nbins, p->chunks.as<int4>());
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
 
 template <int W>
 struct SliceLaunch {
@@ -789,17 +1102,21 @@ extern "C" int cmtip_nufft_info(cmtip_nufft_plan p, int* w, int* n_fine, double*
 
 extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y,
                                   const float* z, void* stream) {
-    (void)stream;
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
     if (npts < 0 || (npts > 0 && (!x || !y || !z))) {
         set_error("setpts: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (npts > 0x7fffffffLL) {
+        set_error("setpts: at most 2^31 - 1 points per plan");
         return CMTIP_ERR_ARG;
     }
     p->npts = npts;
     p->x = x;
     p->y = y;
     p->z = z;
-    return CMTIP_OK;
+    if (npts == 0) return CMTIP_OK;
+    return bin_points(p, as_stream(stream));
 }
 
 extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int flags, const float* support,
@@ -821,7 +1138,7 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     if (p->type == 2) {
         p->slice_ready = 0;
         CMTIP_TRY(grid_to_fine(p, f, grid_real, support, pts_real, st));
-        return dispatch_w<InterpLaunch>(p->w, (const cmtip_nufft_plan_s*)p, !pts_real,
+        return dispatch_w<InterpLaunch>(p->w, p, !pts_real,
                                         pts_real ? (float2*)nullptr : (float2*)c,
                                         pts_real ? (float*)c : (float*)nullptr, st);
     }
@@ -829,7 +1146,7 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     p->slice_ready = 0;
     const int64_t n3 = (int64_t)p->n * p->n * p->n;
     CMTIP_TRY(fill_zero(p->fine.p, pts_real ? n3 : 2 * n3, st));
-    CMTIP_TRY(dispatch_w<SpreadLaunch>(p->w, (const cmtip_nufft_plan_s*)p, !pts_real, (const void*)c, st));
+    CMTIP_TRY(dispatch_w<SpreadLaunch>(p->w, p, !pts_real, (const void*)c, st));
     return fine_to_grid(p, pts_real, f, grid_real, support, scale, st);
 }
 
@@ -842,6 +1159,9 @@ extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     p->tmpB.release();
     p->rinv.release();
     p->part.release();
+    for (DevBuf* b : {&p->bin_of, &p->count, &p->offset, &p->nchunk, &p->perm, &p->xs, &p->ys, &p->zs, &p->chunks,
+                      &p->ctrl})
+        b->release();
     delete p;
     return CMTIP_OK;
 }
