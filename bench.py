@@ -34,6 +34,16 @@ BF16_PEAK_TFS = 2500.0
 FP32_MFMA_PEAK_TFS = 157.3
 
 
+def recorded_traffic(kernel):
+    """HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
+    profiles/traffic.json (each counter scaled by its copy-kernel calibration)."""
+    try:
+        rec = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return 1024.0 * (rec["fetch_size_kib"] * rec["fetch_scale"] + rec["write_size_kib"] * rec["write_scale"])
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -48,6 +58,9 @@ def parse():
     p.add_argument("--nbatches", type=int, default=1)
     p.add_argument("--no-extras", action="store_true", help="time the step only")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic"],
+                   help="slice: one slicing pass of the workload (for counter runs); nufft: setpts / execute "
+                        "timing of both transforms; traffic: a 1 GiB copy (counter calibration) then one slicing pass")
     return p.parse_args()
 
 
@@ -93,6 +106,8 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     M, atoms, side, n_img, n_ref = CONFIGS[args.config]
+    if args.part != "all":
+        return part_main(args, M, atoms, side, n_img, n_ref, dev)
     P = side * side
     log(f"[bench] config {args.config}: M={M} atoms={atoms} P={P} n_img={n_img} n_ref={n_ref}")
     t0 = time.time()
@@ -164,7 +179,8 @@ def main():
         bytes_launch = out_b * N + 24 * P + 16 * n_ref + 4 * n_fine ** 3
         achieved = bytes_launch / (slice_ms * 1e-3) / 1e9
         roof = {"kernel": "slice_real_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic("slice_real_kernel")
+                if args.config == "c2" and args.nbatches == 1 else None,
                 "per_unit": f"{out_b} B per rotated pixel + 24 B per pixel + 16 B per orientation + fine grid",
                 "avg_ms": slice_ms}
     else:
@@ -243,6 +259,65 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def part_main(args, M, atoms, side, n_img, n_ref, dev):
+    """Single-purpose runs used under rocprofv3 (see --part)."""
+    import torch
+
+    from cmtip_amd import alignment, nufft, synth
+    from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_ONES, CMTIP_PTS_REAL
+    from cmtip_amd.autocorrelation import autocorrelation as AC
+
+    if args.part == "traffic":
+        a = torch.ones(256 * 1024 * 1024, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)  # 1 GiB read + 1 GiB written: the FETCH_SIZE / WRITE_SIZE scale
+        torch.cuda.synchronize()
+        del a, b
+    ac = torch.from_numpy(synth.gaussian_ac(M, atoms, seed=0)).to(dev)
+    s, ext = synth.detector(side)
+    if args.part in ("slice", "traffic"):
+        ref = synth.random_quaternions(n_ref, seed=0)
+        sl = alignment.Slicer(ac, s, ext)
+        out = sl.slice(ref)
+        for _ in range(max(0, args.steps - 1)):
+            sl.slice(ref, out=out)
+        torch.cuda.synchronize()
+        log(f"[bench] slicing pass done ({n_ref} references)")
+        return
+    # nufft: setpts and execute of both transforms over the image q-vectors
+    q = synth.random_quaternions(n_img, seed=1)
+    x, y, z = AC.gen_nonuniform_positions(torch.from_numpy(q).to(dev), torch.from_numpy(s).to(dev), recip_extent=ext)
+    x, y, z = (t.reshape(-1).contiguous() for t in (x, y, z))
+    N = x.numel()
+    data = torch.rand(N, device=dev)
+
+    def tm(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    res = {"points": N}
+    for typ, modes, flags, label in ((2, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type2_real"),
+                                     (2, M, CMTIP_GRID_REAL, "type2_complex"),
+                                     (1, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type1_real"),
+                                     (1, 2 * M, CMTIP_GRID_REAL | CMTIP_PTS_ONES, "type1_ones_2M")):
+        p = nufft.get_plan(typ, modes, -1 if typ == 2 else 1)
+        ts = tm(lambda: p.setpts(x, y, z), 3)
+        if typ == 2:
+            out = torch.empty(N, device=dev, dtype=torch.float32 if flags & CMTIP_PTS_REAL else torch.complex64)
+            te = tm(lambda: p.execute(out, ac, flags))
+        else:
+            out = torch.empty((modes,) * 3, device=dev)
+            te = tm(lambda: p.execute(data, out, flags, None, 1.0 / modes ** 3))
+        res[label] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": N / te / 1e6}
+        log(f"[bench] {label}: setpts {ts * 1e3:.2f} ms, execute {te * 1e3:.2f} ms, {N / te / 1e6:.0f} Mpts/s")
+    print(json.dumps({"part": "nufft", "M": M, **res}), flush=True)
+
+
 def cpu_baseline(ac, s, ext, ref, imgs, n_img, n_ref, P):
     """The oracle (FP64 ES NUFFT at eps 1e-6 + sklearn distances + argmin) on a bounded
     sample, extrapolated linearly to the full workload."""
@@ -252,18 +327,16 @@ def cpu_baseline(ac, s, ext, ref, imgs, n_img, n_ref, P):
     from oracle import nufft as O
 
     cores = len(os.sched_getaffinity(0))
-    n_s = 64
+    n_s = 256
     x, y, z = geom.rotated_coords(ref[:n_s], s, ext)
     t = time.perf_counter()
     model = O.nufft2(ac.astype(np.float64), x, y, z, eps=1e-6).real.reshape(n_s, P).astype(np.float32)
     t_slice = time.perf_counter() - t
-    m_s = 1000
+    m_s = min(2000, len(imgs))
     t = time.perf_counter()
-    reps = max(1, m_s // n_s)
-    for _ in range(reps):
-        D = euclidean_distances(model, imgs[:m_s])
-        np.argmin(D, axis=0)
-    t_match = (time.perf_counter() - t) / reps
+    D = euclidean_distances(model, imgs[:m_s])
+    np.argmin(D, axis=0)
+    t_match = time.perf_counter() - t
     per_slice = t_slice / n_s
     per_pair = t_match / (n_s * m_s)
     total = per_slice * n_ref + per_pair * n_ref * n_img
