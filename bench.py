@@ -253,6 +253,24 @@ def main():
             solver.cg(d, xx, 1.0, rl, 1e3, 20, 0.0)
         tcg = tm(cg20, reps=2)
         result["solve"] = {"cg_iterations": 20, "cg_ms_per_iteration": tcg / 20 * 1e3}
+        # differentiable slicing: forward + backward for 2048 orientations x P pixels
+        from cmtip_amd import autograd as AG
+        nb = min(2048, n_ref)
+        qb = ref_d[:nb].clone().requires_grad_(True)
+        acg = ac_d.clone().requires_grad_(True)
+        target = imgs[:nb].reshape(nb, -1) if nb <= n_img else imgs[:1].expand(nb, -1)
+
+        def fwdbwd(kind):
+            qb.grad = None
+            acg.grad = None
+            model = AG.slice_quaternions(acg, qb, s, ext)
+            loss = AG.l2_loss(model, target) if kind == "l2" else AG.poisson_nll(model, target)
+            loss.backward()
+        t_l2 = tm(lambda: fwdbwd("l2"), reps=3)
+        t_pn = tm(lambda: fwdbwd("poisson"), reps=3)
+        result["autograd"] = {"orientations": nb, "points": nb * P, "fwd_bwd_ms_l2": t_l2 * 1e3,
+                              "fwd_bwd_ms_poisson": t_pn * 1e3, "Mpts_s_l2": nb * P / t_l2 / 1e6}
+        log(f"[bench] autograd {result['autograd']}")
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ac, s, ext, ref, imgs.cpu().numpy(), n_img, n_ref, P)

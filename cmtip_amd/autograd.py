@@ -1,0 +1,136 @@
+"""Differentiable slicing: the type-2 slice of a real autocorrelation as a
+``torch.autograd.Function``.
+
+Forward (compute_slices, cmtip/alignment/align.py:39-65):
+    I_j = Re sum_k f_k exp(-i k.x_j)
+Backward, for an upstream gradient g (real, one value per point):
+    dL/df_k     = sum_j g_j cos(k.x_j) = Re(type 1 of g)_k          (the adjoint spread)
+    dL/dx_{j,d} = g_j Im(sum_k k_d f_k exp(-i k.x_j))               (an ik-weighted type 2, d = 0, 1, 2)
+Both run on the same HIP NUFFT plans as the forward.  ``rotated_coords`` builds the
+rotated pixel coordinates from quaternions with torch operations, so gradients reach
+the quaternions for orientation refinement.
+"""
+import numpy as np
+import torch
+
+from ._lib import CMTIP_GRID_REAL, CMTIP_PTS_REAL
+from .nufft.plan import DEFAULT_EPS, get_plan
+
+
+def _modes(M, device):
+    return torch.arange(M, device=device, dtype=torch.float32) - (M // 2)
+
+
+class NufftSlice(torch.autograd.Function):
+    """I = Re(T2(f)) at (H, K, L); differentiable in f and in the coordinates."""
+
+    @staticmethod
+    def forward(ctx, ac, H, K, L, eps=DEFAULT_EPS):
+        ac = ac.contiguous().float()
+        x, y, z = (t.contiguous().float().reshape(-1) for t in (H, K, L))
+        M = ac.shape[0]
+        plan = get_plan(2, M, -1, eps)
+        plan.setpts(x, y, z)
+        out = torch.empty(x.numel(), dtype=torch.float32, device=ac.device)
+        plan.execute(out, ac, CMTIP_GRID_REAL | CMTIP_PTS_REAL)
+        ctx.save_for_backward(ac, x, y, z)
+        ctx.eps = eps
+        ctx.shape = H.shape
+        return out.reshape(H.shape)
+
+    @staticmethod
+    def backward(ctx, grad):
+        ac, x, y, z = ctx.saved_tensors
+        eps = ctx.eps
+        M = ac.shape[0]
+        g = grad.contiguous().float().reshape(-1)
+        grad_ac = grad_H = grad_K = grad_L = None
+        if ctx.needs_input_grad[0]:
+            p1 = get_plan(1, M, +1, eps)
+            p1.setpts(x, y, z)
+            grad_ac = torch.empty_like(ac)
+            p1.execute(g, grad_ac, CMTIP_GRID_REAL | CMTIP_PTS_REAL, None, 1.0)
+        if any(ctx.needs_input_grad[1:4]):
+            p2 = get_plan(2, M, -1, eps)
+            p2.setpts(x, y, z)
+            k = _modes(M, ac.device)
+            outs = []
+            for d in range(3):
+                if not ctx.needs_input_grad[1 + d]:
+                    outs.append(None)
+                    continue
+                shape = [1, 1, 1]
+                shape[d] = M
+                kf = (ac * k.reshape(shape)).contiguous()
+                c = torch.empty(x.numel(), dtype=torch.complex64, device=ac.device)
+                p2.execute(c, kf, CMTIP_GRID_REAL)
+                outs.append((g * c.imag).reshape(ctx.shape))
+            grad_H, grad_K, grad_L = outs
+        return grad_ac, grad_H, grad_K, grad_L, None
+
+
+def nufft_slice(ac, H, K, L, eps=DEFAULT_EPS):
+    return NufftSlice.apply(ac, H, K, L, eps)
+
+
+def quaternion_rotation(q):
+    """Rotation matrices of (normalised) scalar-first quaternions, cmtip/eval/geom.py:37-45."""
+    q = q / torch.linalg.vector_norm(q, dim=-1, keepdim=True)
+    q0, q1, q2, q3 = q.unbind(-1)
+    R = torch.stack([
+        1 - 2 * (q2 * q2 + q3 * q3), 2 * (q1 * q2 - q0 * q3), 2 * (q1 * q3 + q0 * q2),
+        2 * (q1 * q2 + q0 * q3), 1 - 2 * (q1 * q1 + q3 * q3), 2 * (q2 * q3 - q0 * q1),
+        2 * (q1 * q3 - q0 * q2), 2 * (q2 * q3 + q0 * q1), 1 - 2 * (q1 * q1 + q2 * q2),
+    ], -1)
+    return R.reshape(q.shape[:-1] + (3, 3))
+
+
+def rotated_coords(q, pixel_position_reciprocal, reciprocal_extent):
+    """x = pi R(q)^-1 s / reciprocal_extent for every (orientation, pixel), differentiable
+    in q (R^-1 = R^T for the normalised quaternion).  Returns (H, K, L), each (n, P)."""
+    s = torch.as_tensor(pixel_position_reciprocal, device=q.device, dtype=q.dtype).reshape(3, -1)
+    Rinv = quaternion_rotation(q).transpose(-1, -2)
+    v = torch.einsum("nij,jp->inp", Rinv, s) * (np.pi / float(reciprocal_extent))
+    return v[0], v[1], v[2]
+
+
+def slice_quaternions(ac, q, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS):
+    """Slices (n, P) of ac at quaternions q, differentiable in ac and q."""
+    H, K, L = rotated_coords(q, pixel_position_reciprocal, reciprocal_extent)
+    return nufft_slice(ac, H, K, L, eps)
+
+
+def l2_loss(model, images, gamma=None):
+    """sum (gamma model - images)^2 (the matching distance, align.py:122-128, unweighted)."""
+    if gamma is None:
+        gamma = 1.0
+    return ((gamma * model - images) ** 2).sum()
+
+
+def poisson_nll(model, images, floor=1e-6):
+    """sum (model - images log model) for Poisson counts (model clamped at floor)."""
+    m = torch.clamp(model, min=floor)
+    return (m - images * torch.log(m)).sum()
+
+
+def refine_orientations(ac, q0, images, pixel_position_reciprocal, reciprocal_extent, steps=50, lr=2e-3,
+                        loss="l2", eps=DEFAULT_EPS):
+    """Gradient refinement of quaternions q0 (n, 4) against images (n, P); returns the
+    refined, normalised quaternions (scalar part >= 0)."""
+    q = q0.detach().clone().float().requires_grad_(True)
+    opt = torch.optim.Adam([q], lr=lr)
+    imgs = images.float()
+    ac = ac.float()
+    for _ in range(steps):
+        opt.zero_grad()
+        model = slice_quaternions(ac, q, pixel_position_reciprocal, reciprocal_extent, eps)
+        if loss == "l2":
+            obj = l2_loss(model, imgs) / imgs.numel()
+        else:
+            obj = poisson_nll(model, imgs) / imgs.numel()
+        obj.backward()
+        opt.step()
+    with torch.no_grad():
+        qn = q / torch.linalg.vector_norm(q, dim=-1, keepdim=True)
+        qn = torch.where(qn[:, :1] < 0, -qn, qn)
+    return qn.detach()
