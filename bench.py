@@ -58,7 +58,7 @@ def parse():
     p.add_argument("--nbatches", type=int, default=1)
     p.add_argument("--no-extras", action="store_true", help="time the step only")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic"],
+    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic", "solve"],
                    help="slice: one slicing pass of the workload (for counter runs); nufft: setpts / execute "
                         "timing of both transforms; traffic: a 1 GiB copy (counter calibration) then one slicing pass")
     return p.parse_args()
@@ -302,7 +302,7 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         torch.cuda.synchronize()
         log(f"[bench] slicing pass done ({n_ref} references)")
         return
-    # nufft: setpts and execute of both transforms over the image q-vectors
+    # nufft / solve: the image q-vectors
     q = synth.random_quaternions(n_img, seed=1)
     x, y, z = AC.gen_nonuniform_positions(torch.from_numpy(q).to(dev), torch.from_numpy(s).to(dev), recip_extent=ext)
     x, y, z = (t.reshape(-1).contiguous() for t in (x, y, z))
@@ -318,6 +318,23 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps
 
+    if args.part == "solve":
+        # the autocorrelation solve of the workload: kernel and rhs spreads, then CG
+        imgs = alignment.Slicer(ac, s, ext).slice(q)
+        t = time.perf_counter()
+        acs, info = AC.solve_ac(0, s, ext, imgs.reshape(n_img, 1, -1), M, orientations=q, maxiter=20, rtol=0.0,
+                                rlambda=1e-4, return_info=True)
+        torch.cuda.synchronize()
+        t_total = time.perf_counter() - t
+        solver = AC.get_solver(M)
+        d = torch.rand(M ** 3, device=dev)
+        xx = torch.zeros(M ** 3, device=dev)
+        t_cg = tm(lambda: solver.cg(d, xx.zero_(), 1.0, 1e-4, 1e3, 20, 0.0), reps=3)
+        res = {"part": "solve", "M": M, "points": N, "solve_ms": t_total * 1e3, "iterations": info["iterations"],
+               "cg_ms_per_iteration": t_cg / 20 * 1e3}
+        log(f"[bench] solve {res}")
+        print(json.dumps(res), flush=True)
+        return
     res = {"points": N}
     for typ, modes, flags, label in ((2, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type2_real"),
                                      (2, M, CMTIP_GRID_REAL, "type2_complex"),

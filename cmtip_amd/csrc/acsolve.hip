@@ -28,13 +28,22 @@ struct cmtip_acsolver_s {
     DevBuf tmp2;     // M L^2 float2
     DevBuf r, p, q;  // M^3 float CG vectors
     DevBuf scal;     // FP64 CG scalars
+    DevBuf part;     // kRedBlocks doubles: per-block partial sums (fixed-order reductions)
+    DevBuf ctrl;     // int32 CG state: stop, converged, iterations, maxiter
+    int* host_state = nullptr;  // pinned host-mapped mirror: stop, converged, iterations, pad, rho (double)
+    int* host_state_dev = nullptr;
     const float* support = nullptr;
+    const int* cur_skip = nullptr;  // set while CG runs: passes do nothing once it stops
+    // CG runs on the solver's own stream (capturable; the legacy default stream is not),
+    // ordered after and before the caller's stream with two events
+    hipStream_t cg_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 namespace {
 
 constexpr int kT = 256;
-enum { S_RHO = 0, S_RHO_PREV = 1, S_PQ = 2, S_RR_NEW = 3, S_X2 = 4, S_B2 = 5, S_COUNT = 8 };
+enum { S_RHO = 0, S_RHO_PREV = 1, S_PQ = 2, S_RR_NEW = 3, S_ATOL = 4, S_B2 = 5, S_COUNT = 8 };
 
 unsigned nblk(int64_t n, int64_t cap = 8192) {
     int64_t b = (n + kT - 1) / kT;
@@ -107,45 +116,130 @@ __global__ void threshold_kernel(const double* __restrict__ in, float* __restric
 }
 
 // ---------------------------------------------------------------- CG vector kernels
+//
+// scipy.sparse.linalg.cg (iterative.py:375-422) with the loop state on the device:
+// ctrl = {stop, converged, iterations, maxiter}.  Every kernel of an iteration does
+// nothing once stop is set, so several iterations can be launched (or replayed from
+// a captured graph) before the host looks at the state, with exactly scipy's
+// iteration count: at the start of iteration i the loop ends when i == maxiter (info
+// maxiter) or when ||r|| < atol (info 0).  Reductions are two-stage: per-block
+// partial sums, then one block adds them in a fixed order, so results do not depend
+// on scheduling.
+
+constexpr int kRedBlocks = 512;
+enum { C_STOP = 0, C_CONV = 1, C_ITERS = 2, C_MAXITER = 3 };
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
 }
 
-__global__ void dot_kernel(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
-                           double* __restrict__ out) {
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ part) {
+    __shared__ double red[kT / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kT / 64; ++i) t += red[i];
+        part[blockIdx.x] = t;
+    }
+}
+
+__device__ __forceinline__ bool stopped(const int* ctrl) { return ctrl && ctrl[C_STOP]; }
+
+__global__ void dot_partial_kernel(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
+                                   double* __restrict__ part, const int* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
     double s = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kT;
     for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) s += (double)a[i] * (double)b[i];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, s);
+    block_partial(s, part);
+}
+
+// out = sum of the kRedBlocks partials (fixed order); mode 1 also rotates rho.
+__global__ void reduce_partials_kernel(const double* __restrict__ part, double* __restrict__ s, int slot, int mode,
+                                       const int* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    __shared__ double red[kRedBlocks];
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int h = kRedBlocks / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (mode == 1) {
+            s[S_RHO_PREV] = s[S_RHO];
+            s[S_RHO] = red[0];
+        } else {
+            s[slot] = red[0];
+        }
+    }
 }
 
 __global__ void set_scalars_kernel(double* __restrict__ s, int first, int count, double v) {
     if (threadIdx.x < count) s[first + threadIdx.x] = v;
 }
 
-// r = b - Ax (or r = b when Ax is null)
+// CG state init: stop when b == 0 (scipy returns b itself), atol = rtol ||b||
+__global__ void cg_init_kernel(double* __restrict__ s, int* __restrict__ ctrl, int maxiter, double rtol) {
+    if (threadIdx.x != 0) return;
+    const double b2 = s[S_B2];
+    s[S_ATOL] = rtol * sqrt(b2);
+    ctrl[C_STOP] = b2 == 0.0 ? 1 : 0;
+    ctrl[C_CONV] = b2 == 0.0 ? 1 : 0;
+    ctrl[C_ITERS] = 0;
+    ctrl[C_MAXITER] = maxiter;
+}
+
+// start of an iteration: loop exhausted or converged?  else count it
+__global__ void cg_check_kernel(const double* __restrict__ s, int* __restrict__ ctrl) {
+    if (threadIdx.x != 0 || ctrl[C_STOP]) return;
+    if (ctrl[C_ITERS] >= ctrl[C_MAXITER]) {
+        ctrl[C_STOP] = 1;
+        return;
+    }
+    if (sqrt(s[S_RHO]) < s[S_ATOL]) {
+        ctrl[C_CONV] = 1;
+        ctrl[C_STOP] = 1;
+        return;
+    }
+    ctrl[C_ITERS] += 1;
+}
+
+// r = b - Ax
 __global__ void residual_kernel(const float* __restrict__ b, const float* __restrict__ ax, float* __restrict__ r,
-                                int64_t n) {
+                                int64_t n, const int* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
     const int64_t stride = (int64_t)gridDim.x * kT;
-    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride)
-        r[i] = ax ? b[i] - ax[i] : b[i];
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) r[i] = b[i] - ax[i];
+}
+
+// x = b when b == 0 (the early return of scipy's cg)
+__global__ void copy_if_zero_rhs_kernel(const float* __restrict__ b, float* __restrict__ x, int64_t n,
+                                        const double* __restrict__ s) {
+    if (s[S_B2] != 0.0) return;
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) x[i] = b[i];
 }
 
 // p = r (first iteration) or p = beta p + r with beta = rho / rho_prev
 __global__ void update_p_kernel(const float* __restrict__ r, float* __restrict__ p, const double* __restrict__ s,
-                                int first, int64_t n) {
+                                int64_t n, const int* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    const bool first = ctrl[C_ITERS] == 1;
     const float beta = first ? 0.0f : (float)(s[S_RHO] / s[S_RHO_PREV]);
     const int64_t stride = (int64_t)gridDim.x * kT;
     for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride)
         p[i] = first ? r[i] : beta * p[i] + r[i];
 }
 
-// alpha = rho / (p.q); x += alpha p; r -= alpha q; accumulate r.r into s[S_RR_NEW]
+// alpha = rho / (p.q); x += alpha p; r -= alpha q; per-block partials of r.r
 __global__ void update_xr_kernel(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
-                                 const float* __restrict__ q, double* __restrict__ s, int64_t n) {
+                                 const float* __restrict__ q, const double* __restrict__ s, int64_t n,
+                                 double* __restrict__ part, const int* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
     const float alpha = (float)(s[S_RHO] / s[S_PQ]);
     double acc = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kT;
@@ -155,16 +249,16 @@ __global__ void update_xr_kernel(float* __restrict__ x, float* __restrict__ r, c
         r[i] = rn;
         acc += (double)rn * (double)rn;
     }
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(s + S_RR_NEW, acc);
+    block_partial(acc, part);
 }
 
-// rho_prev = rho; rho = rr_new
-__global__ void rotate_scalars_kernel(double* __restrict__ s) {
-    if (threadIdx.x == 0) {
-        s[S_RHO_PREV] = s[S_RHO];
-        s[S_RHO] = s[S_RR_NEW];
-    }
+// copy the loop state to pinned host memory (read by the host after a sync)
+__global__ void publish_kernel(const int* __restrict__ ctrl, const double* __restrict__ s, int* __restrict__ host) {
+    if (threadIdx.x != 0) return;
+    host[0] = ctrl[C_STOP];
+    host[1] = ctrl[C_CONV];
+    host[2] = ctrl[C_ITERS];
+    *reinterpret_cast<double*>(host + 4) = s[S_RHO];
 }
 
 bool valid(cmtip_acsolver s) {
@@ -201,6 +295,7 @@ int conv_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale,
     d.nu = M;
     d.nv = M;
     d.sign = -1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
     // 2: axis 1, lines (i0, p2)
     d = AxisDesc();
@@ -218,6 +313,7 @@ int conv_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale,
     d.nu = M;
     d.nv = L;
     d.sign = -1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
     // 3: axis 0 forward, x kernel spectrum, inverse; lines (p1, p2); in place
     d = AxisDesc();
@@ -240,6 +336,7 @@ int conv_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale,
     d.mid_su = L;
     d.mid_sv = 1;
     d.mid_sa = LL;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
     // 4: inverse axis 1, lines (i0, p2), keep positions [0, M)
     d = AxisDesc();
@@ -256,6 +353,7 @@ int conv_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale,
     d.nu = M;
     d.nv = L;
     d.sign = +1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
     // 5: inverse axis 2, lines (i0, i1), real part, crop, support, scale, axpy
     d = AxisDesc();
@@ -277,6 +375,7 @@ int conv_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale,
     d.nu = M;
     d.nv = M;
     d.sign = +1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
     return CMTIP_OK;
 }
@@ -309,6 +408,7 @@ int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, 
     d.nu = M;
     d.nv = M;
     d.sign = -1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
     // 2: axis 1
     d = AxisDesc();
@@ -325,6 +425,7 @@ int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, 
     d.nu = M;
     d.nv = M;
     d.sign = -1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
     // 3: axis 0 forward, x mask, inverse (in place)
     d = AxisDesc();
@@ -345,6 +446,7 @@ int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, 
     d.mid_su = M;
     d.mid_sv = 1;
     d.mid_sa = MM;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
     // 4: inverse axis 1
     d = AxisDesc();
@@ -361,6 +463,7 @@ int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, 
     d.nu = M;
     d.nv = M;
     d.sign = +1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
     // 5: inverse axis 2, real part, support, scale, axpy
     d = AxisDesc();
@@ -382,6 +485,7 @@ int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, 
     d.nu = M;
     d.nv = M;
     d.sign = +1;
+    d.skip = s->cur_skip;
     CMTIP_TRY(fft_axis_pass(s->fftM, d, st));
     return CMTIP_OK;
 }
@@ -395,8 +499,12 @@ int matvec(cmtip_acsolver_s* s, const float* u, float* out, double alpha, double
     return CMTIP_OK;
 }
 
-int dot(const float* a, const float* b, int64_t n, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(dot_kernel, dim3(nblk(n, 1024)), dim3(kT), 0, st, a, b, n, out);
+// out = a.b in FP64 with a fixed summation order
+int dot(cmtip_acsolver_s* sv, const float* a, const float* b, int64_t n, double* scal, int slot, hipStream_t st,
+        const int* ctrl) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(kRedBlocks), dim3(kT), 0, st, a, b, n, sv->part.as<double>(), ctrl);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kRedBlocks), 0, st, sv->part.as<const double>(), scal,
+                       slot, 0, ctrl);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -434,6 +542,23 @@ extern "C" int cmtip_acsolver_create(int M, int L, cmtip_acsolver* out) {
     if (st == CMTIP_OK) st = s->p.ensure(sizeof(float) * (size_t)M3);
     if (st == CMTIP_OK) st = s->q.ensure(sizeof(float) * (size_t)M3);
     if (st == CMTIP_OK) st = s->scal.ensure(sizeof(double) * S_COUNT);
+    if (st == CMTIP_OK) st = s->part.ensure(sizeof(double) * kRedBlocks);
+    if (st == CMTIP_OK) st = s->ctrl.ensure(sizeof(int) * 4);
+    if (st == CMTIP_OK) {
+        void* h = nullptr;
+        if (hipHostMalloc(&h, 64, hipHostMallocMapped) != hipSuccess ||
+            hipHostGetDevicePointer((void**)&s->host_state_dev, h, 0) != hipSuccess) {
+            set_error("acsolver_create: pinned host state allocation failed");
+            st = CMTIP_ERR_ALLOC;
+        }
+        s->host_state = (int*)h;
+    }
+    if (st == CMTIP_OK && (hipStreamCreateWithFlags(&s->cg_stream, hipStreamNonBlocking) != hipSuccess ||
+                           hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming) != hipSuccess ||
+                           hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming) != hipSuccess)) {
+        set_error("acsolver_create: stream / event creation failed");
+        st = CMTIP_ERR_HIP;
+    }
     if (st == CMTIP_OK) {
         DevBuf arr;
         st = arr.ensure(sizeof(float) * (size_t)M3);
@@ -503,6 +628,12 @@ extern "C" int cmtip_acsolver_destroy(cmtip_acsolver s) {
     s->p.release();
     s->q.release();
     s->scal.release();
+    s->part.release();
+    s->ctrl.release();
+    if (s->host_state) (void)hipHostFree(s->host_state);
+    if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+    if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+    if (s->cg_stream) (void)hipStreamDestroy(s->cg_stream);
     delete s;
     return CMTIP_OK;
 }
@@ -674,10 +805,35 @@ extern "C" int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* ou
     return matvec(s, u, out, alpha, rho, phi, as_stream(stream));
 }
 
-static int read_scalar(const double* dev, double* host, hipStream_t st) {
-    CMTIP_HIP(hipMemcpyAsync(host, dev, sizeof(double), hipMemcpyDeviceToHost, st));
-    CMTIP_HIP(hipStreamSynchronize(st));
+// One CG iteration (every kernel a no-op once ctrl[C_STOP] is set).
+static int cg_iteration(cmtip_acsolver_s* s, const float* d, float* x, double alpha, double rho, double phi,
+                        hipStream_t st) {
+    const int64_t n = (int64_t)s->M * s->M * s->M;
+    double* sc = s->scal.as<double>();
+    int* ctrl = s->ctrl.as<int>();
+    float* r = s->r.as<float>();
+    float* p = s->p.as<float>();
+    float* q = s->q.as<float>();
+    (void)d;
+    hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(64), 0, st, (const double*)sc, ctrl);
+    hipLaunchKernelGGL(update_p_kernel, dim3(nblk(n, 2048)), dim3(kT), 0, st, (const float*)r, p, (const double*)sc,
+                       n, (const int*)ctrl);
+    CMTIP_LAUNCH_CHECK();
+    CMTIP_TRY(matvec(s, p, q, alpha, rho, phi, st));
+    CMTIP_TRY(dot(s, p, q, n, sc, S_PQ, st, ctrl));
+    hipLaunchKernelGGL(update_xr_kernel, dim3(kRedBlocks), dim3(kT), 0, st, x, r, (const float*)p, (const float*)q,
+                       (const double*)sc, n, s->part.as<double>(), (const int*)ctrl);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kRedBlocks), 0, st, s->part.as<const double>(), sc, 0, 1,
+                       (const int*)ctrl);
+    CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
+}
+
+constexpr int kCgGraphIters = 5;
+
+static bool use_graphs() {
+    const char* e = getenv("CMTIP_CG_GRAPH");
+    return !(e && e[0] == '0');
 }
 
 extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
@@ -687,60 +843,81 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
         set_error("acsolver_cg: bad arguments");
         return CMTIP_ERR_ARG;
     }
-    hipStream_t st = as_stream(stream);
+    hipStream_t caller = as_stream(stream);
+    hipStream_t st = s->cg_stream;
+    CMTIP_HIP(hipEventRecord(s->ev_in, caller));
+    CMTIP_HIP(hipStreamWaitEvent(st, s->ev_in, 0));
     const int64_t n = (int64_t)s->M * s->M * s->M;
     double* sc = s->scal.as<double>();
-    float* r = s->r.as<float>();
-    float* p = s->p.as<float>();
-    float* q = s->q.as<float>();
+    int* ctrl = s->ctrl.as<int>();
     const unsigned g = nblk(n, 2048);
-    int count = 0, inf = maxiter;
 
-    // bnrm2 = ||b||; x.any()
+    // b = d; ||b||, atol, early exit for b == 0; r = b - A x (A 0 = 0 exactly, so this is
+    // scipy's "b - A x if x.any() else b"); rho = r.r
+    s->cur_skip = nullptr;
     CMTIP_TRY(set_scalars(sc, 0, S_COUNT, 0.0, st));
-    CMTIP_TRY(dot(d, d, n, sc + S_B2, st));
-    CMTIP_TRY(dot(x, x, n, sc + S_X2, st));
-    double b2 = 0.0, x2 = 0.0;
-    CMTIP_TRY(read_scalar(sc + S_B2, &b2, st));
-    CMTIP_TRY(read_scalar(sc + S_X2, &x2, st));
-    const double bnrm = sqrt(b2);
-    const double atol = rtol * bnrm;
-    if (bnrm == 0.0) {
-        // scipy returns b itself
-        CMTIP_HIP(hipMemcpyAsync(x, d, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-        if (iters) *iters = 0;
-        if (info) *info = 0;
-        return CMTIP_OK;
-    }
-    if (x2 > 0.0) {
-        CMTIP_TRY(matvec(s, x, q, alpha, rho, phi, st));
-        hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(kT), 0, st, d, (const float*)q, r, n);
-    } else {
-        hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(kT), 0, st, d, (const float*)nullptr, r, n);
-    }
+    CMTIP_TRY(dot(s, d, d, n, sc, S_B2, st, nullptr));
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, sc, ctrl, maxiter, rtol);
+    hipLaunchKernelGGL(copy_if_zero_rhs_kernel, dim3(g), dim3(kT), 0, st, d, x, n, (const double*)sc);
     CMTIP_LAUNCH_CHECK();
-    CMTIP_TRY(dot(r, r, n, sc + S_RHO, st));
-    double rr = 0.0;
-    CMTIP_TRY(read_scalar(sc + S_RHO, &rr, st));
-    for (int it = 0; it < maxiter; ++it) {
-        if (sqrt(rr) < atol) {
-            inf = 0;
+    s->cur_skip = ctrl;
+    int rc = matvec(s, x, s->q.as<float>(), alpha, rho, phi, st);
+    if (rc == CMTIP_OK) {
+        hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(kT), 0, st, d, (const float*)s->q.as<float>(),
+                           s->r.as<float>(), n, (const int*)ctrl);
+        rc = dot(s, s->r.as<const float>(), s->r.as<const float>(), n, sc, S_RHO, st, ctrl);
+    }
+    // iterations: replay a captured block of kCgGraphIters iterations until the device stops
+    hipGraphExec_t exec = nullptr;
+    if (rc == CMTIP_OK && use_graphs() && maxiter > 0) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            int crc = CMTIP_OK;
+            for (int k = 0; k < kCgGraphIters && crc == CMTIP_OK; ++k) crc = cg_iteration(s, d, x, alpha, rho, phi, st);
+            hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
+                               s->host_state_dev);
+            const hipError_t e = hipStreamEndCapture(st, &graph);
+            if (crc != CMTIP_OK || e != hipSuccess ||
+                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                exec = nullptr;
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();  // a failed capture falls back to direct launches below
+            set_error("");
+        }
+    }
+    volatile int* hs = s->host_state;
+    for (int done = (maxiter == 0) ? 1 : 0, rounds = 0; rc == CMTIP_OK; ++rounds) {
+        if (exec) {
+            if (hipGraphLaunch(exec, st) != hipSuccess) {
+                set_error("acsolver_cg: graph launch failed");
+                rc = CMTIP_ERR_HIP;
+                break;
+            }
+        } else {
+            for (int k = 0; k < kCgGraphIters && rc == CMTIP_OK; ++k) rc = cg_iteration(s, d, x, alpha, rho, phi, st);
+            hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
+                               s->host_state_dev);
+        }
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            set_error("acsolver_cg: stream synchronisation failed");
+            rc = CMTIP_ERR_HIP;
             break;
         }
-        hipLaunchKernelGGL(update_p_kernel, dim3(g), dim3(kT), 0, st, (const float*)r, p, (const double*)sc,
-                           it == 0 ? 1 : 0, n);
-        CMTIP_LAUNCH_CHECK();
-        CMTIP_TRY(matvec(s, p, q, alpha, rho, phi, st));
-        CMTIP_TRY(set_scalars(sc, S_PQ, 2, 0.0, st));
-        CMTIP_TRY(dot(p, q, n, sc + S_PQ, st));
-        hipLaunchKernelGGL(update_xr_kernel, dim3(g), dim3(kT), 0, st, x, r, (const float*)p, (const float*)q, sc,
-                           n);
-        hipLaunchKernelGGL(rotate_scalars_kernel, dim3(1), dim3(64), 0, st, sc);
-        CMTIP_LAUNCH_CHECK();
-        ++count;
-        CMTIP_TRY(read_scalar(sc + S_RHO, &rr, st));
+        if (done || hs[0] || rounds > maxiter / kCgGraphIters + 2) break;
     }
-    if (iters) *iters = count;
-    if (info) *info = inf;
+    if (exec) (void)hipGraphExecDestroy(exec);
+    s->cur_skip = nullptr;
+    if (hipEventRecord(s->ev_out, st) != hipSuccess || hipStreamWaitEvent(caller, s->ev_out, 0) != hipSuccess) {
+        set_error("acsolver_cg: stream ordering failed");
+        if (rc == CMTIP_OK) rc = CMTIP_ERR_HIP;
+    }
+    if (rc != CMTIP_OK) return rc;
+    if (maxiter == 0) {
+        hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
+                           s->host_state_dev);
+        CMTIP_HIP(hipStreamSynchronize(st));
+    }
+    if (iters) *iters = hs[2];
+    if (info) *info = hs[1] ? 0 : maxiter;
     return CMTIP_OK;
 }

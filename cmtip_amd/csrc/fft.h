@@ -65,6 +65,10 @@ struct AxisDesc {
     const float2* mid_c = nullptr;
     const float* mid_r = nullptr;
     int64_t mid_sa = 0, mid_su = 0, mid_sv = 0;
+
+    // Optional device flag: when *skip != 0 the pass does nothing (lets a captured
+    // sequence of passes be switched off from the device, e.g. after CG converged).
+    const int* skip = nullptr;
 };
 
 int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream);

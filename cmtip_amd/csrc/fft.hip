@@ -9,6 +9,7 @@
 // autocorrelation edge lengths whose Fourier-corner projection needs an exact
 // M-point DFT) runs a direct O(R^2) DFT stage.
 #include <math.h>
+#include <stdlib.h>
 
 #include <vector>
 
@@ -19,7 +20,18 @@ namespace cmtip {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsBudget = 64 * 1024;
+// LDS per workgroup for the two line buffers.  Strided passes want many lines per
+// workgroup (consecutive lines are contiguous in memory, so more lines make longer
+// contiguous runs); CMTIP_FFT_LDS_KB overrides the budget for tuning.
+int lds_budget() {
+    static int b = 0;
+    if (!b) {
+        const char* e = getenv("CMTIP_FFT_LDS_KB");
+        const int kb = e ? atoi(e) : 64;
+        b = (kb >= 16 && kb <= 160 ? kb : 64) * 1024;
+    }
+    return b;
+}
 
 struct PassArgs {
     AxisDesc d;
@@ -176,6 +188,7 @@ __device__ __forceinline__ void split_elem(const PassArgs& a, uint32_t e, uint32
 
 template <bool IN_REAL, int OUT_MODE, bool MID>
 __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
+    if (a.d.skip && *a.d.skip) return;
     extern __shared__ float2 lds[];
     float2* b0 = lds;
     float2* b1 = lds + a.lb * a.ldl;
@@ -339,7 +352,8 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     // lines per workgroup: a power of two that keeps two LDS line buffers under budget
     const int ldl = plan.n | 1;
     int lb = 1;
-    while (lb < 64 && 2 * (2 * lb) * ldl * (int)sizeof(float2) <= kLdsBudget) lb *= 2;
+    const int budget = lds_budget();
+    while (lb < 64 && 2 * (2 * lb) * ldl * (int)sizeof(float2) <= budget) lb *= 2;
     int lbshift = 0;
     while ((1 << lbshift) < lb) ++lbshift;
     a.lb = lb;
@@ -361,8 +375,16 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     const size_t lds = (size_t)2 * lb * ldl * sizeof(float2);
     const unsigned grid = (unsigned)((nlines + lb - 1) / lb);
     const bool mid = d.mid_c || d.mid_r;
-#define CMTIP_AXIS_LAUNCH(IR, OM, MD) \
-    hipLaunchKernelGGL((axis_pass_kernel<IR, OM, MD>), dim3(grid), dim3(kThreads), lds, stream, a)
+#define CMTIP_AXIS_LAUNCH(IR, OM, MD)                                                                       \
+    do {                                                                                                    \
+        static bool attr_set_ = false;                                                                      \
+        if (!attr_set_ && lds > 64 * 1024) {                                                                \
+            (void)hipFuncSetAttribute((const void*)axis_pass_kernel<IR, OM, MD>,                            \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);              \
+            attr_set_ = true;                                                                               \
+        }                                                                                                   \
+        hipLaunchKernelGGL((axis_pass_kernel<IR, OM, MD>), dim3(grid), dim3(kThreads), lds, stream, a);    \
+    } while (0)
     if (d.in_real) {
         if (d.out_mode == 0) {
             if (mid) CMTIP_AXIS_LAUNCH(true, 0, true); else CMTIP_AXIS_LAUNCH(true, 0, false);

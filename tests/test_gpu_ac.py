@@ -121,3 +121,41 @@ def test_cg_iterates_track_oracle():
         want, its, _, _, _ = OAC.solve(q, s, ext, data, M, maxiter=k)
         assert info["iterations"] == its == k
         assert rel_l2(got, want) < 1e-4
+
+
+def _solver_system(M=11):
+    s, ext, q, acm, x, y, z = _geometry(M, n_img=80)
+    data = O.nufft2(acm.astype(np.float64), x, y, z, eps=1e-12).real
+    sup = np.ones((M,) * 3, np.float32)
+    rl = M ** 3 / data.size / 1000
+    W, d = A.setup_linops(torch.as_tensor(x), torch.as_tensor(y), torch.as_tensor(z), data.astype(np.float32), sup,
+                          np.ones(data.size, np.float32), np.zeros(M ** 3), M, M ** 3, data.size, ext, 1, rl, 1e3,
+                          scaled=True)
+    return W, d, rl
+
+
+def test_cg_graph_replay_matches_direct_launches(monkeypatch):
+    W, d, rl = _solver_system()
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("CMTIP_CG_GRAPH", mode)
+        x = torch.zeros_like(d)
+        say(f"cg graph={mode}")
+        out[mode] = (W.solver.cg(d, x, 1.0, rl, 1e3, 17, 1e-5), x.clone())
+    assert out["1"][0] == out["0"][0]
+    assert torch.equal(out["1"][1], out["0"][1])
+
+
+def test_cg_edge_cases():
+    W, d, rl = _solver_system()
+    say("cg b = 0 and maxiter = 0")
+    x = torch.full_like(d, 3.0)
+    it, info = W.solver.cg(torch.zeros_like(d), x, 1.0, rl, 1e3, 10, 1e-5)
+    assert (it, info) == (0, 0) and bool((x == 0).all())  # scipy returns b itself
+    x = torch.full_like(d, 0.5)
+    it, info = W.solver.cg(d, x, 1.0, rl, 1e3, 0, 1e-5)
+    assert (it, info) == (0, 0)  # maxiter 0: no iteration, info = maxiter = 0
+    assert bool((x == 0.5).all())
+    x = torch.zeros_like(d)
+    it, info = W.solver.cg(d, x, 1.0, rl, 1e3, 3, 1e-12)
+    assert (it, info) == (3, 3)  # loop exhausted: info = maxiter
