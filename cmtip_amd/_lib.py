@@ -51,6 +51,10 @@ SIGNATURES = {
     "cmtip_eudist_f32": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_match_topk_bf16": [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _i, _vp, _vp],
     "cmtip_match_rescore": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _i, _vp, _vp, _vp],
+    "cmtip_image_stats_masked": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp],
+    "cmtip_match_prepare_masked": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "cmtip_center_rows_sq": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "cmtip_match_rescore_masked": [_vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _i, _vp, _vp, _vp],
     "cmtip_acsolver_create": [_i, _i, ctypes.POINTER(_vp)],
     "cmtip_acsolver_destroy": [_vp],
     "cmtip_ac_support": [_vp, _i, _vp, _vp],

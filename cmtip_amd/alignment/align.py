@@ -150,9 +150,11 @@ class _Matcher:
     """State shared by one alignment call: images, weights and per-image best keys.
 
     Model slices and images are centred on the mean image c before the cross term
-    (d2 is unchanged; the FP32 / BF16 expanded form then does not cancel)."""
+    (d2 is unchanged; the FP32 / BF16 expanded form then does not cancel).  With
+    per-image masks (1 = valid pixel) the cross term runs over 2P columns:
+    [y^2, y] . [-m w^2 / 2, m w^2 (b - c)] with y = gamma a - c."""
 
-    def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk):
+    def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk, masks=None):
         self.P = pixel_position_reciprocal.shape[-1]
         B = _arrays.to_dev(slices_, torch.float32).reshape(-1, self.P)
         self.B = B
@@ -164,16 +166,36 @@ class _Matcher:
         self.k = int(topk)
         dev = B.device
         self.c = torch.empty(self.P, dtype=torch.float32, device=dev)
+        self.bn = torch.empty(self.n_img, dtype=torch.float32, device=dev)
+        self.mask = None
+        if masks is not None:
+            self.mask = _arrays.to_dev(masks, torch.uint8).reshape(self.n_img, self.P).contiguous()
+            colsum = torch.empty(3 * self.P, dtype=torch.float64, device=dev)
+            mom = torch.empty(3, dtype=torch.float64, device=dev)
+            check(L.cmtip_image_stats_masked(ptr(B), ptr(self.mask), self.n_img, self.P, ptr(colsum), ptr(self.c),
+                                             ptr(mom), stream()), "cmtip_image_stats_masked")
+            count = mom[2].clamp(min=1.0)
+            mean = mom[0] / count
+            self.std_data = torch.sqrt(torch.clamp(mom[1] / count - mean * mean, min=0.0))
+            self.K = 2 * self.P
+            self.Bw = torch.empty((self.n_img, self.K), dtype=torch.float32, device=dev) \
+                if precision == "fp32" else None
+            self.Bw16 = torch.empty((self.n_img, self.K), dtype=torch.int16, device=dev) \
+                if precision == "bf16" else None
+            check(L.cmtip_match_prepare_masked(ptr(B), ptr(self.mask), self.n_img, self.P, ptr(self.w2), ptr(self.c),
+                                               ptr(self.Bw), ptr(self.Bw16), ptr(self.bn), stream()),
+                  "cmtip_match_prepare_masked")
+            return
         colsum = torch.empty(2 * self.P, dtype=torch.float64, device=dev)
         mom = torch.empty(2, dtype=torch.float64, device=dev)
         check(L.cmtip_image_stats(ptr(B), self.n_img, self.P, ptr(colsum), ptr(self.c), ptr(mom), stream()),
               "cmtip_image_stats")
+        self.K = self.P
         self.Bw = torch.empty_like(B) if precision == "fp32" else None
         self.Bw16 = torch.empty(B.shape, dtype=torch.int16, device=dev) if precision == "bf16" else None
         check(L.cmtip_match_prepare_images_centered(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.c),
                                                     ptr(self.Bw), ptr(self.Bw16), stream()),
               "cmtip_match_prepare_images_centered")
-        self.bn = torch.empty(self.n_img, dtype=torch.float32, device=dev)
         check(L.cmtip_center_rows(ptr(B), self.n_img, self.P, None, ptr(self.c), ptr(self.w2), None, None,
                                   ptr(self.bn), stream()), "cmtip_center_rows")
         self.std_data = _std_from_moments(mom, B.numel())
@@ -184,19 +206,20 @@ class _Matcher:
         quot = n_ref // nbatches
         dev = self.B.device
         L = _lib.lib()
+        masked = self.mask is not None
         best = torch.full((self.n_img,), EMPTY_KEY, dtype=torch.int64, device=dev)
-        an = torch.empty(quot, dtype=torch.float32, device=dev)
+        an = torch.zeros(quot, dtype=torch.float32, device=dev)
+        A_all = torch.empty((n_ref if self.precision == "bf16" else quot, self.P), dtype=torch.float32, device=dev)
         if self.precision == "bf16":
             cand = torch.full((self.n_img, self.k), EMPTY_KEY, dtype=torch.int64, device=dev)
-            A_all = torch.empty((n_ref, self.P), dtype=torch.float32, device=dev)
             gamma_ref = torch.empty(n_ref, dtype=torch.float64, device=dev)
-            A16 = torch.empty((quot, self.P), dtype=torch.int16, device=dev)
-        else:
-            A = torch.empty((quot, self.P), dtype=torch.float32, device=dev)
+            A16 = torch.empty((quot, self.K), dtype=torch.int16, device=dev)
+        elif masked:
+            A2 = torch.empty((quot, self.K), dtype=torch.float32, device=dev)
         for nb in range(nbatches):
             start, end = nb * quot, (nb + 1) * quot
             mom = torch.zeros(2, dtype=torch.float64, device=dev)
-            Ab = A_all[start:end] if self.precision == "bf16" else A
+            Ab = A_all[start:end] if self.precision == "bf16" else A_all
             slicer.slice(ref[start:end], out=Ab, moments=mom)
             # data/model std ratio (align.py:122-124), rounded to float32 as numpy keeps it
             gamma = (self.std_data / _std_from_moments(mom, Ab.numel())).float().double().reshape(1)
@@ -204,21 +227,31 @@ class _Matcher:
                 print(f"Data/Model std ratio: {float(gamma):.7g}.", flush=True)
             if self.precision == "bf16":
                 gamma_ref[start:end] = gamma
-                check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), None,
-                                          ptr(A16), ptr(an), stream()), "cmtip_center_rows")
-                check(L.cmtip_match_topk_bf16(ptr(A16), quot, start, ptr(self.Bw16), self.n_img, self.P, ptr(an),
+                if masked:
+                    check(L.cmtip_center_rows_sq(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), None, ptr(A16),
+                                                 stream()), "cmtip_center_rows_sq")
+                else:
+                    check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), None,
+                                              ptr(A16), ptr(an), stream()), "cmtip_center_rows")
+                check(L.cmtip_match_topk_bf16(ptr(A16), quot, start, ptr(self.Bw16), self.n_img, self.K, ptr(an),
                                               ptr(self.bn), None, self.k, ptr(cand), stream()),
                       "cmtip_match_topk_bf16")
             else:
-                check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), ptr(Ab),
-                                          None, ptr(an), stream()), "cmtip_center_rows")
-                check(L.cmtip_match_argmin_f32(ptr(Ab), quot, start, ptr(self.Bw), self.n_img, self.P, ptr(an),
+                if masked:
+                    check(L.cmtip_center_rows_sq(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(A2), None,
+                                                 stream()), "cmtip_center_rows_sq")
+                    Aop = A2
+                else:
+                    check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), ptr(Ab),
+                                              None, ptr(an), stream()), "cmtip_center_rows")
+                    Aop = Ab
+                check(L.cmtip_match_argmin_f32(ptr(Aop), quot, start, ptr(self.Bw), self.n_img, self.K, ptr(an),
                                                ptr(self.bn), None, ptr(best), stream()),
                       "cmtip_match_argmin_f32")
         if self.precision == "bf16":
-            check(L.cmtip_match_rescore(ptr(A_all), n_ref, ptr(self.B), self.n_img, self.P, ptr(self.w2),
-                                        ptr(gamma_ref), self.k, ptr(cand), ptr(best), stream()),
-                  "cmtip_match_rescore")
+            check(L.cmtip_match_rescore_masked(ptr(A_all), n_ref, ptr(self.B), ptr(self.mask), self.n_img, self.P,
+                                               ptr(self.w2), ptr(gamma_ref), self.k, ptr(cand), ptr(best), stream()),
+                  "cmtip_match_rescore_masked")
         return best
 
 
@@ -257,18 +290,21 @@ def key_d2(best):
 
 def match_orientations(generation, pixel_position_reciprocal, reciprocal_extent, slices_, ac, n_ref_orientations,
                        nbatches=1, order=0, use_gpu=False, true_orientations=None, *, ref_orientations=None,
-                       precision="fp32", topk=8, eps=DEFAULT_EPS, seed=0, verbose=False, return_index=False):
+                       precision="fp32", topk=8, eps=DEFAULT_EPS, seed=0, verbose=False, return_index=False,
+                       masks=None):
     """Quaternions (n_images, 4) float32 of the best-matching reference slices (align.py:68-132).
 
     precision "fp32": cross term in FP32 on MFMA, argmin fused.  "bf16": cross term in
     BF16 on MFMA with FP32 accumulation, the ``topk`` best per image rescored in FP32.
+    masks (n_images, P), 1 = valid pixel: distances over each image's valid pixels only
+    (the data std is then taken over valid pixels); not in the reference.
     """
     assert precision in ("fp32", "bf16")
     n_ref_orientations = _round_up_refs(n_ref_orientations, nbatches)
     ref = _ref_set(n_ref_orientations, ref_orientations, true_orientations, seed)
     as_torch = _arrays.is_torch(slices_, ac)
     slicer = Slicer(ac, pixel_position_reciprocal, reciprocal_extent, eps)
-    m = _Matcher(pixel_position_reciprocal, slices_, order, precision, topk)
+    m = _Matcher(pixel_position_reciprocal, slices_, order, precision, topk, masks)
     ref_dev = torch.from_numpy(ref).to(m.B.device)
     best = m.run(slicer, ref_dev, nbatches, verbose)
     index = key_index(best)

@@ -190,6 +190,132 @@ __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t coun
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) p[i] = v;
 }
 
+// ---------------------------------------------------------------- per-image pixel masks
+//
+// With a mask m_i (1 = valid pixel) the distance is
+//     d2[r,i] = sum_p m_ip w_p^2 ((gamma a_rp - c_p) - (b_ip - c_p))^2
+//             = sum_p [y_rp^2, y_rp] . [m_ip w_p^2, -2 m_ip w_p^2 (b_ip - c_p)] + bn_i,  y = gamma a - c
+// i.e. one cross term over 2P columns: the model rows become [y^2, y] and the image
+// rows [-m w^2 / 2, m w^2 (b - c)], so the matching kernels run unchanged with an = 0
+// (they form -2 <a', b'> + bn).
+
+// colsum[p] = sum_r m x, colsum[P + p] = sum_r m x^2, colsum[2P + p] = sum_r m (FP64, fixed order)
+__global__ void __launch_bounds__(64 * kStatWaves) column_sums_masked_kernel(const float* __restrict__ X,
+                                                                             const uint8_t* __restrict__ mask,
+                                                                             int64_t rows, int64_t P,
+                                                                             double* __restrict__ colsum) {
+    __shared__ double red[3][kStatWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+    double s1 = 0.0, s2 = 0.0, s0 = 0.0;
+    if (p < P) {
+        for (int64_t r = wave; r < rows; r += kStatWaves) {
+            const int64_t o = r * P + p;
+            if (mask[o]) {
+                const double v = (double)X[o];
+                s1 += v;
+                s2 += v * v;
+                s0 += 1.0;
+            }
+        }
+    }
+    red[0][wave][lane] = s1;
+    red[1][wave][lane] = s2;
+    red[2][wave][lane] = s0;
+    __syncthreads();
+    if (wave == 0 && p < P) {
+        double t1 = 0.0, t2 = 0.0, t0 = 0.0;
+        for (int w = 0; w < kStatWaves; ++w) {
+            t1 += red[0][w][lane];
+            t2 += red[1][w][lane];
+            t0 += red[2][w][lane];
+        }
+        colsum[p] = t1;
+        colsum[P + p] = t2;
+        colsum[2 * P + p] = t0;
+    }
+}
+
+// c = (sum m x) / max(sum m, 1); mom = (sum m x, sum m x^2, sum m) over everything
+__global__ void finish_stats_masked_kernel(const double* __restrict__ colsum, int64_t P, float* __restrict__ c,
+                                           double* __restrict__ mom) {
+    __shared__ double red[3][kThreads];
+    double s1 = 0.0, s2 = 0.0, s0 = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+        const double a = colsum[p], cnt = colsum[2 * P + p];
+        s1 += a;
+        s2 += colsum[P + p];
+        s0 += cnt;
+        if (c) c[p] = (float)(a / (cnt > 0.0 ? cnt : 1.0));
+    }
+    red[0][threadIdx.x] = s1;
+    red[1][threadIdx.x] = s2;
+    red[2][threadIdx.x] = s0;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && mom) {
+        mom[0] = red[0][0];
+        mom[1] = red[1][0];
+        mom[2] = red[2][0];
+    }
+}
+
+// image rows -> [-m w^2 / 2, m w^2 (b - c)] (2P columns, float32 and/or bf16); bn = sum m w^2 (b - c)^2
+__global__ void prepare_masked_kernel(const float* __restrict__ B, const uint8_t* __restrict__ mask, int64_t P,
+                                      const float* __restrict__ w2, const float* __restrict__ c,
+                                      float* __restrict__ Bw, uint16_t* __restrict__ Bw16, float* __restrict__ bn) {
+    __shared__ double red[kThreads / 64];
+    const int64_t row = blockIdx.x;
+    double s = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+        const int64_t o = row * P + p;
+        const float m = mask[o] ? w2[p] : 0.0f;
+        const float d = B[o] - c[p];
+        const float lo = -0.5f * m, hi = m * d;
+        s += (double)m * (double)d * (double)d;
+        const int64_t q = row * 2 * P;
+        if (Bw) {
+            Bw[q + p] = lo;
+            Bw[q + P + p] = hi;
+        }
+        if (Bw16) {
+            Bw16[q + p] = f32_to_bf16(lo);
+            Bw16[q + P + p] = f32_to_bf16(hi);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+        bn[row] = (float)t;
+    }
+}
+
+// model rows -> [y^2, y] with y = gamma a - c (2P columns, float32 and/or bf16)
+__global__ void center_rows_sq_kernel(const float* __restrict__ X, int64_t P, const double* __restrict__ gamma,
+                                      const float* __restrict__ c, float* __restrict__ Y, uint16_t* __restrict__ Y16) {
+    const float g = gamma ? (float)gamma[0] : 1.0f;
+    const int64_t row = blockIdx.x;
+    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+        const float y = fmaf(g, X[row * P + p], -c[p]);
+        const int64_t q = row * 2 * P;
+        if (Y) {
+            Y[q + p] = y * y;
+            Y[q + P + p] = y;
+        }
+        if (Y16) {
+            Y16[q + p] = f32_to_bf16(y * y);
+            Y16[q + P + p] = f32_to_bf16(y);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- FP32 MFMA argmin
 //
 // 256 threads = 4 waves as 2 (refs) x 2 (images); each wave owns a 64 x 64 block
@@ -501,7 +627,8 @@ __global__ void __launch_bounds__(kThreads) rescore_kernel(const float* __restri
                                                            const float* __restrict__ w2,
                                                            const double* __restrict__ gamma_of_ref, int k,
                                                            const unsigned long long* __restrict__ cand,
-                                                           unsigned long long* __restrict__ best) {
+                                                           unsigned long long* __restrict__ best,
+                                                           const uint8_t* __restrict__ mask) {
     __shared__ unsigned long long red[kThreads / 64];
     const int64_t i = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -517,7 +644,8 @@ __global__ void __launch_bounds__(kThreads) rescore_kernel(const float* __restri
         float acc = 0.f;
         for (int64_t p = lane; p < P; p += 64) {
             const float dv = fmaf(g, a[p], -b[p]);
-            acc = fmaf(w2[p] * dv, dv, acc);
+            const float wp = (mask && !mask[i * P + p]) ? 0.0f : w2[p];
+            acc = fmaf(wp * dv, dv, acc);
         }
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
         mine = umin64(mine, pack_key(acc, r));
@@ -690,6 +818,14 @@ extern "C" int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t r
 extern "C" int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, const float* B, int64_t n_img,
                                    int64_t P, const float* w2, const double* gamma_of_ref, int k,
                                    const unsigned long long* cand, unsigned long long* best, void* stream) {
+    return cmtip_match_rescore_masked(A_all, n_ref_total, B, nullptr, n_img, P, w2, gamma_of_ref, k, cand, best,
+                                      stream);
+}
+
+extern "C" int cmtip_match_rescore_masked(const float* A_all, int64_t n_ref_total, const float* B,
+                                          const uint8_t* mask, int64_t n_img, int64_t P, const float* w2,
+                                          const double* gamma_of_ref, int k, const unsigned long long* cand,
+                                          unsigned long long* best, void* stream) {
     if (k < 1 || k > kTopK || n_img < 0 || P < 1 || n_ref_total < 0 ||
         (n_img > 0 && (!A_all || !B || !w2 || !gamma_of_ref || !cand || !best))) {
         set_error("match_rescore: bad arguments");
@@ -701,7 +837,49 @@ extern "C" int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, cons
         return CMTIP_ERR_ARG;
     }
     hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
-                       n_ref_total, B, P, w2, gamma_of_ref, k, cand, best);
+                       n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_image_stats_masked(const float* X, const uint8_t* mask, int64_t rows, int64_t P, double* colsum,
+                                        float* c, double* mom, void* stream) {
+    if (rows < 0 || P < 1 || !colsum || (rows > 0 && (!X || !mask))) {
+        set_error("image_stats_masked: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(column_sums_masked_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64 * kStatWaves), 0, st, X,
+                       mask, rows, P, colsum);
+    hipLaunchKernelGGL(finish_stats_masked_kernel, dim3(1), dim3(kThreads), 0, st, (const double*)colsum, P, c, mom);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_prepare_masked(const float* B, const uint8_t* mask, int64_t n_img, int64_t P,
+                                          const float* w2, const float* c, float* Bw2, uint16_t* Bw2_bf16, float* bn,
+                                          void* stream) {
+    if (n_img < 0 || P < 1 || n_img > 0x7fffffff ||
+        (n_img > 0 && (!B || !mask || !w2 || !c || !bn || (!Bw2 && !Bw2_bf16)))) {
+        set_error("match_prepare_masked: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_img == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(prepare_masked_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), B, mask, P,
+                       w2, c, Bw2, Bw2_bf16, bn);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_center_rows_sq(const float* X, int64_t rows, int64_t P, const double* gamma, const float* c,
+                                    float* Y2, uint16_t* Y2_bf16, void* stream) {
+    if (rows < 0 || P < 1 || rows > 0x7fffffff || (rows > 0 && (!X || !c || (!Y2 && !Y2_bf16)))) {
+        set_error("center_rows_sq: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (rows == 0) return CMTIP_OK;
+    hipLaunchKernelGGL(center_rows_sq_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P, gamma,
+                       c, Y2, Y2_bf16);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

@@ -181,6 +181,23 @@ int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, const float* B,
                         int64_t P, const float* w2, const double* gamma_of_ref, int k,
                         const unsigned long long* cand, unsigned long long* best, void* stream);
 
+/* Per-image pixel masks (uint8, n_img x P, 1 = valid; an extension for detectors with
+ * beamstops, absent from the reference).  d2 = sum_p m_ip w_p^2 (gamma a - b)^2 is one
+ * cross term over 2P columns: model rows [y^2, y] with y = gamma a - c
+ * (cmtip_center_rows_sq), image rows [-m w^2 / 2, m w^2 (b - c)] and bn = sum m w^2 (b - c)^2
+ * (cmtip_match_prepare_masked); the matching kernels then run with K = 2P and an = 0.
+ * cmtip_image_stats_masked: colsum (3P doubles workspace), c = masked column mean,
+ * mom = (sum m x, sum m x^2, sum m). */
+int cmtip_image_stats_masked(const float* X, const uint8_t* mask, int64_t rows, int64_t P, double* colsum,
+                             float* c, double* mom, void* stream);
+int cmtip_match_prepare_masked(const float* B, const uint8_t* mask, int64_t n_img, int64_t P, const float* w2,
+                               const float* c, float* Bw2, uint16_t* Bw2_bf16, float* bn, void* stream);
+int cmtip_center_rows_sq(const float* X, int64_t rows, int64_t P, const double* gamma, const float* c,
+                         float* Y2, uint16_t* Y2_bf16, void* stream);
+int cmtip_match_rescore_masked(const float* A_all, int64_t n_ref_total, const float* B, const uint8_t* mask,
+                               int64_t n_img, int64_t P, const float* w2, const double* gamma_of_ref, int k,
+                               const unsigned long long* cand, unsigned long long* best, void* stream);
+
 /* ---------------------------------------------------------------- autocorrelation solve
  * The W x = d system of cmtip/autocorrelation/autocorrelation.py:113-173 and its scipy
  * CG solve (:239).  A solver holds FFT plans for the M^3 grid and the L^3 Toeplitz

@@ -196,3 +196,45 @@ def test_realistic_dynamic_range_vs_fp64(precision):
     got = idx.cpu().numpy()
     assert ok.mean() > 0.9
     assert np.array_equal(got[ok], want[ok]), f"{(got[ok] != want[ok]).sum()} mismatches of {ok.sum()}"
+
+
+def _masked_oracle(ref, s, ext, imgs, ac, masks, nbatches):
+    """d2 = sum_p m w^2 (gamma a - b)^2 in FP64, gamma = masked data std / batch model std."""
+    P = s.shape[-1]
+    w2 = OA.generate_weights(s, 0).reshape(-1) ** 2
+    data = imgs.astype(np.float64)
+    mk = masks.astype(bool)
+    std_data = data[mk].std()
+    quot = len(ref) // nbatches
+    D = np.zeros((len(ref), len(imgs)))
+    for nb in range(nbatches):
+        a = OA.compute_slices(ref[nb * quot:(nb + 1) * quot], s, ext, ac, eps=1e-12).reshape(quot, P)
+        g = std_data / a.std()
+        diff = g * a[:, None, :] - data[None, :, :]
+        D[nb * quot:(nb + 1) * quot] = (mk[None] * w2 * diff * diff).sum(-1)
+    return D
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("side", [32, 24])
+def test_masked_matching_vs_oracle(precision, side):
+    # two detector binnings with random beamstop masks of uneven size per image
+    s, ext, ac, ref, _, imgs = _dataset(n_side=side, n_ref=200, n_img=40, seed=side)
+    rng = np.random.default_rng(side)
+    P = side * side
+    yy, xx = np.mgrid[:side, :side]
+    masks = np.ones((40, P), np.uint8)
+    for i in range(40):
+        cy, cx = rng.integers(side // 4, 3 * side // 4, 2)
+        r = rng.uniform(1, side / 4)
+        masks[i] = (((yy - cy) ** 2 + (xx - cx) ** 2) > r * r).ravel()
+        masks[i, rng.random(P) < 0.05] = 0
+    say(f"masked matching {precision} side={side}")
+    q, idx = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), nbatches=2, ref_orientations=ref,
+                                         precision=precision, return_index=True, masks=masks)
+    D = _masked_oracle(ref, s, ext, imgs, ac, masks, 2)
+    want = np.argmin(D, axis=0)
+    srt = np.sort(D, axis=0)
+    ok = (srt[1] - srt[0]) > 1e-4 * srt[0]
+    assert ok.mean() > 0.5
+    assert np.array_equal(np.asarray(idx)[ok], want[ok])
