@@ -37,6 +37,11 @@ SIGNATURES = {
     "cmtip_nufft_setpts": [_vp, _i64, _vp, _vp, _vp, _vp],
     "cmtip_nufft_execute": [_vp, _vp, _vp, _i, _vp, _f, _vp],
     "cmtip_nufft_destroy": [_vp],
+    "cmtip_nufft_set_bins": [_vp, _i, _i, _i],
+    "cmtip_nufft_get_bins": [_vp, _pi, _pi, _pi],
+    "cmtip_nufft_autotune": [_vp, _vp, _vp, _i, _vp, _f, _vp, _i, _i, _f, _vp, _pi, _vp],
+    "cmtip_slice_set_route": [_vp, _i],
+    "cmtip_slice_get_route": [_vp, _pi, _pi],
     "cmtip_rotate_coords": [_vp, _i64, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "cmtip_slice_prepare": [_vp, _vp, _vp, _vp],
     "cmtip_slice_real": [_vp, _vp, _i64, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],

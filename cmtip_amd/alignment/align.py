@@ -106,13 +106,15 @@ class Slicer:
 
     The fine grid is prepared once per autocorrelation; ``slice`` then writes
     Re(T2(ac)) at the rotated pixels of a batch of quaternions.  Pixels are processed
-    in ``pixel_order`` (the output layout does not change).
+    in ``pixel_order`` (the output layout does not change).  The library picks the
+    fused-tile or the binned route per detector geometry (``plan.set_slice_route``
+    fixes one); both give the same values.
     """
 
     def __init__(self, ac, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS, tile_pixels=True):
         self.ac = _arrays.to_dev(ac, torch.float32)
         M = self.ac.shape[0]
-        self.plan = get_plan(2, M, -1, eps)
+        self.plan = get_plan(2, M, -1, eps, tag="slice")
         self.s = _s64(pixel_position_reciprocal)
         self.P = self.s.shape[1]
         self.ext = _ext32(reciprocal_extent)

@@ -180,10 +180,12 @@ class WOperator:
 
 
 def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_extent, alambda, rlambda, flambda,
-                 use_gpu=False, use_recip_sym=True, *, L_embed=None, eps=DEFAULT_EPS, scaled=False):
+                 use_gpu=False, use_recip_sym=True, *, L_embed=None, eps=DEFAULT_EPS, scaled=False, autotune=False):
     """(W, d) of W x = d (autocorrelation.py:113-173); d is a float32 device vector.
 
-    With ``scaled`` the coordinates are already x = H / reciprocal_extent * pi."""
+    With ``scaled`` the coordinates are already x = H / reciprocal_extent * pi.  With
+    ``autotune`` the two spreads pick their bin and launch settings with the NUFFT
+    autotuner (first call per shape; later calls reuse the choice)."""
     if not use_recip_sym:
         raise NotImplementedError("the GPU solver keeps real vectors (use_recip_sym=True)")
     if scaled:
@@ -198,11 +200,12 @@ def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_
     solver.set_support(sup)
     solver.set_antisupport(torch.as_tensor(gen_F_antisupport(M), dtype=torch.float32))
     # kernel: real type-1 of ones at M_ups modes over (2M)^3 (:141-148; 1/(2M)^3 also when L > 2M)
-    ugrid_conv = _adjoint(None, H_, K_, L_, Lm, True, None, eps, scale=1.0 / float(2 * M) ** 3, ones=True)
+    ugrid_conv = _adjoint(None, H_, K_, L_, Lm, True, None, eps, scale=1.0 / float(2 * M) ** 3, ones=True,
+                          autotune=autotune)
     solver.set_kernel(ugrid_conv)
     # rhs: d = alambda * A^H (data * weights) + rlambda * x0 (:162-171)
     nuvect_Db = _arrays.to_dev(data, torch.float32).reshape(-1) * _arrays.to_dev(weights, torch.float32).reshape(-1)
-    uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps).reshape(-1)
+    uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps, autotune=autotune).reshape(-1)
     if bool(torch.isnan(uvect_ADb).any()):
         print("Warning: nans in the adjoint calculation; intensities may be too large", flush=True)
     x0d = _arrays.to_dev(x0, torch.float32).reshape(-1)
@@ -212,11 +215,12 @@ def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_
 
 def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, M, orientations=None,
              ac_estimate=None, use_gpu=False, use_recip_sym=True, *, maxiter=100, rlambda=None, flambda=1e3,
-             alambda=1, L=None, eps=DEFAULT_EPS, rtol=1e-5, seed=None, return_info=False):
+             alambda=1, L=None, eps=DEFAULT_EPS, rtol=1e-5, seed=None, return_info=False, autotune=False):
     """Autocorrelation (M, M, M) from images and orientations (autocorrelation.py:176-248).
 
     Defaults follow the reference (alpha 1, rho = M^3/N/1000, phi 1e3, maxiter 100,
-    rtol 1e-5).  ``L`` picks the Toeplitz embedding (even, >= 2M; default 2M).  The
+    rtol 1e-5).  ``L`` picks the Toeplitz embedding (even, >= 2M; default 2M);
+    ``autotune`` tunes the spreads' bin settings (see ``setup_linops``).  The
     result dtype follows the reference's CG dtype: float64 when the initial guess is
     float64 (including the zero guess of generation 0), else float32.
     """
@@ -255,7 +259,7 @@ def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, 
     if rlambda is None:
         rlambda = Mtot / N / 1000
     W, d = setup_linops(H_, K_, L_, data, ac_support, weights, x0, M, Mtot, N, ext, alambda, rlambda, flambda,
-                        use_gpu=True, use_recip_sym=True, L_embed=L, eps=eps, scaled=True)
+                        use_gpu=True, use_recip_sym=True, L_embed=L, eps=eps, scaled=True, autotune=autotune)
     x = x0.clone()
     iters, info = W.solver.cg(d, x, alambda, rlambda, flambda, maxiter, rtol)
     ac = x.reshape((M,) * 3)

@@ -41,8 +41,20 @@ struct cmtip_nufft_plan_s {
     const float* z = nullptr;
     // spatial bins of the points (set by setpts): counting sort into bins of bs^3 fine
     // cells, bin-sorted coordinate copies, and the chunk list worked by the kernels
-    int bs = 0, nbx = 0;
+    int bs = 0, nbx = 0, chunk = 0;
     int64_t nbins = 0;
+    // requested launch settings (0 = default): bin edge, points per chunk, persistent
+    // workgroups per CU of the spread / interpolation kernels
+    int bs_req = 0, chunk_req = 0, wpc_req = 0;
+    // slicing route: requested (0 auto, 1 fused tiles, 2 binned) and last used; the
+    // automatic choice is cached per detector geometry
+    int route_req = 0, route_used = 0, route_auto = 0;
+    const double* route_s = nullptr;
+    const int* route_order = nullptr;
+    int64_t route_P = -1;
+    float route_ext = 0.f;
+    DevBuf probe;       // 2 int32: tiles staged in LDS, tiles probed
+    DevBuf cx, cy, cz;  // rotated coordinates of the binned slicing route
     DevBuf bin_of;    // N int32
     DevBuf count;     // nbins int32 (also the scatter cursors)
     DevBuf offset;    // nbins + 1 int32
@@ -86,14 +98,25 @@ __device__ __forceinline__ float fine_coord(float x, float hi, float lo, int n) 
 // setpts sorts the points by the fine-grid bin holding the first node of their kernel
 // footprint (l0 mod n, per axis): bins are bs^3 cells, so every footprint of a bin's
 // points lies in its subgrid of (bs + W - 1)^3 cells.  Bins are cut into chunks of at
-// most kChunk points; each chunk is one unit of work for the spread (type 1) and
+// most chunk <= kChunk points; each chunk is one unit of work for the spread (type 1) and
 // interpolation (type 2) kernels, which stage the chunk's subgrid in LDS.
 
-constexpr int kChunk = 4096;
+constexpr int kChunk = 4096;  // largest chunk (the fixed-point spreading margin assumes it)
 constexpr int kSpreadBlock = 256;
+constexpr size_t kMaxLds = 160 * 1024;     // LDS per workgroup on gfx950
+constexpr size_t kDefaultLds = 64 * 1024;  // dynamic LDS allowed without raising the limit
+constexpr size_t kStaticLds = 256;         // reserve for the binned kernels' static LDS
 
-// Subgrids are 16^3 cells for w <= 12 and 20^3 above (bin edge = subgrid edge - w + 1).
+// Default subgrids are 16^3 cells for w <= 12 and 20^3 above (bin edge = subgrid edge -
+// w + 1); a plan can be given another bin edge, chunk size and launch width
+// (cmtip_nufft_set_bins, chosen by cmtip_nufft_autotune).
 __host__ __device__ inline int bin_size_for(int w) { return w <= 12 ? 17 - w : 21 - w; }
+
+// LDS bytes of the spread (64-bit accumulators) or interpolation (float) subgrid.
+inline size_t subgrid_lds(int bs, int W, bool spread, bool cplx) {
+    const size_t S = (size_t)(bs + W - 1);
+    return S * S * (S + 1) * (spread ? 8 : 4) * (cplx ? 2 : 1);
+}
 
 __device__ __forceinline__ int first_node_wrapped(float x, float hi, float lo, int n, int W, float* t_out) {
     const float t = fine_coord(x, hi, lo, n);
@@ -141,9 +164,9 @@ __global__ void bin_count_kernel(const float* __restrict__ x, const float* __res
 }
 
 // One workgroup: offset = exclusive scan of count, nchunk = exclusive scan of
-// ceil(count / kChunk); ctrl[0] = total chunks.  Counts are then cleared for use as
+// ceil(count / chunk); ctrl[0] = total chunks.  Counts are then cleared for use as
 // scatter cursors.
-__global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count, int64_t nbins,
+__global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count, int64_t nbins, int chunk,
                                                         int* __restrict__ offset, int* __restrict__ nchunk,
                                                         int* __restrict__ ctrl) {
     __shared__ int sa[1024], sb[1024];
@@ -156,7 +179,7 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count,
     for (int64_t base = 0; base < nbins; base += 1024) {
         const int64_t i = base + threadIdx.x;
         const int c = i < nbins ? count[i] : 0;
-        int va = c, vb = (c + kChunk - 1) / kChunk;
+        int va = c, vb = (c + chunk - 1) / chunk;
         sa[threadIdx.x] = va;
         sb[threadIdx.x] = vb;
         __syncthreads();
@@ -204,12 +227,12 @@ __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __r
 }
 
 __global__ void chunk_fill_kernel(const int* __restrict__ offset, const int* __restrict__ nchunk, int64_t nbins,
-                                  int4* __restrict__ chunks) {
+                                  int chunk, int4* __restrict__ chunks) {
     const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (b >= nbins) return;
     const int start = offset[b], end = offset[b + 1];
     int c = nchunk[b];
-    for (int s0 = start; s0 < end; s0 += kChunk, ++c) chunks[c] = make_int4((int)b, s0, min(end, s0 + kChunk), 0);
+    for (int s0 = start; s0 < end; s0 += chunk, ++c) chunks[c] = make_int4((int)b, s0, min(end, s0 + chunk), 0);
 }
 
 // ---------------------------------------------------------------- binned spreading / interpolation
@@ -534,6 +557,31 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// Moments (striped partials) and the optional bf16 copy of slices written by the
+// binned interpolation route.
+__global__ void __launch_bounds__(kBlock) slice_finish_kernel(const float* __restrict__ out, int64_t count,
+                                                              uint16_t* __restrict__ out_bf16,
+                                                              double* __restrict__ part) {
+    double s1v = 0.0, s2v = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < count; j += stride) {
+        const float v = out[j];
+        if (out_bf16) out_bf16[j] = f32_to_bf16(v);
+        s1v += (double)v;
+        s2v += (double)v * (double)v;
+    }
+    if (!part) return;
+    for (int off = 32; off > 0; off >>= 1) {
+        s1v += __shfl_down(s1v, off, 64);
+        s2v += __shfl_down(s2v, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        double* slot = part + 2 * ((blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) % kMomentSlots);
+        unsafeAtomicAdd(slot, s1v);
+        unsafeAtomicAdd(slot + 1, s2v);
+    }
+}
+
 // ---------------------------------------------------------------- fused slicing
 //
 // Each wave takes one tile of 64 pixels of one orientation.  With the pixels in a
@@ -588,7 +636,8 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
                                                             int64_t tiles_per_orient, int64_t total_tiles,
                                                             float ext, float* __restrict__ out,
                                                             uint16_t* __restrict__ out_bf16,
-                                                            double* __restrict__ moments) {
+                                                            double* __restrict__ moments,
+                                                            int* __restrict__ fit_count) {
     __shared__ float box_all[kBlock / 64][kBoxFloats];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -616,6 +665,11 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
     const bool any = mnx < big;
     const bool use_lds = any && ex <= n && ey <= n && ez <= n && ey * ez <= 64 * kBoxGroups &&
                          ex * ey * ez <= kBoxFloats;
+    if (fit_count && any && lane == 0) {
+        // route probe: tiles staged in LDS / tiles with pixels
+        atomicAdd(fit_count, use_lds ? 1 : 0);
+        atomicAdd(fit_count + 1, 1);
+    }
     if (use_lds) {
         // lane q of group gi owns the (y, z) pair q = gi * 64 + lane of the box's x planes:
         // its fine-grid and LDS offsets are fixed, so each x plane costs one scalar offset
@@ -746,6 +800,14 @@ int cu_count() {
     return cus;
 }
 
+// Raise a kernel's dynamic-LDS limit above the default 64 KB: the most it may ask for
+// is the workgroup's LDS less the kernel's own static LDS.
+void allow_large_lds(const void* fn) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, fn) == hipSuccess)
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kMaxLds - a.sharedSizeBytes));
+}
+
 int reset_work_counter(cmtip_nufft_plan_s* p, hipStream_t st) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->ctrl.as<int>() + 1, (int64_t)1, 0);
     CMTIP_LAUNCH_CHECK();
@@ -757,10 +819,14 @@ struct InterpLaunch {
     static int run(cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, hipStream_t st) {
         if (p->npts == 0) return CMTIP_OK;
         CMTIP_TRY(reset_work_counter(p, st));
-        const int S = p->bs + W - 1;
-        const size_t lds = (size_t)S * S * (S + 1) * (cplx ? sizeof(float2) : sizeof(float));
-        const unsigned grid = (unsigned)(cu_count() * (cplx ? 2 : 4));
+        const size_t lds = subgrid_lds(p->bs, W, false, cplx);
+        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : (cplx ? 2 : 4)));
         const int4* ch = p->chunks.as<const int4>();
+        static bool attr_set[2] = {false, false};
+        if (!attr_set[cplx] && lds > kDefaultLds) {
+            allow_large_lds(cplx ? (const void*)interp_binned_kernel<W, true> : (const void*)interp_binned_kernel<W, false>);
+            attr_set[cplx] = true;
+        }
         if (cplx)
             hipLaunchKernelGGL((interp_binned_kernel<W, true>), dim3(grid), dim3(kSpreadBlock), lds, st,
                                p->fine.as<const float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
@@ -789,10 +855,14 @@ struct SpreadLaunch {
             hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)std::min<int64_t>(4096, (cnt + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, st, reinterpret_cast<const float*>(c), cnt, cmax);
         }
-        const int S = p->bs + W - 1;
-        const size_t lds = (size_t)S * S * (S + 1) * (cplx ? 2 : 1) * sizeof(unsigned long long);
-        const unsigned grid = (unsigned)(cu_count() * (lds > 40000 ? 2 : 4));
+        const size_t lds = subgrid_lds(p->bs, W, true, cplx);
+        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : (lds > 40000 ? 2 : 4)));
         const int4* ch = p->chunks.as<const int4>();
+        static bool attr_set[2] = {false, false};
+        if (!attr_set[cplx] && lds > kDefaultLds) {
+            allow_large_lds(cplx ? (const void*)spread_binned_kernel<W, true> : (const void*)spread_binned_kernel<W, false>);
+            attr_set[cplx] = true;
+        }
         if (cplx)
             hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadBlock), lds, st,
                                p->fine.as<float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
@@ -813,10 +883,11 @@ struct SpreadLaunch {
 // Sort the registered points into spatial bins (counting sort) and build the chunk list.
 int bin_points(cmtip_nufft_plan_s* p, hipStream_t st) {
     const int64_t N = p->npts;
-    p->bs = bin_size_for(p->w);
+    p->bs = p->bs_req > 0 ? p->bs_req : bin_size_for(p->w);
+    p->chunk = p->chunk_req > 0 ? p->chunk_req : kChunk;
     p->nbx = (p->n + p->bs - 1) / p->bs;
     p->nbins = (int64_t)p->nbx * p->nbx * p->nbx;
-    p->max_chunks = p->nbins + N / kChunk + 1;
+    p->max_chunks = p->nbins + N / p->chunk + 1;
     CMTIP_TRY(p->bin_of.ensure(sizeof(int) * (size_t)N));
     CMTIP_TRY(p->count.ensure(sizeof(int) * (size_t)p->nbins));
     CMTIP_TRY(p->offset.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
@@ -832,13 +903,13 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), p->nbins, 0);
     hipLaunchKernelGGL(bin_count_kernel, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N, p->n, p->w, p->bs,
                        p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
-    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins,
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                        p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
                        p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(),
                        p->xs.as<float>(), p->ys.as<float>(), p->zs.as<float>());
     hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
-                       p->nchunk.as<const int>(), p->nbins, p->chunks.as<int4>());
+                       p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -846,7 +917,7 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st) {
 template <int W>
 struct SliceLaunch {
     static int run(const cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient,
-                   float ext, float* out, uint16_t* out_bf16, double* moments, hipStream_t st) {
+                   float ext, float* out, uint16_t* out_bf16, double* moments, int* fit_count, hipStream_t st) {
         const int64_t tpo = (P + 63) / 64;
         const int64_t tiles = tpo * n_orient;
         const int64_t blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
@@ -856,7 +927,8 @@ struct SliceLaunch {
         }
         hipLaunchKernelGGL((slice_real_kernel<W>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
                            p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo,
-                           p->rinv.as<const double>(), s, order, P, tpo, tiles, ext, out, out_bf16, moments);
+                           p->rinv.as<const double>(), s, order, P, tpo, tiles, ext, out, out_bf16, moments,
+                           fit_count);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -1100,6 +1172,36 @@ extern "C" int cmtip_nufft_info(cmtip_nufft_plan p, int* w, int* n_fine, double*
     return CMTIP_OK;
 }
 
+extern "C" int cmtip_nufft_set_bins(cmtip_nufft_plan p, int bin_size, int chunk, int wg_per_cu) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    const int bs = bin_size > 0 ? bin_size : bin_size_for(p->w);
+    // the subgrid must fit LDS for every execute form of this plan type (complex worst case)
+    if (bin_size < 0 || bs > p->n || subgrid_lds(bs, p->w, p->type == 1, true) + kStaticLds > kMaxLds) {
+        set_error("set_bins: bin size %d does not fit (w=%d, n=%d)", bin_size, p->w, p->n);
+        return CMTIP_ERR_ARG;
+    }
+    if (chunk < 0 || (chunk > 0 && (chunk < 64 || chunk > kChunk))) {
+        set_error("set_bins: chunk %d outside [64, %d]", chunk, kChunk);
+        return CMTIP_ERR_ARG;
+    }
+    if (wg_per_cu < 0 || wg_per_cu > 16) {
+        set_error("set_bins: wg_per_cu %d outside [0, 16]", wg_per_cu);
+        return CMTIP_ERR_ARG;
+    }
+    p->bs_req = bin_size;
+    p->chunk_req = chunk;
+    p->wpc_req = wg_per_cu;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_nufft_get_bins(cmtip_nufft_plan p, int* bin_size, int* chunk, int* wg_per_cu) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (bin_size) *bin_size = p->bs_req > 0 ? p->bs_req : bin_size_for(p->w);
+    if (chunk) *chunk = p->chunk_req > 0 ? p->chunk_req : kChunk;
+    if (wg_per_cu) *wg_per_cu = p->wpc_req;
+    return CMTIP_OK;
+}
+
 extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y,
                                   const float* z, void* stream) {
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
@@ -1150,6 +1252,89 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     return fine_to_grid(p, pts_real, f, grid_real, support, scale, st);
 }
 
+namespace {
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    int init() {
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            set_error("autotune: hipEventCreate failed");
+            return CMTIP_ERR_HIP;
+        }
+        return CMTIP_OK;
+    }
+    int elapsed(float* ms) {
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(ms, a, b) != hipSuccess) {
+            set_error("autotune: event timing failed");
+            return CMTIP_ERR_HIP;
+        }
+        return CMTIP_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" int cmtip_nufft_autotune(cmtip_nufft_plan p, void* c, void* f, int flags, const float* support,
+                                    float scale, const int* cands, int n_cand, int reps, float setpts_weight,
+                                    float* times_ms, int* best, void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (!cands || n_cand < 1 || reps < 1 || !(setpts_weight >= 0.0f)) {
+        set_error("autotune: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (p->npts > 0 && !p->x) {
+        set_error("autotune: setpts not called");
+        return CMTIP_ERR_STATE;
+    }
+    hipStream_t st = as_stream(stream);
+    const int keep_bs = p->bs_req, keep_chunk = p->chunk_req, keep_wpc = p->wpc_req;
+    EventPair ev;
+    CMTIP_TRY(ev.init());
+    int best_i = -1;
+    float best_cost = 0.f;
+    for (int i = 0; i < n_cand; ++i) {
+        float t_set = -1.f, t_exec = -1.f;
+        if (cmtip_nufft_set_bins(p, cands[3 * i], cands[3 * i + 1], cands[3 * i + 2]) == CMTIP_OK) {
+            if (hipEventRecord(ev.a, st) != hipSuccess) return CMTIP_ERR_HIP;
+            CMTIP_TRY(cmtip_nufft_setpts(p, p->npts, p->x, p->y, p->z, st));
+            if (hipEventRecord(ev.b, st) != hipSuccess) return CMTIP_ERR_HIP;
+            CMTIP_TRY(ev.elapsed(&t_set));
+            CMTIP_TRY(cmtip_nufft_execute(p, c, f, flags, support, scale, st));  // warm-up
+            if (hipEventRecord(ev.a, st) != hipSuccess) return CMTIP_ERR_HIP;
+            for (int r = 0; r < reps; ++r) CMTIP_TRY(cmtip_nufft_execute(p, c, f, flags, support, scale, st));
+            if (hipEventRecord(ev.b, st) != hipSuccess) return CMTIP_ERR_HIP;
+            CMTIP_TRY(ev.elapsed(&t_exec));
+            t_exec /= (float)reps;
+            const float cost = setpts_weight * t_set + t_exec;
+            if (best_i < 0 || cost < best_cost) {
+                best_i = i;
+                best_cost = cost;
+            }
+        }
+        if (times_ms) {
+            times_ms[2 * i] = t_set;
+            times_ms[2 * i + 1] = t_exec;
+        }
+    }
+    if (best_i < 0) {
+        // no candidate fits: restore the previous settings and binning
+        p->bs_req = keep_bs;
+        p->chunk_req = keep_chunk;
+        p->wpc_req = keep_wpc;
+        CMTIP_TRY(cmtip_nufft_setpts(p, p->npts, p->x, p->y, p->z, st));
+        set_error("autotune: no candidate fits this plan");
+        return CMTIP_ERR_ARG;
+    }
+    CMTIP_TRY(cmtip_nufft_set_bins(p, cands[3 * best_i], cands[3 * best_i + 1], cands[3 * best_i + 2]));
+    CMTIP_TRY(cmtip_nufft_setpts(p, p->npts, p->x, p->y, p->z, st));
+    if (best) *best = best_i;
+    return CMTIP_OK;
+}
+
 extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     if (!p) return CMTIP_OK;
     fft1d_free(p->fft);
@@ -1160,7 +1345,7 @@ extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     p->rinv.release();
     p->part.release();
     for (DevBuf* b : {&p->bin_of, &p->count, &p->offset, &p->nchunk, &p->perm, &p->xs, &p->ys, &p->zs, &p->chunks,
-                      &p->ctrl})
+                      &p->ctrl, &p->probe, &p->cx, &p->cy, &p->cz})
         b->release();
     delete p;
     return CMTIP_OK;
@@ -1206,6 +1391,82 @@ extern "C" int cmtip_slice_prepare(cmtip_nufft_plan p, const float* ac, const fl
     return CMTIP_OK;
 }
 
+namespace {
+
+constexpr int kProbeOrients = 16;           // orientations sliced by the route probe
+constexpr double kFusedMinFit = 0.8;        // fused tiles win when this share fits LDS
+constexpr int64_t kBinnedBatchPts = 1 << 27;  // points per binned-route batch
+
+// Fraction of fused tiles whose box fits LDS, on the first orientations (writes their
+// slices to out).  One host synchronisation.
+int probe_fit(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
+              float* out, hipStream_t st, double* frac) {
+    CMTIP_TRY(p->probe.ensure(2 * sizeof(int)));
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->probe.as<int>(), (int64_t)2, 0);
+    CMTIP_LAUNCH_CHECK();
+    CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P,
+                                      std::min<int64_t>(n_orient, kProbeOrients), ext, out, (uint16_t*)nullptr,
+                                      (double*)nullptr, p->probe.as<int>(), st));
+    int h[2] = {0, 0};
+    if (hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(h, p->probe.p, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("slice_real: route probe failed");
+        return CMTIP_ERR_HIP;
+    }
+    *frac = h[1] > 0 ? (double)h[0] / (double)h[1] : 1.0;
+    return CMTIP_OK;
+}
+
+// Binned route: rotated coordinates -> bin sort -> LDS-subgrid interpolation, in
+// batches of orientations.  The plan's registered points are replaced.
+int slice_binned(cmtip_nufft_plan_s* p, const double* s, int64_t P, int64_t n_orient, float ext, float* out,
+                 hipStream_t st) {
+    const int64_t per = std::max<int64_t>(1, kBinnedBatchPts / P);
+    const int64_t cap = std::min(per, n_orient) * P;
+    if (cap > 0x7fffffffLL) {
+        set_error("slice_real: %lld pixels per orientation is too many for the binned route", (long long)P);
+        return CMTIP_ERR_ARG;
+    }
+    CMTIP_TRY(p->cx.ensure(sizeof(float) * (size_t)cap));
+    CMTIP_TRY(p->cy.ensure(sizeof(float) * (size_t)cap));
+    CMTIP_TRY(p->cz.ensure(sizeof(float) * (size_t)cap));
+    for (int64_t o0 = 0; o0 < n_orient; o0 += per) {
+        const int64_t no = std::min(per, n_orient - o0), N = no * P;
+        hipLaunchKernelGGL(rotate_coords_kernel, dim3(blocks_for(N)), dim3(kBlock), 0, st,
+                           p->rinv.as<const double>() + 9 * o0, s, P, N, ext, p->cx.as<float>(), p->cy.as<float>(),
+                           p->cz.as<float>());
+        CMTIP_LAUNCH_CHECK();
+        p->npts = N;
+        p->x = p->cx.as<const float>();
+        p->y = p->cy.as<const float>();
+        p->z = p->cz.as<const float>();
+        CMTIP_TRY(bin_points(p, st));
+        CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, false, (float2*)nullptr, out + o0 * P, st));
+    }
+    // the coordinates are internal: execute needs a new setpts
+    p->x = p->y = p->z = nullptr;
+    return CMTIP_OK;
+}
+
+}  // namespace
+
+extern "C" int cmtip_slice_set_route(cmtip_nufft_plan p, int route) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (route < 0 || route > 2) {
+        set_error("slice_set_route: route %d not in {0 auto, 1 fused, 2 binned}", route);
+        return CMTIP_ERR_ARG;
+    }
+    p->route_req = route;
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_slice_get_route(cmtip_nufft_plan p, int* requested, int* last_used) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (requested) *requested = p->route_req;
+    if (last_used) *last_used = p->route_used;
+    return CMTIP_OK;
+}
+
 extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t n_orient, const double* s,
                                 const int* order, int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
                                 double* moments, void* stream) {
@@ -1221,6 +1482,23 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
     if (n_orient == 0 || P == 0) return CMTIP_OK;
     hipStream_t st = as_stream(stream);
     CMTIP_TRY(ensure_rinv(p, quats, n_orient, st));
+    int route = p->route_req;
+    if (route == 0) {
+        if (p->route_auto && p->route_s == s && p->route_P == P && p->route_ext == recip_extent &&
+            p->route_order == order) {
+            route = p->route_auto;
+        } else {
+            double frac = 1.0;
+            CMTIP_TRY(probe_fit(p, s, order, P, n_orient, recip_extent, out, st, &frac));
+            route = frac >= kFusedMinFit ? 1 : 2;
+            p->route_auto = route;
+            p->route_s = s;
+            p->route_P = P;
+            p->route_ext = recip_extent;
+            p->route_order = order;
+        }
+    }
+    p->route_used = route;
     double* part = nullptr;
     if (moments) {
         CMTIP_TRY(p->part.ensure(sizeof(double) * 2 * kMomentSlots));
@@ -1229,8 +1507,19 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
                            (int64_t)2 * kMomentSlots, 0.0);
         CMTIP_LAUNCH_CHECK();
     }
-    CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P, n_orient, recip_extent, out,
-                                      out_bf16, part, st));
+    if (route == 1) {
+        CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P, n_orient, recip_extent,
+                                          out, out_bf16, part, (int*)nullptr, st));
+    } else {
+        CMTIP_TRY(slice_binned(p, s, P, n_orient, recip_extent, out, st));
+        if (out_bf16 || part) {
+            const int64_t total = n_orient * P;
+            hipLaunchKernelGGL(slice_finish_kernel,
+                               dim3((unsigned)std::min<int64_t>(4096, (total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               st, (const float*)out, total, out_bf16, part);
+            CMTIP_LAUNCH_CHECK();
+        }
+    }
     if (moments) {
         hipLaunchKernelGGL(finish_moments_kernel, dim3(1), dim3(64), 0, st, (const double*)part, moments);
         CMTIP_LAUNCH_CHECK();

@@ -7,10 +7,10 @@ import torch
 
 from .. import _arrays
 from .._lib import CMTIP_GRID_REAL, CMTIP_PTS_ONES, CMTIP_PTS_REAL
-from .plan import DEFAULT_EPS, get_plan
+from .plan import DEFAULT_EPS, get_plan, tuned_setpts
 
 
-def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False):
+def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False, autotune=False):
     assert _arrays.max_abs(H_, K_, L_) < 3 * np.pi  # adjoint.py:29, 66
     as_torch = _arrays.is_torch(nuvect, H_)
     x = _arrays.to_dev(H_, torch.float32).reshape(-1)
@@ -36,8 +36,12 @@ def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, one
         f = torch.empty((M, M, M), dtype=torch.complex64, device=x.device)
     sup = None if support is None else _arrays.to_dev(support, torch.float32)
     plan = get_plan(1, M, +1, eps)
-    plan.setpts(x, y, z)
-    plan.execute(c, f, flags, sup, float(1.0 / M ** 3) if scale is None else float(scale))
+    scale = float(1.0 / M ** 3) if scale is None else float(scale)
+    if autotune:
+        tuned_setpts(plan, x, y, z, c, f, flags, sup, scale)
+    else:
+        plan.setpts(x, y, z)
+    plan.execute(c, f, flags, sup, scale)
     return _arrays.out(f, as_torch)
 
 

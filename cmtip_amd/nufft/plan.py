@@ -44,8 +44,56 @@ class NufftPlan:
         check(_lib.lib().cmtip_nufft_execute(self._h, ptr(c), ptr(f), int(flags), ptr(support), float(scale),
                                              stream()), "cmtip_nufft_execute")
 
+    def set_bins(self, bin_size=0, chunk=0, wg_per_cu=0):
+        """Bin edge, points per chunk and workgroups per CU (0 = default), used from the
+        next setpts on."""
+        check(_lib.lib().cmtip_nufft_set_bins(self._h, int(bin_size), int(chunk), int(wg_per_cu)),
+              "cmtip_nufft_set_bins")
+
+    def get_bins(self):
+        b, c, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().cmtip_nufft_get_bins(self._h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(w)),
+              "cmtip_nufft_get_bins")
+        return b.value, c.value, w.value
+
+    def autotune(self, c, f, flags, support=None, scale=1.0, candidates=None, reps=3, setpts_weight=0.0):
+        """Time execute(c, f, flags, support, scale) over candidate (bin_size, chunk,
+        wg_per_cu) settings with the registered points and keep the fastest (cost =
+        setpts_weight * setpts + execute).  Returns (best, table) with table rows
+        (bin_size, chunk, wg_per_cu, setpts_ms, execute_ms); -1 ms marks candidates
+        that do not fit.  The outputs of execute are overwritten."""
+        assert self._pts is not None, "autotune needs setpts first"
+        if candidates is None:
+            # staged search: bin edge at the default chunk and width, then chunk and width
+            bs, t1 = self.autotune(c, f, flags, support, scale, spread_candidates(self.w, self.type), reps,
+                                   setpts_weight)
+            best, t2 = self.autotune(c, f, flags, support, scale,
+                                     [(bs[0], ch, wpc) for ch in (1024, 2048, 4096) for wpc in (0, 1, 2, 4)],
+                                     reps, setpts_weight)
+            return best, t1 + t2
+        cands = list(candidates)
+        n = len(cands)
+        arr = (ctypes.c_int * (3 * n))(*[int(v) for cand in cands for v in cand])
+        times = (ctypes.c_float * (2 * n))()
+        best = ctypes.c_int(-1)
+        check(_lib.lib().cmtip_nufft_autotune(self._h, ptr(c), ptr(f), int(flags), ptr(support), float(scale), arr,
+                                              n, int(reps), float(setpts_weight), times, ctypes.byref(best),
+                                              stream()), "cmtip_nufft_autotune")
+        table = [tuple(cands[i]) + (times[2 * i], times[2 * i + 1]) for i in range(n)]
+        return tuple(cands[best.value]), table
+
     def slice_prepare(self, ac, support=None):
         check(_lib.lib().cmtip_slice_prepare(self._h, ptr(ac), ptr(support), stream()), "cmtip_slice_prepare")
+
+    def set_slice_route(self, route):
+        """0 automatic, 1 fused tiles, 2 binned (rotated coordinates + bin sort)."""
+        check(_lib.lib().cmtip_slice_set_route(self._h, int(route)), "cmtip_slice_set_route")
+
+    def slice_route(self):
+        """(requested route, route used by the last slice_real)."""
+        r, u = ctypes.c_int(), ctypes.c_int()
+        check(_lib.lib().cmtip_slice_get_route(self._h, ctypes.byref(r), ctypes.byref(u)), "cmtip_slice_get_route")
+        return r.value, u.value
 
     def slice_real(self, quats, s, P, recip_extent, out, out_bf16=None, moments=None, order=None):
         n_orient = quats.shape[0]
@@ -60,11 +108,26 @@ class NufftPlan:
             self._h = None
 
 
+def spread_candidates(w, nufft_type):
+    """Candidate (bin_size, 0, 0) settings: subgrid edges 10..24 cells (bin edge =
+    subgrid edge - w + 1) whose LDS subgrid fits a workgroup in every execute form of
+    the plan type."""
+    out = []
+    per_cell = 16 if nufft_type == 1 else 8  # complex: 64-bit pairs (spread) / float2 (interp)
+    for S in range(10, 25, 2):
+        bs = S - w + 1
+        if bs >= 2 and S * S * (S + 1) * per_cell + 256 <= 160 * 1024:
+            out.append((bs, 0, 0))
+    return out
+
+
 _CACHE = {}
 
 
-def get_plan(nufft_type, n_modes, iflag, eps=DEFAULT_EPS):
-    key = (int(nufft_type), int(n_modes), int(iflag), float(eps), torch.cuda.current_device())
+def get_plan(nufft_type, n_modes, iflag, eps=DEFAULT_EPS, tag=None):
+    """Cached plan per (type, modes, sign, eps, device, tag); users that keep state in
+    a plan between calls (a prepared slicing grid) take their own tag."""
+    key = (int(nufft_type), int(n_modes), int(iflag), float(eps), torch.cuda.current_device(), tag)
     p = _CACHE.get(key)
     if p is None:
         p = NufftPlan(nufft_type, n_modes, iflag, eps)
@@ -74,3 +137,24 @@ def get_plan(nufft_type, n_modes, iflag, eps=DEFAULT_EPS):
 
 def clear_plans():
     _CACHE.clear()
+    _TUNED.clear()
+
+
+# tuned (bin_size, chunk, wg_per_cu) per (plan type, modes, eps, execute flags, points)
+_TUNED = {}
+
+
+def tuned_setpts(plan, x, y, z, c, f, flags, support=None, scale=1.0):
+    """setpts with autotuned bin settings: the first call for a (plan, flags, number of
+    points) runs the autotuner (overwriting the execute outputs), later calls reuse its
+    choice.  The plan keeps the tuned settings."""
+    key = (plan.type, plan.n_modes, plan.eps, int(flags), x.numel(), torch.cuda.current_device())
+    best = _TUNED.get(key)
+    if best is not None:
+        plan.set_bins(*best)
+        plan.setpts(x, y, z)
+        return best
+    plan.setpts(x, y, z)
+    best, _ = plan.autotune(c, f, flags, support, scale)
+    _TUNED[key] = best
+    return best

@@ -63,7 +63,8 @@ int cmtip_nufft_info(cmtip_nufft_plan plan, int* w, int* n_fine, double* beta);
 
 /* Register npts nonuniform points (device float arrays, caller-owned, kept by
  * reference until the next setpts).  Replaces cufinufft set_pts (forward.py:78,
- * adjoint.py:74).  Type-1 plans also sort the points into spreading bins here. */
+ * adjoint.py:74).  The points are also sorted into the spreading / interpolation
+ * bins here (counting sort on the device). */
 int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, const float* y,
                        const float* z, void* stream);
 
@@ -83,6 +84,28 @@ int cmtip_nufft_execute(cmtip_nufft_plan plan, void* c, void* f, int flags, cons
                         float scale, void* stream);
 
 int cmtip_nufft_destroy(cmtip_nufft_plan plan);
+
+/* Bin and launch settings of a plan, used from the next setpts on: bin_size = edge
+ * of a spatial bin in fine cells (its LDS subgrid is bin_size + w - 1 per side),
+ * chunk = points per unit of work (64..4096), wg_per_cu = persistent workgroups per
+ * CU of the spread / interpolation kernels; 0 selects the default of each.  Replaces
+ * cufinufft's plan options (gpu_method=1 with its default bin sizes and subproblem
+ * size, forward.py:77, adjoint.py:73), which the reference leaves at their defaults. */
+int cmtip_nufft_set_bins(cmtip_nufft_plan plan, int bin_size, int chunk, int wg_per_cu);
+int cmtip_nufft_get_bins(cmtip_nufft_plan plan, int* bin_size, int* chunk, int* wg_per_cu);
+
+/* Autotuner for the settings above: with the plan's points registered, for each of
+ * the n_cand candidates (cands: n_cand x {bin_size, chunk, wg_per_cu}) re-bins the
+ * points and runs cmtip_nufft_execute(c, f, flags, support, scale) once untimed and
+ * reps times timed (HIP events on the stream; the call synchronises).  times_ms
+ * (2 x n_cand floats, may be NULL) receives {setpts ms, execute ms} per candidate, -1
+ * for candidates that do not fit the plan.  The plan keeps the candidate with the
+ * least setpts_weight * setpts + execute, re-binned; *best (may be NULL) is its index.
+ * Outputs of execute (f for type 1, c for type 2) are overwritten.  The reference has
+ * no counterpart: it is the bin / tile-size selection behind the spreading kernels. */
+int cmtip_nufft_autotune(cmtip_nufft_plan plan, void* c, void* f, int flags, const float* support, float scale,
+                         const int* cands, int n_cand, int reps, float setpts_weight, float* times_ms, int* best,
+                         void* stream);
 
 /* ---------------------------------------------------------------- rotated q-vectors
  * Coordinates of every pixel under every orientation, orientation-major
@@ -112,11 +135,26 @@ int cmtip_rotate_coords(const float* quats, int64_t n_orient, const double* s, i
  * order.  The result does not depend on it.  out_bf16 (optional, uint16 bfloat16 bits, round to
  * nearest even) receives the same values.  moments (optional, 2 doubles, accumulated,
  * not cleared) receives sum(out) and sum(out^2) for the reference's per-batch std
- * scaling (align.py:122-124). */
+ * scaling (align.py:122-124).
+ * Two routes give bit-identical values: fused tiles (coordinates, LDS box and
+ * interpolation in one kernel) and, for fine grids whose tile boxes do not fit LDS
+ * (large M), binned (rotated coordinates into plan-owned buffers, bin sort, LDS-subgrid
+ * interpolation).  The binned route replaces the plan's registered points: call
+ * cmtip_nufft_setpts again before cmtip_nufft_execute.  With the automatic route the
+ * first call for a detector geometry (s, P, recip_extent, order) probes the fused
+ * kernel on up to 16 orientations and synchronises the stream once. */
 int cmtip_slice_prepare(cmtip_nufft_plan plan, const float* ac, const float* support, void* stream);
 int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient, const double* s,
                      const int* order, int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
                      double* moments, void* stream);
+
+/* Slicing route of a plan: 0 automatic (default; fused tiles when at least 80% of the
+ * probed tiles fit LDS, else binned), 1 fused tiles, 2 binned.  A fixed route never
+ * synchronises (use one under stream capture).  get_route reports the request and the
+ * route the last cmtip_slice_real used (0 before the first).  No counterpart in the
+ * reference, whose compute_slices (align.py:51-63) always takes the generic NUFFT. */
+int cmtip_slice_set_route(cmtip_nufft_plan plan, int route);
+int cmtip_slice_get_route(cmtip_nufft_plan plan, int* requested, int* last_used);
 
 /* ---------------------------------------------------------------- matching
  * Orientation matching (cmtip/alignment/align.py:113-131): for every image i, the

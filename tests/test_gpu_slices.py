@@ -89,3 +89,67 @@ def test_slice_bf16_and_moments():
     o64 = out.double()
     assert abs(float(mom[0]) - float(o64.sum())) <= 1e-9 * float(o64.abs().sum())
     assert abs(float(mom[1]) - float((o64 * o64).sum())) <= 1e-9 * float((o64 * o64).sum())
+
+
+def _slice_route(sl, q, route):
+    d = dev()
+    sl.plan.set_slice_route(route)
+    n = q.shape[0]
+    out = torch.empty((n, sl.P), device=d)
+    b16 = torch.empty((n, sl.P), device=d, dtype=torch.int16)
+    mom = torch.zeros(2, dtype=torch.float64, device=d)
+    sl.slice(q, out=out, out_bf16=b16, moments=mom)
+    return out, b16, mom, sl.plan.slice_route()[1]
+
+
+@pytest.mark.parametrize("M,n_side,auto", [(41, 64, 1), (151, 128, 2), (151, 96, 2)])
+def test_slice_routes_bit_identical(M, n_side, auto):
+    # fused tiles and the binned route (rotated coordinates, bin sort, LDS subgrids)
+    # evaluate the same sums in the same order; the automatic choice follows the share
+    # of tiles whose box fits LDS (most at M = 41, none at M = 151 with 128 or 96 pixels)
+    ac = synth.gaussian_ac(M, 50, seed=5)
+    s, ext = synth.detector(n_side)
+    q = torch.as_tensor(synth.random_quaternions(40, seed=6), device=dev())
+    sl = alignment.Slicer(ac, s, ext)
+    say(f"slice routes M={M} side={n_side}")
+    o1, b1, m1, u1 = _slice_route(sl, q, 1)
+    o2, b2, m2, u2 = _slice_route(sl, q, 2)
+    o0, b0, m0, u0 = _slice_route(sl, q, 0)
+    assert (u1, u2, u0) == (1, 2, auto)
+    assert torch.equal(o1, o2) and torch.equal(o0, o1)
+    assert torch.equal(b1, b2) and torch.equal(b0, b1)
+    for m in (m2, m0):
+        assert torch.allclose(m, m1, rtol=1e-12, atol=0)
+    # spot check of the binned route against exact FP64 sums
+    rng = np.random.default_rng(M)
+    idx = rng.choice(40 * sl.P, 100, replace=False)
+    x, y, z = (np.asarray(v).ravel()[idx] for v in geom.rotated_coords(q.cpu().numpy(), s, ext))
+    from oracle import nufft as ON
+    want = ON.nudft2(ac.astype(np.float64), x, y, z).real
+    assert rel_l2(o2.cpu().numpy().ravel()[idx], want) < 1e-5
+
+
+def test_binned_route_batches_and_replaces_points():
+    # more than one batch of the binned route (2^27 points: 8192 orientations of 16384
+    # pixels); afterwards the plan's points must be registered again before execute
+    from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_REAL, CmtipError
+    M, n = 41, 8200
+    d = dev()
+    ac = synth.gaussian_ac(M, 50, seed=7)
+    s, ext = synth.detector(128)
+    q = torch.as_tensor(synth.random_quaternions(n, seed=8), device=d)
+    sl = alignment.Slicer(ac, s, ext)
+    say(f"binned route batches n={n}")
+    o1 = _slice_route(sl, q, 1)[0]
+    o2 = _slice_route(sl, q, 2)[0]
+    assert torch.equal(o1, o2)
+    del o1, o2
+    p = sl.plan
+    x = torch.zeros(10, device=d)
+    c = torch.empty(10, device=d)
+    acd = torch.as_tensor(ac, device=d)
+    with pytest.raises(CmtipError):
+        p.execute(c, acd, CMTIP_GRID_REAL | CMTIP_PTS_REAL)
+    p.setpts(x, x, x)
+    p.execute(c, acd, CMTIP_GRID_REAL | CMTIP_PTS_REAL)
+    assert abs(float(c[0]) - float(acd.double().sum())) <= 1e-5 * float(acd.double().abs().sum())
