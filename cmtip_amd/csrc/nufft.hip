@@ -53,8 +53,7 @@ struct cmtip_nufft_plan_s {
     const int* route_order = nullptr;
     int64_t route_P = -1;
     float route_ext = 0.f;
-    DevBuf probe;       // 2 int32: tiles staged in LDS, tiles probed
-    DevBuf cx, cy, cz;  // rotated coordinates of the binned slicing route
+    DevBuf probe;  // 2 int32: tiles staged in LDS, tiles probed
     DevBuf bin_of;    // N int32
     DevBuf count;     // nbins int32 (also the scatter cursors)
     DevBuf offset;    // nbins + 1 int32
@@ -112,10 +111,41 @@ constexpr size_t kStaticLds = 256;         // reserve for the binned kernels' st
 // (cmtip_nufft_set_bins, chosen by cmtip_nufft_autotune).
 __host__ __device__ inline int bin_size_for(int w) { return w <= 12 ? 17 - w : 21 - w; }
 
-// LDS bytes of the spread (64-bit accumulators) or interpolation (float) subgrid.
+// Row reads of the interpolation kernels (WideRow): 4 = quad + pair + single when
+// W = 7 (else as 2), 2 = pairs + singles, 1 = single floats.  Measured on gfx950 at
+// M = 81: binned interpolation 5.06 ms (1) -> 3.42 (2) -> 2.69 (4) for 81.92 M points;
+// the fused tile kernel, bound by staging its tile boxes, 10.9 (1) / 10.5 (2) / 11.3 (4)
+// ms for 163.8 M points.
+#ifndef CMTIP_INTERP_READS
+#define CMTIP_INTERP_READS 4
+#endif
+#ifndef CMTIP_SLICE_READS
+#define CMTIP_SLICE_READS 2
+#endif
+
+// Row pitch SP and plane pitch SSP of the interpolation subgrid (S^3 cells, layout
+// x * SSP + y * SP + z).  Real subgrids are read in wide rows: both pitches are
+// multiples of the row alignment (4 for w = 7, else 2), padded by one alignment unit so
+// that consecutive rows and planes start in different LDS banks.  Complex subgrids keep
+// an odd row pitch.
+__host__ __device__ inline void interp_pitch(int S, int W, bool cplx, int* SP, int* SSP) {
+    if (cplx) {
+        *SP = S + 1;
+        *SSP = S * (S + 1);
+        return;
+    }
+    const int A = CMTIP_INTERP_READS == 1 ? 1 : ((W == 7 && CMTIP_INTERP_READS == 4) ? 4 : 2);
+    *SP = (S + A - 1) / A * A + A;
+    *SSP = S * *SP + A;
+}
+
+// LDS bytes of the spread (64-bit accumulators) or interpolation (float / float2) subgrid.
 inline size_t subgrid_lds(int bs, int W, bool spread, bool cplx) {
-    const size_t S = (size_t)(bs + W - 1);
-    return S * S * (S + 1) * (spread ? 8 : 4) * (cplx ? 2 : 1);
+    const int S = bs + W - 1;
+    if (spread) return (size_t)S * S * (S + 1) * 8 * (cplx ? 2 : 1);
+    int SP, SSP;
+    interp_pitch(S, W, cplx, &SP, &SSP);
+    return (size_t)S * SSP * (cplx ? 8 : 4);
 }
 
 __device__ __forceinline__ int first_node_wrapped(float x, float hi, float lo, int n, int W, float* t_out) {
@@ -146,17 +176,33 @@ __device__ __forceinline__ int wave_bin_add(int* __restrict__ counter, int b, bo
     return slot;
 }
 
-__global__ void bin_count_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                 const float* __restrict__ z, int64_t N, int n, int W, int bs, int nbx, float hi,
-                                 float lo, int* __restrict__ bin_of, int* __restrict__ count) {
+// Point sources of the binning and interpolation kernels.  PtsArrays: registered
+// coordinate arrays.  PtsRotated (defined with the rotation helpers below): the
+// rotated pixels of the slicing route, point j = o * P + slot at pixel order[slot],
+// generated from (rinv, s) wherever they are needed instead of being stored.
+struct PtsArrays {
+    const float* x;
+    const float* y;
+    const float* z;
+    __device__ __forceinline__ void at(int64_t j, float* px, float* py, float* pz) const {
+        *px = x[j];
+        *py = y[j];
+        *pz = z[j];
+    }
+};
+
+template <class Src>
+__global__ void bin_count_kernel(Src pts, int64_t N, int n, int W, int bs, int nbx, float hi, float lo,
+                                 int* __restrict__ bin_of, int* __restrict__ count) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = j < N;
     int b = 0;
     if (valid) {
-        float t;
-        const int bx = first_node_wrapped(x[j], hi, lo, n, W, &t) / bs;
-        const int by = first_node_wrapped(y[j], hi, lo, n, W, &t) / bs;
-        const int bz = first_node_wrapped(z[j], hi, lo, n, W, &t) / bs;
+        float t, x, y, z;
+        pts.at(j, &x, &y, &z);
+        const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
+        const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
+        const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
         b = (bx * nbx + by) * nbx + bz;
         bin_of[j] = b;
     }
@@ -210,6 +256,9 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count,
     }
 }
 
+// COPY: also write the coordinates in sorted order (registered points); the slicing
+// route regenerates them from perm instead.
+template <bool COPY>
 __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                    const float* __restrict__ z, int64_t N, const int* __restrict__ bin_of,
                                    const int* __restrict__ offset, int* __restrict__ cursor, int* __restrict__ perm,
@@ -221,9 +270,11 @@ __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __r
     if (!valid) return;
     const int pos = offset[b] + slot;
     perm[pos] = (int)j;
-    xs[pos] = x[j];
-    ys[pos] = y[j];
-    zs[pos] = z[j];
+    if (COPY) {
+        xs[pos] = x[j];
+        ys[pos] = y[j];
+        zs[pos] = z[j];
+    }
 }
 
 __global__ void chunk_fill_kernel(const int* __restrict__ offset, const int* __restrict__ nchunk, int64_t nbins,
@@ -263,6 +314,96 @@ __device__ __forceinline__ void local_weights(float t, int l0w, int origin, floa
     const float tw = 2.0f / (float)W;
 #pragma unroll
     for (int a = 0; a < W; ++a) wt[a] = es_phi(d + (float)a, tw, beta_l2e);
+}
+
+// ---------------------------------------------------------------- wide LDS row reads
+//
+// Interpolation reads W consecutive floats of an LDS row for every (x, y) of a point's
+// footprint.  With the row and plane pitches multiples of kAlign, all row starts of a
+// lane share one alignment m = start mod kAlign, and every row is read with aligned
+// wide loads: for W = 7 one b128, one b64 and one b32 read whatever m is; for other
+// widths b64 pairs and one or two b32 reads.  The lane evaluates its z weights in the
+// order the values arrive (load_offsets), so rows need no shuffling.  With scattered
+// row starts, wide reads move about twice as many bytes per LDS cycle as single floats
+// (measured on gfx950: 36 B/clk per CU for b32, 62 for b64, 76 for b128).
+template <int W, int READS = 4>
+struct WideRow {
+    static constexpr bool kQuad = (W == 7) && READS == 4;
+    static constexpr bool kSingles = READS == 1;
+    static constexpr int kAlign = kQuad ? 4 : (kSingles ? 1 : 2);
+    static constexpr int kPairs = kQuad ? 1 : (kSingles ? 0 : ((W & 1) ? (W - 1) / 2 : W / 2 - 1));
+    int oq, op, os;  // offsets from the row start of the quad (W = 7), first pair, single
+    __device__ __forceinline__ explicit WideRow(int start) {
+        const int m = start & (kAlign - 1);
+        if (kSingles) {
+            oq = op = os = 0;
+        } else if (kQuad) {
+            // m = 0: [0..3] [4,5] [6];  1: [3..6] [1,2] [0];  2: [2..5] [0,1] [6];  3: [1..4] [5,6] [0]
+            oq = (4 - m) & 3;
+            op = (0x5014 >> (4 * m)) & 15;
+            os = (m & 1) ? 0 : 6;
+        } else {
+            oq = 0;
+            op = m;
+            os = m ? 0 : W - ((W & 1) ? 1 : 2);
+        }
+    }
+    // z offsets (0 .. W-1) of the values in load order
+    __device__ __forceinline__ void load_offsets(int* z) const {
+        if (kSingles) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) z[i] = i;
+            return;
+        }
+        int k = 0;
+        if (kQuad)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) z[k++] = oq + i;
+#pragma unroll
+        for (int i = 0; i < kPairs; ++i) {
+            z[k++] = op + 2 * i;
+            z[k++] = op + 2 * i + 1;
+        }
+        z[k++] = os;
+        if (!kQuad && !(W & 1)) z[k++] = W - 1;
+    }
+    // sum_k wl[k] * value_k over the row starting at row (16-byte aligned base + start)
+    __device__ __forceinline__ float dot(const float* row, const float* wl) const {
+        float q = 0.f;
+        if (kSingles) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) q = fmaf(wl[i], row[i], q);
+            return q;
+        }
+        int k = 0;
+        if (kQuad) {
+            const float4 a = *reinterpret_cast<const float4*>(row + oq);
+            q = wl[0] * a.x;
+            q = fmaf(wl[1], a.y, q);
+            q = fmaf(wl[2], a.z, q);
+            q = fmaf(wl[3], a.w, q);
+            k = 4;
+        }
+#pragma unroll
+        for (int i = 0; i < kPairs; ++i) {
+            const float2 b = *reinterpret_cast<const float2*>(row + op + 2 * i);
+            q = fmaf(wl[k], b.x, q);
+            q = fmaf(wl[k + 1], b.y, q);
+            k += 2;
+        }
+        q = fmaf(wl[k], row[os], q);
+        if (!kQuad && !(W & 1)) q = fmaf(wl[k + 1], row[W - 1], q);
+        return q;
+    }
+};
+
+// ES weights of one axis at the load-order offsets z of a WideRow.
+template <int W>
+__device__ __forceinline__ void weights_at(float t, const int* z, float beta_l2e, float* wl) {
+    const float d = ceilf(t - 0.5f * (float)W) - t;
+    const float tw = 2.0f / (float)W;
+#pragma unroll
+    for (int k = 0; k < W; ++k) wl[k] = es_phi(d + (float)z[k], tw, beta_l2e);
 }
 
 // Spreading accumulates in LDS in 64-bit fixed point: integer LDS atomics are an
@@ -367,17 +508,34 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
     }
 }
 
-template <int W, bool CPLX>
+// Sorted point k of the interpolation: coordinates and output index.  SortedCopies:
+// the sorted coordinate copies of setpts; SortedRotated (below): regenerated from
+// perm[k] for the slicing route.
+struct SortedCopies {
+    const float* xs;
+    const float* ys;
+    const float* zs;
+    const int* perm;
+    __device__ __forceinline__ int64_t at(int k, float* px, float* py, float* pz) const {
+        *px = xs[k];
+        *py = ys[k];
+        *pz = zs[k];
+        return perm[k];
+    }
+};
+
+template <int W, bool CPLX, class Sorted>
 __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
     const typename Cell<CPLX>::T* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx,
-    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
-    const int* __restrict__ perm, float2* __restrict__ out_c, float* __restrict__ out_r,
-    const int4* __restrict__ chunks, int* __restrict__ ctrl) {
+    Sorted pts, float2* __restrict__ out_c, float* __restrict__ out_r, const int4* __restrict__ chunks,
+    int* __restrict__ ctrl) {
     typedef typename Cell<CPLX>::T T;
-    extern __shared__ float sub_raw[];
+    extern __shared__ float4 sub_raw[];  // 16-byte aligned for the wide row reads
     T* sub = reinterpret_cast<T*>(sub_raw);
-    // LDS layout (x * S + y) * SP + z with an odd row pitch SP = S + 1 (bank spread)
-    const int S = bs + W - 1, SP = S + 1, SSP = S * SP, S3 = S * S * S;
+    // LDS layout x * SSP + y * SP + z (interp_pitch)
+    const int S = bs + W - 1, S3 = S * S * S;
+    int SP, SSP;
+    interp_pitch(S, W, CPLX, &SP, &SSP);
     __shared__ int chunk_id;
     const int total = ctrl[0];
     for (;;) {
@@ -394,14 +552,15 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
             gx = gx >= n ? gx - n : gx;
             gy = gy >= n ? gy - n : gy;
             gz = gz >= n ? gz - n : gz;
-            sub[(ix * S + iy) * SP + iz] = grid[(uint32_t)((gx * n + gy) * n + gz)];
+            sub[ix * SSP + iy * SP + iz] = grid[(uint32_t)((gx * n + gy) * n + gz)];
         }
         __syncthreads();
         for (int k = ch.y + threadIdx.x; k < ch.z; k += kSpreadBlock) {
-            float tx, ty, tz;
-            const int lxw = first_node_wrapped(xs[k], hi, lo, n, W, &tx);
-            const int lyw = first_node_wrapped(ys[k], hi, lo, n, W, &ty);
-            const int lzw = first_node_wrapped(zs[k], hi, lo, n, W, &tz);
+            float tx, ty, tz, px, py, pz;
+            const int64_t j = pts.at(k, &px, &py, &pz);
+            const int lxw = first_node_wrapped(px, hi, lo, n, W, &tx);
+            const int lyw = first_node_wrapped(py, hi, lo, n, W, &ty);
+            const int lzw = first_node_wrapped(pz, hi, lo, n, W, &tz);
             int lx, ly, lz;
             float wx[W], wy[W], wz[W];
             local_weights<W>(tx, lxw, ox, beta_l2e, &lx, wx);
@@ -413,32 +572,44 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
                 ly = 0;
                 lz = 0;
             }
-            const T* base = sub + (lx * S + ly) * SP + lz;
+            const int boff = lx * SSP + ly * SP + lz;
             float sr = 0.f, si = 0.f;
+            if (CPLX) {
+                const T* base = sub + boff;
 #pragma unroll
-            for (int a = 0; a < W; ++a) {
-                float pr = 0.f, pi = 0.f;
+                for (int a = 0; a < W; ++a) {
+                    float pr = 0.f, pi = 0.f;
 #pragma unroll
-                for (int bb = 0; bb < W; ++bb) {
-                    const T* row = base + a * SSP + bb * SP;
-                    float qr = 0.f, qi = 0.f;
+                    for (int bb = 0; bb < W; ++bb) {
+                        const T* row = base + a * SSP + bb * SP;
+                        float qr = 0.f, qi = 0.f;
 #pragma unroll
-                    for (int cc = 0; cc < W; ++cc) {
-                        if (CPLX) {
+                        for (int cc = 0; cc < W; ++cc) {
                             const float2 v = *reinterpret_cast<const float2*>(row + cc);
                             qr = fmaf(wz[cc], v.x, qr);
                             qi = fmaf(wz[cc], v.y, qi);
-                        } else {
-                            qr = fmaf(wz[cc], *reinterpret_cast<const float*>(row + cc), qr);
                         }
+                        pr = fmaf(wy[bb], qr, pr);
+                        pi = fmaf(wy[bb], qi, pi);
                     }
-                    pr = fmaf(wy[bb], qr, pr);
-                    if (CPLX) pi = fmaf(wy[bb], qi, pi);
+                    sr = fmaf(wx[a], pr, sr);
+                    si = fmaf(wx[a], pi, si);
                 }
-                sr = fmaf(wx[a], pr, sr);
-                if (CPLX) si = fmaf(wx[a], pi, si);
+            } else {
+                const WideRow<W, CMTIP_INTERP_READS> wr(boff);
+                int zo[W];
+                wr.load_offsets(zo);
+                float wl[W];
+                weights_at<W>(tz, zo, beta_l2e, wl);
+                const float* base = reinterpret_cast<const float*>(sub) + boff;
+#pragma unroll
+                for (int a = 0; a < W; ++a) {
+                    float pr = 0.f;
+#pragma unroll
+                    for (int bb = 0; bb < W; ++bb) pr = fmaf(wy[bb], wr.dot(base + a * SSP + bb * SP, wl), pr);
+                    sr = fmaf(wx[a], pr, sr);
+                }
             }
-            const int j = perm[k];
             if (out_c) out_c[j] = make_float2(sr, si);
             if (out_r) out_r[j] = sr;
         }
@@ -534,6 +705,35 @@ __device__ __forceinline__ float rot_coord(const double* r, double s0, double s1
     return ext == 0.0f ? hf : (hf / ext) * 3.14159265358979323846f;
 }
 
+struct PtsRotated {
+    const double* rinv;  // 9 doubles per orientation
+    const double* s;     // 3 x P
+    const int* order;    // P pixel indices or NULL
+    int64_t P;
+    float ext;
+    // coordinates of point j (j < 2^31: the route works in batches); returns its output
+    // index o * P + pixel
+    __device__ __forceinline__ int64_t at(int64_t j, float* px, float* py, float* pz) const {
+        const uint32_t ju = (uint32_t)j, Pu = (uint32_t)P;
+        const uint32_t o = ju / Pu, slot = ju - o * Pu;
+        const int64_t p = order ? (int64_t)order[slot] : (int64_t)slot;
+        const double* r = rinv + 9 * o;
+        const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
+        *px = rot_coord(r, s0, s1, s2, ext);
+        *py = rot_coord(r + 3, s0, s1, s2, ext);
+        *pz = rot_coord(r + 6, s0, s1, s2, ext);
+        return (int64_t)o * P + p;
+    }
+};
+
+struct SortedRotated {
+    PtsRotated rot;
+    const int* perm;
+    __device__ __forceinline__ int64_t at(int k, float* px, float* py, float* pz) const {
+        return rot.at(perm[k], px, py, pz);
+    }
+};
+
 __global__ void __launch_bounds__(kBlock) rotate_coords_kernel(const double* __restrict__ rinv,
                                                                const double* __restrict__ s, int64_t P,
                                                                int64_t total, float ext,
@@ -593,7 +793,14 @@ __global__ void __launch_bounds__(kBlock) slice_finish_kernel(const float* __res
 // the fine grid directly.  Workgroups are renumbered so that consecutive tiles of an
 // orientation run on the same XCD and share its L2.
 
-constexpr int kBoxFloats = 3328;  // per wave: 52 KB per 256-thread workgroup, three per CU
+#ifndef CMTIP_SLICE_WAVES
+#define CMTIP_SLICE_WAVES 4
+#endif
+#ifndef CMTIP_BOX_FLOATS
+#define CMTIP_BOX_FLOATS 3328
+#endif
+constexpr int kSliceBlock = 64 * CMTIP_SLICE_WAVES;  // waves (tiles) per workgroup
+constexpr int kBoxFloats = CMTIP_BOX_FLOATS;         // per wave: 52 KB per workgroup, three per CU
 constexpr int kBoxGroups = 6;   // (y, z) pairs of an x plane: at most 64 * kBoxGroups
 constexpr int kBoxPlanes = 4;   // x planes loaded per step (loads in flight per lane)
 
@@ -628,7 +835,7 @@ __device__ __forceinline__ int wrap_n(int l, int n) {
 }
 
 template <int W>
-__global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restrict__ g, int n, float beta,
+__global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __restrict__ g, int n, float beta,
                                                             float hi, float lo,
                                                             const double* __restrict__ rinv,
                                                             const double* __restrict__ s,
@@ -638,11 +845,11 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
                                                             uint16_t* __restrict__ out_bf16,
                                                             double* __restrict__ moments,
                                                             int* __restrict__ fit_count) {
-    __shared__ float box_all[kBlock / 64][kBoxFloats];
+    __shared__ float4 box_all[kSliceBlock / 64][kBoxFloats / 4];  // 16-byte aligned rows
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float* box = box_all[wave];
-    const int64_t tile = (int64_t)xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + wave;
+    float* box = reinterpret_cast<float*>(box_all[wave]);
+    const int64_t tile = (int64_t)xcd_block(blockIdx.x, gridDim.x) * (kSliceBlock / 64) + wave;
     const bool tile_ok = tile < total_tiles;
     const int64_t o = tile_ok ? tile / tiles_per_orient : 0;
     const int64_t slot = (tile_ok ? tile - o * tiles_per_orient : 0) * 64 + lane;
@@ -663,8 +870,11 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
     const int ey = __builtin_amdgcn_readfirstlane(wave_max_i(active ? ly : -big) + W - mny);
     const int ez = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lz : -big) + W - mnz);
     const bool any = mnx < big;
+    // box layout x * eyzp + y * ezp + z, the z pitch padded to the wide-row alignment
+    constexpr int kA = WideRow<W, CMTIP_SLICE_READS>::kAlign;
+    const int ezp = (ez + kA - 1) & ~(kA - 1), eyzp = ey * ezp;
     const bool use_lds = any && ex <= n && ey <= n && ez <= n && ey * ez <= 64 * kBoxGroups &&
-                         ex * ey * ez <= kBoxFloats;
+                         ex * eyzp <= kBoxFloats;
     if (fit_count && any && lane == 0) {
         // route probe: tiles staged in LDS / tiles with pixels
         atomicAdd(fit_count, use_lds ? 1 : 0);
@@ -676,7 +886,7 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
         const int bx = wrap_n(mnx, n), by = wrap_n(mny, n), bz = wrap_n(mnz, n);
         const int eyz = ey * ez;
         const float inv_z = 1.0f / (float)ez;
-        int goff[kBoxGroups];
+        int goff[kBoxGroups], loff[kBoxGroups];
 #pragma unroll
         for (int gi = 0; gi < kBoxGroups; ++gi) {
             const int q = gi * 64 + lane;
@@ -686,6 +896,7 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
             gy = gy >= n ? gy - n : gy;
             gz = gz >= n ? gz - n : gz;
             goff[gi] = q < eyz ? gy * n + gz : -1;
+            loff[gi] = j * ezp + k;
         }
         const int nn = n * n;
         for (int i0 = 0; i0 < ex; i0 += kBoxPlanes) {
@@ -703,34 +914,34 @@ __global__ void __launch_bounds__(kBlock) slice_real_kernel(const float* __restr
             for (int di = 0; di < kBoxPlanes; ++di)
 #pragma unroll
                 for (int gi = 0; gi < kBoxGroups; ++gi)
-                    if (i0 + di < ex && goff[gi] >= 0) box[(i0 + di) * eyz + gi * 64 + lane] = v[di][gi];
+                    if (i0 + di < ex && goff[gi] >= 0) box[(i0 + di) * eyzp + loff[gi]] = v[di][gi];
         }
     }
     __syncthreads();
     float val = 0.f;
     if (active) {
-        float wx[W], wy[W], wz[W];
+        float wx[W], wy[W];
         row_weights<W>(tx, lx, beta, wx);
         row_weights<W>(ty, ly, beta, wy);
-        row_weights<W>(tz, lz, beta, wz);
         float acc = 0.f;
         if (use_lds) {
-            const int eyz = ey * ez;
-            const float* base = box + ((lx - mnx) * ey + (ly - mny)) * ez + (lz - mnz);
+            const int boff = (lx - mnx) * eyzp + (ly - mny) * ezp + (lz - mnz);
+            const WideRow<W, CMTIP_SLICE_READS> wr(boff);
+            int zo[W];
+            wr.load_offsets(zo);
+            float wl[W];
+            weights_at<W>(tz, zo, beta, wl);
+            const float* base = box + boff;
 #pragma unroll
             for (int a = 0; a < W; ++a) {
                 float pa = 0.f;
 #pragma unroll
-                for (int b = 0; b < W; ++b) {
-                    const float* row = base + a * eyz + b * ez;
-                    float q = 0.f;
-#pragma unroll
-                    for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[c], q);
-                    pa = fmaf(wy[b], q, pa);
-                }
+                for (int b = 0; b < W; ++b) pa = fmaf(wy[b], wr.dot(base + a * eyzp + b * ezp, wl), pa);
                 acc = fmaf(wx[a], pa, acc);
             }
         } else {
+            float wz[W];
+            row_weights<W>(tz, lz, beta, wz);
             int ix[W], iy[W], iz[W];
             const int bx = wrap_n(lx, n), by = wrap_n(ly, n), bz = wrap_n(lz, n);
 #pragma unroll
@@ -816,27 +1027,34 @@ int reset_work_counter(cmtip_nufft_plan_s* p, hipStream_t st) {
 
 template <int W>
 struct InterpLaunch {
-    static int run(cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, hipStream_t st) {
+    template <bool CPLX, class Sorted>
+    static void launch(cmtip_nufft_plan_s* p, Sorted src, float2* out_c, float* out_r, hipStream_t st) {
+        const size_t lds = subgrid_lds(p->bs, W, false, CPLX);
+        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : (CPLX ? 2 : 4)));
+        static bool attr_set = false;
+        if (!attr_set && lds > kDefaultLds) {
+            allow_large_lds((const void*)interp_binned_kernel<W, CPLX, Sorted>);
+            attr_set = true;
+        }
+        hipLaunchKernelGGL((interp_binned_kernel<W, CPLX, Sorted>), dim3(grid), dim3(kSpreadBlock), lds, st,
+                           p->fine.as<const typename Cell<CPLX>::T>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs,
+                           p->nbx, src, out_c, out_r, p->chunks.as<const int4>(), p->ctrl.as<int>());
+    }
+    // rot: the slicing route's rotated pixels (real output only), else the setpts points
+    static int run(cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, const PtsRotated* rot,
+                   hipStream_t st) {
         if (p->npts == 0) return CMTIP_OK;
         CMTIP_TRY(reset_work_counter(p, st));
-        const size_t lds = subgrid_lds(p->bs, W, false, cplx);
-        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : (cplx ? 2 : 4)));
-        const int4* ch = p->chunks.as<const int4>();
-        static bool attr_set[2] = {false, false};
-        if (!attr_set[cplx] && lds > kDefaultLds) {
-            allow_large_lds(cplx ? (const void*)interp_binned_kernel<W, true> : (const void*)interp_binned_kernel<W, false>);
-            attr_set[cplx] = true;
+        if (rot) {
+            launch<false>(p, SortedRotated{*rot, p->perm.as<const int>()}, out_c, out_r, st);
+        } else {
+            const SortedCopies src{p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
+                                   p->perm.as<const int>()};
+            if (cplx)
+                launch<true>(p, src, out_c, out_r, st);
+            else
+                launch<false>(p, src, out_c, out_r, st);
         }
-        if (cplx)
-            hipLaunchKernelGGL((interp_binned_kernel<W, true>), dim3(grid), dim3(kSpreadBlock), lds, st,
-                               p->fine.as<const float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
-                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
-                               p->perm.as<const int>(), out_c, out_r, ch, p->ctrl.as<int>());
-        else
-            hipLaunchKernelGGL((interp_binned_kernel<W, false>), dim3(grid), dim3(kSpreadBlock), lds, st,
-                               p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
-                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
-                               p->perm.as<const int>(), out_c, out_r, ch, p->ctrl.as<int>());
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -880,8 +1098,9 @@ struct SpreadLaunch {
     }
 };
 
-// Sort the registered points into spatial bins (counting sort) and build the chunk list.
-int bin_points(cmtip_nufft_plan_s* p, hipStream_t st) {
+// Sort the registered points (or, with rot, the slicing route's rotated pixels) into
+// spatial bins (counting sort) and build the chunk list.
+int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nullptr) {
     const int64_t N = p->npts;
     p->bs = p->bs_req > 0 ? p->bs_req : bin_size_for(p->w);
     p->chunk = p->chunk_req > 0 ? p->chunk_req : kChunk;
@@ -893,21 +1112,33 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st) {
     CMTIP_TRY(p->offset.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
     CMTIP_TRY(p->nchunk.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
     CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)N));
-    CMTIP_TRY(p->xs.ensure(sizeof(float) * (size_t)N));
-    CMTIP_TRY(p->ys.ensure(sizeof(float) * (size_t)N));
-    CMTIP_TRY(p->zs.ensure(sizeof(float) * (size_t)N));
+    if (!rot) {
+        CMTIP_TRY(p->xs.ensure(sizeof(float) * (size_t)N));
+        CMTIP_TRY(p->ys.ensure(sizeof(float) * (size_t)N));
+        CMTIP_TRY(p->zs.ensure(sizeof(float) * (size_t)N));
+    }
     CMTIP_TRY(p->chunks.ensure(sizeof(int4) * (size_t)p->max_chunks));
     CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
     const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
     const unsigned gp = blocks_for(N);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), p->nbins, 0);
-    hipLaunchKernelGGL(bin_count_kernel, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N, p->n, p->w, p->bs,
-                       p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+    if (rot)
+        hipLaunchKernelGGL(bin_count_kernel<PtsRotated>, dim3(gp), dim3(kBlock), 0, st, *rot, N, p->n, p->w, p->bs,
+                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+    else
+        hipLaunchKernelGGL(bin_count_kernel<PtsArrays>, dim3(gp), dim3(kBlock), 0, st, PtsArrays{p->x, p->y, p->z}, N,
+                           p->n, p->w, p->bs, p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                        p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
-                       p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(),
-                       p->xs.as<float>(), p->ys.as<float>(), p->zs.as<float>());
+    if (rot)
+        hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(gp), dim3(kBlock), 0, st, (const float*)nullptr,
+                           (const float*)nullptr, (const float*)nullptr, N, p->bin_of.as<const int>(),
+                           p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(), (float*)nullptr,
+                           (float*)nullptr, (float*)nullptr);
+    else
+        hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
+                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
+                           p->perm.as<int>(), p->xs.as<float>(), p->ys.as<float>(), p->zs.as<float>());
     hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                        p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
     CMTIP_LAUNCH_CHECK();
@@ -920,12 +1151,12 @@ struct SliceLaunch {
                    float ext, float* out, uint16_t* out_bf16, double* moments, int* fit_count, hipStream_t st) {
         const int64_t tpo = (P + 63) / 64;
         const int64_t tiles = tpo * n_orient;
-        const int64_t blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
+        const int64_t blocks = (tiles + kSliceBlock / 64 - 1) / (kSliceBlock / 64);
         if (blocks > 0x7fffffffLL) {
             set_error("slice_real: too many orientations for one launch");
             return CMTIP_ERR_ARG;
         }
-        hipLaunchKernelGGL((slice_real_kernel<W>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((slice_real_kernel<W>), dim3((unsigned)blocks), dim3(kSliceBlock), 0, st,
                            p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo,
                            p->rinv.as<const double>(), s, order, P, tpo, tiles, ext, out, out_bf16, moments,
                            fit_count);
@@ -1242,7 +1473,7 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
         CMTIP_TRY(grid_to_fine(p, f, grid_real, support, pts_real, st));
         return dispatch_w<InterpLaunch>(p->w, p, !pts_real,
                                         pts_real ? (float2*)nullptr : (float2*)c,
-                                        pts_real ? (float*)c : (float*)nullptr, st);
+                                        pts_real ? (float*)c : (float*)nullptr, (const PtsRotated*)nullptr, st);
     }
     // type 1
     p->slice_ready = 0;
@@ -1345,7 +1576,7 @@ extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     p->rinv.release();
     p->part.release();
     for (DevBuf* b : {&p->bin_of, &p->count, &p->offset, &p->nchunk, &p->perm, &p->xs, &p->ys, &p->zs, &p->chunks,
-                      &p->ctrl, &p->probe, &p->cx, &p->cy, &p->cz})
+                      &p->ctrl, &p->probe})
         b->release();
     delete p;
     return CMTIP_OK;
@@ -1395,7 +1626,7 @@ namespace {
 
 constexpr int kProbeOrients = 16;           // orientations sliced by the route probe
 constexpr double kFusedMinFit = 0.8;        // fused tiles win when this share fits LDS
-constexpr int64_t kBinnedBatchPts = 1 << 27;  // points per binned-route batch
+constexpr int64_t kBinnedBatchPts = 1 << 27;  // points per binned-route batch (8 B of sort state each)
 
 // Fraction of fused tiles whose box fits LDS, on the first orientations (writes their
 // slices to out).  One host synchronisation.
@@ -1417,34 +1648,27 @@ int probe_fit(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t 
     return CMTIP_OK;
 }
 
-// Binned route: rotated coordinates -> bin sort -> LDS-subgrid interpolation, in
-// batches of orientations.  The plan's registered points are replaced.
-int slice_binned(cmtip_nufft_plan_s* p, const double* s, int64_t P, int64_t n_orient, float ext, float* out,
-                 hipStream_t st) {
+// Binned route: bin sort of the rotated pixels -> LDS-subgrid interpolation, in
+// batches of orientations.  Coordinates are regenerated from (rinv, s, order) by the
+// count and interpolation kernels (never stored); points are counted in the tile
+// order, so the pixels of a wave share few bins.  The plan's registered points are
+// dropped.
+int slice_binned(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
+                 float* out, hipStream_t st) {
     const int64_t per = std::max<int64_t>(1, kBinnedBatchPts / P);
-    const int64_t cap = std::min(per, n_orient) * P;
-    if (cap > 0x7fffffffLL) {
+    if (std::min(per, n_orient) * P > 0x7fffffffLL) {
         set_error("slice_real: %lld pixels per orientation is too many for the binned route", (long long)P);
         return CMTIP_ERR_ARG;
     }
-    CMTIP_TRY(p->cx.ensure(sizeof(float) * (size_t)cap));
-    CMTIP_TRY(p->cy.ensure(sizeof(float) * (size_t)cap));
-    CMTIP_TRY(p->cz.ensure(sizeof(float) * (size_t)cap));
+    p->x = p->y = p->z = nullptr;
     for (int64_t o0 = 0; o0 < n_orient; o0 += per) {
         const int64_t no = std::min(per, n_orient - o0), N = no * P;
-        hipLaunchKernelGGL(rotate_coords_kernel, dim3(blocks_for(N)), dim3(kBlock), 0, st,
-                           p->rinv.as<const double>() + 9 * o0, s, P, N, ext, p->cx.as<float>(), p->cy.as<float>(),
-                           p->cz.as<float>());
-        CMTIP_LAUNCH_CHECK();
+        const PtsRotated rot{p->rinv.as<const double>() + 9 * o0, s, order, P, ext};
         p->npts = N;
-        p->x = p->cx.as<const float>();
-        p->y = p->cy.as<const float>();
-        p->z = p->cz.as<const float>();
-        CMTIP_TRY(bin_points(p, st));
-        CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, false, (float2*)nullptr, out + o0 * P, st));
+        CMTIP_TRY(bin_points(p, st, &rot));
+        CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, false, (float2*)nullptr, out + o0 * P, &rot, st));
     }
-    // the coordinates are internal: execute needs a new setpts
-    p->x = p->y = p->z = nullptr;
+    // npts > 0 with no coordinates: execute reports that setpts is needed
     return CMTIP_OK;
 }
 
@@ -1511,7 +1735,7 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
         CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P, n_orient, recip_extent,
                                           out, out_bf16, part, (int*)nullptr, st));
     } else {
-        CMTIP_TRY(slice_binned(p, s, P, n_orient, recip_extent, out, st));
+        CMTIP_TRY(slice_binned(p, s, order, P, n_orient, recip_extent, out, st));
         if (out_bf16 || part) {
             const int64_t total = n_orient * P;
             hipLaunchKernelGGL(slice_finish_kernel,
