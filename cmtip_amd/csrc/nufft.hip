@@ -802,6 +802,12 @@ __global__ void __launch_bounds__(kBlock) slice_finish_kernel(const float* __res
 constexpr int kSliceBlock = 64 * CMTIP_SLICE_WAVES;  // waves (tiles) per workgroup
 constexpr int kBoxFloats = CMTIP_BOX_FLOATS;         // per wave: 52 KB per workgroup, three per CU
 constexpr int kBoxGroups = 6;   // (y, z) pairs of an x plane: at most 64 * kBoxGroups
+// tile-binned route (see slice_tiles_kernel)
+constexpr int kTileS = 24;                    // subgrid edge (cells)
+constexpr int kTileE = 17;                    // widest tile footprint per axis, kernel width included
+constexpr int kTileBs = kTileS - kTileE + 1;  // bin edge (cells)
+constexpr int kTileChunk = 128;               // tiles per unit of work
+constexpr int kTileBlock = 512;               // eight waves: one tile each at a time
 constexpr int kBoxPlanes = 4;   // x planes loaded per step (loads in flight per lane)
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -832,6 +838,37 @@ __device__ __forceinline__ int first_node(float t, int W) { return (int)ceilf(t 
 __device__ __forceinline__ int wrap_n(int l, int n) {
     int b = l % n;
     return b < 0 ? b + n : b;
+}
+
+// Interpolation of one point straight from the fine grid (no LDS staging): first
+// nodes lx, ly, lz (unwrapped), x / y weights given, z weights from tz.
+template <int W>
+__device__ __forceinline__ float direct_gather(const float* __restrict__ g, int n, float tz, int lx, int ly, int lz,
+                                               const float* wx, const float* wy, float beta) {
+    float wz[W];
+    row_weights<W>(tz, lz, beta, wz);
+    int ix[W], iy[W], iz[W];
+    const int bx = wrap_n(lx, n), by = wrap_n(ly, n), bz = wrap_n(lz, n);
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        ix[a] = bx + a >= n ? bx + a - n : bx + a;
+        iy[a] = by + a >= n ? by + a - n : by + a;
+        iz[a] = bz + a >= n ? bz + a - n : bz + a;
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        float pa = 0.f;
+        for (int b = 0; b < W; ++b) {
+            const float* row = g + (uint32_t)((ix[a] * n + iy[b]) * n);
+            float q = 0.f;
+#pragma unroll
+            for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
+            pa = fmaf(wy[b], q, pa);
+        }
+        acc = fmaf(wx[a], pa, acc);
+    }
+    return acc;
 }
 
 template <int W>
@@ -876,9 +913,11 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
     const bool use_lds = any && ex <= n && ey <= n && ez <= n && ey * ez <= 64 * kBoxGroups &&
                          ex * eyzp <= kBoxFloats;
     if (fit_count && any && lane == 0) {
-        // route probe: tiles staged in LDS / tiles with pixels
+        // route probe: tiles staged in LDS, tiles with pixels, tiles within the tile
+        // route's footprint margin
         atomicAdd(fit_count, use_lds ? 1 : 0);
         atomicAdd(fit_count + 1, 1);
+        atomicAdd(fit_count + 2, (ex <= kTileE && ey <= kTileE && ez <= kTileE) ? 1 : 0);
     }
     if (use_lds) {
         // lane q of group gi owns the (y, z) pair q = gi * 64 + lane of the box's x planes:
@@ -940,28 +979,7 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
                 acc = fmaf(wx[a], pa, acc);
             }
         } else {
-            float wz[W];
-            row_weights<W>(tz, lz, beta, wz);
-            int ix[W], iy[W], iz[W];
-            const int bx = wrap_n(lx, n), by = wrap_n(ly, n), bz = wrap_n(lz, n);
-#pragma unroll
-            for (int a = 0; a < W; ++a) {
-                ix[a] = bx + a >= n ? bx + a - n : bx + a;
-                iy[a] = by + a >= n ? by + a - n : by + a;
-                iz[a] = bz + a >= n ? bz + a - n : bz + a;
-            }
-#pragma unroll
-            for (int a = 0; a < W; ++a) {
-                float pa = 0.f;
-                for (int b = 0; b < W; ++b) {
-                    const float* row = g + (uint32_t)((ix[a] * n + iy[b]) * n);
-                    float q = 0.f;
-#pragma unroll
-                    for (int c = 0; c < W; ++c) q = fmaf(wz[c], row[iz[c]], q);
-                    pa = fmaf(wy[b], q, pa);
-                }
-                acc = fmaf(wx[a], pa, acc);
-            }
+            acc = direct_gather<W>(g, n, tz, lx, ly, lz, wx, wy, beta);
         }
         val = acc;
         const int64_t j = o * P + p;
@@ -978,6 +996,184 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
             double* slot = moments + 2 * (blockIdx.x % kMomentSlots);
             unsafeAtomicAdd(slot, s1v);
             unsafeAtomicAdd(slot + 1, s2v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- tile-binned slicing
+//
+// A tile is 64 consecutive pixels of the pixel order (an 8 x 8 detector block) of one
+// orientation.  tile_key_kernel sorts tiles by the bin (kTileBs^3 fine cells) of their
+// footprint's low corner; every footprint no wider than kTileE cells then lies inside
+// its bin's subgrid of kTileS^3 cells.  slice_tiles_kernel stages a bin's subgrid in
+// LDS once per chunk of up to kTileChunk tiles and interpolates all of them from it
+// with wide row reads, so the staging cost is shared by many tiles of many
+// orientations.  Wider tiles go to an overflow bin whose chunks read the fine grid
+// directly.
+
+struct TileRot {
+    const double* rinv;
+    const double* s;
+    const int* order;
+    int64_t P, tpo;  // pixels, tiles per orientation
+    float ext;
+    // pixel of lane `lane` in tile t: returns false past the last pixel
+    __device__ __forceinline__ bool pixel(int64_t t, int lane, int64_t* o, int64_t* p) const {
+        *o = t / tpo;
+        const int64_t slot = (t - *o * tpo) * 64 + lane;
+        const bool ok = slot < P;
+        *p = order ? (int64_t)order[ok ? slot : 0] : (ok ? slot : 0);
+        return ok;
+    }
+    __device__ __forceinline__ void coords(int64_t o, int64_t p, float hi, float lo, int n, float* tx, float* ty,
+                                           float* tz) const {
+        const double* r = rinv + 9 * o;
+        const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
+        *tx = fine_coord(rot_coord(r, s0, s1, s2, ext), hi, lo, n);
+        *ty = fine_coord(rot_coord(r + 3, s0, s1, s2, ext), hi, lo, n);
+        *tz = fine_coord(rot_coord(r + 6, s0, s1, s2, ext), hi, lo, n);
+    }
+};
+
+// Low corner (unwrapped first nodes) and footprint extents of the wave's tile.
+template <int W>
+__device__ __forceinline__ void tile_box(bool active, int lx, int ly, int lz, int* mn, int* ex) {
+    const int big = 1 << 29;
+    mn[0] = __builtin_amdgcn_readfirstlane(wave_min_i(active ? lx : big));
+    mn[1] = __builtin_amdgcn_readfirstlane(wave_min_i(active ? ly : big));
+    mn[2] = __builtin_amdgcn_readfirstlane(wave_min_i(active ? lz : big));
+    ex[0] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lx : -big) + W - mn[0]);
+    ex[1] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? ly : -big) + W - mn[1]);
+    ex[2] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lz : -big) + W - mn[2]);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) tile_key_kernel(TileRot rot, int64_t ntiles, int n, float hi, float lo,
+                                                          int nbx, int* __restrict__ tile_bin,
+                                                          int* __restrict__ count) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (t >= ntiles) return;  // whole waves leave together
+    int64_t o, p;
+    const bool active = rot.pixel(t, lane, &o, &p);
+    float tx, ty, tz;
+    rot.coords(o, p, hi, lo, n, &tx, &ty, &tz);
+    int mn[3], ex[3];
+    tile_box<W>(active, first_node(tx, W), first_node(ty, W), first_node(tz, W), mn, ex);
+    if (lane == 0) {
+        const int nbins = nbx * nbx * nbx;
+        int key = nbins;  // overflow
+        if (ex[0] <= kTileE && ex[1] <= kTileE && ex[2] <= kTileE && ex[0] <= n && ex[1] <= n && ex[2] <= n)
+            key = ((wrap_n(mn[0], n) / kTileBs) * nbx + wrap_n(mn[1], n) / kTileBs) * nbx + wrap_n(mn[2], n) / kTileBs;
+        tile_bin[t] = key;
+        atomicAdd(count + key, 1);
+    }
+}
+
+__global__ void tile_scatter_kernel(int64_t ntiles, const int* __restrict__ tile_bin, const int* __restrict__ offset,
+                                    int* __restrict__ cursor, int* __restrict__ tperm) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = t < ntiles;
+    const int b = valid ? tile_bin[t] : 0;
+    const int slot = wave_bin_add(cursor, b, valid);
+    if (valid) tperm[offset[b] + slot] = (int)t;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
+    const float* __restrict__ g, int n, float beta, float hi, float lo, TileRot rot, int nbx,
+    const int* __restrict__ tperm, const int4* __restrict__ chunks, int* __restrict__ ctrl, float* __restrict__ out,
+    uint16_t* __restrict__ out_bf16, double* __restrict__ part) {
+    extern __shared__ float4 sub4[];  // 16-byte aligned rows
+    float* sub = reinterpret_cast<float*>(sub4);
+    int SP, SSP;
+    interp_pitch(kTileS, W, false, &SP, &SSP);
+    constexpr int S3 = kTileS * kTileS * kTileS;
+    __shared__ int chunk_id;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int total = ctrl[0];
+    const int nbins = nbx * nbx * nbx;
+    double m1 = 0.0, m2 = 0.0;
+    for (;;) {
+        if (threadIdx.x == 0) chunk_id = atomicAdd(ctrl + 1, 1);
+        __syncthreads();
+        const int cid = chunk_id;
+        if (cid >= total) break;
+        const int4 ch = chunks[cid];
+        const bool staged = ch.x < nbins;
+        int ox = 0, oy = 0, oz = 0;
+        if (staged) {
+            ox = (ch.x / (nbx * nbx)) * kTileBs;
+            oy = ((ch.x / nbx) % nbx) * kTileBs;
+            oz = (ch.x % nbx) * kTileBs;
+            for (int i = threadIdx.x; i < S3; i += kTileBlock) {
+                const int ix = i / (kTileS * kTileS), iy = (i / kTileS) % kTileS, iz = i % kTileS;
+                int gx = ox + ix, gy = oy + iy, gz = oz + iz;
+                gx = gx >= n ? gx - n : gx;
+                gy = gy >= n ? gy - n : gy;
+                gz = gz >= n ? gz - n : gz;
+                sub[ix * SSP + iy * SP + iz] = g[(uint32_t)((gx * n + gy) * n + gz)];
+            }
+        }
+        __syncthreads();
+        for (int k = ch.y + wave; k < ch.z; k += kTileBlock / 64) {
+            int64_t o, p;
+            const bool active = rot.pixel(tperm[k], lane, &o, &p);
+            float tx, ty, tz;
+            rot.coords(o, p, hi, lo, n, &tx, &ty, &tz);
+            const int lx = first_node(tx, W), ly = first_node(ty, W), lz = first_node(tz, W);
+            int mn[3], ex[3];
+            tile_box<W>(active, lx, ly, lz, mn, ex);
+            // local first nodes in the subgrid; a tile that does not fit (never, for tiles
+            // binned by tile_key_kernel) is read from the fine grid instead
+            const int ux = active ? lx - mn[0] + wrap_n(mn[0], n) - ox : 0;
+            const int uy = active ? ly - mn[1] + wrap_n(mn[1], n) - oy : 0;
+            const int uz = active ? lz - mn[2] + wrap_n(mn[2], n) - oz : 0;
+            const int mx = __builtin_amdgcn_readfirstlane(wave_max_i(active ? max(ux, max(uy, uz)) : 0));
+            const int mnl = __builtin_amdgcn_readfirstlane(wave_min_i(active ? min(ux, min(uy, uz)) : 0));
+            const bool fits = staged && mnl >= 0 && mx + W <= kTileS;
+            float wx[W], wy[W];
+            row_weights<W>(tx, lx, beta, wx);
+            row_weights<W>(ty, ly, beta, wy);
+            float acc = 0.f;
+            if (fits) {
+                const int boff = ux * SSP + uy * SP + uz;
+                const WideRow<W, CMTIP_INTERP_READS> wr(boff);
+                int zo[W];
+                wr.load_offsets(zo);
+                float wl[W];
+                weights_at<W>(tz, zo, beta, wl);
+                const float* base = sub + boff;
+#pragma unroll
+                for (int a = 0; a < W; ++a) {
+                    float pa = 0.f;
+#pragma unroll
+                    for (int b = 0; b < W; ++b) pa = fmaf(wy[b], wr.dot(base + a * SSP + b * SP, wl), pa);
+                    acc = fmaf(wx[a], pa, acc);
+                }
+            } else {
+                acc = direct_gather<W>(g, n, tz, lx, ly, lz, wx, wy, beta);
+            }
+            if (active) {
+                const int64_t j = o * rot.P + p;
+                out[j] = acc;
+                if (out_bf16) out_bf16[j] = f32_to_bf16(acc);
+                m1 += (double)acc;
+                m2 += (double)acc * (double)acc;
+            }
+        }
+        __syncthreads();
+    }
+    if (part) {
+        for (int off = 32; off > 0; off >>= 1) {
+            m1 += __shfl_down(m1, off, 64);
+            m2 += __shfl_down(m2, off, 64);
+        }
+        if (lane == 0) {
+            double* slot = part + 2 * ((blockIdx.x * (kTileBlock / 64) + wave) % kMomentSlots);
+            unsafeAtomicAdd(slot, m1);
+            unsafeAtomicAdd(slot + 1, m2);
         }
     }
 }
@@ -1144,6 +1340,50 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
+
+template <int W>
+struct TileLaunch {
+    static int run(cmtip_nufft_plan_s* p, const TileRot& rot, int64_t ntiles, float* out, uint16_t* out_bf16,
+                   double* part, hipStream_t st) {
+        const int nbx = (p->n + kTileBs - 1) / kTileBs;
+        const int64_t nbins = (int64_t)nbx * nbx * nbx;  // + 1 overflow bin
+        CMTIP_TRY(p->bin_of.ensure(sizeof(int) * (size_t)ntiles));
+        CMTIP_TRY(p->count.ensure(sizeof(int) * (size_t)(nbins + 1)));
+        CMTIP_TRY(p->offset.ensure(sizeof(int) * (size_t)(nbins + 2)));
+        CMTIP_TRY(p->nchunk.ensure(sizeof(int) * (size_t)(nbins + 2)));
+        CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)ntiles));
+        CMTIP_TRY(p->chunks.ensure(sizeof(int4) * (size_t)(nbins + 2 + ntiles / kTileChunk)));
+        CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
+        const unsigned gb = (unsigned)((nbins + 1 + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), nbins + 1, 0);
+        hipLaunchKernelGGL(tile_key_kernel<W>, dim3((unsigned)((ntiles + kBlock / 64 - 1) / (kBlock / 64))),
+                           dim3(kBlock), 0, st, rot, ntiles, p->n, p->tc_hi, p->tc_lo, nbx, p->bin_of.as<int>(),
+                           p->count.as<int>());
+        hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), nbins + 1, kTileChunk,
+                           p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
+        hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
+                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
+                           p->perm.as<int>());
+        hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
+                           p->nchunk.as<const int>(), nbins + 1, kTileChunk, p->chunks.as<int4>());
+        CMTIP_LAUNCH_CHECK();
+        CMTIP_TRY(reset_work_counter(p, st));
+        int SP, SSP;
+        interp_pitch(kTileS, W, false, &SP, &SSP);
+        const size_t lds = (size_t)kTileS * SSP * sizeof(float);
+        static bool attr_set = false;
+        if (!attr_set && lds > kDefaultLds) {
+            allow_large_lds((const void*)slice_tiles_kernel<W>);
+            attr_set = true;
+        }
+        const unsigned grid = (unsigned)(cu_count() * std::max<size_t>(1, kMaxLds / (lds + kStaticLds)));
+        hipLaunchKernelGGL(slice_tiles_kernel<W>, dim3(grid), dim3(kTileBlock), lds, st, p->fine.as<const float>(),
+                           p->n, p->beta_l2e, p->tc_hi, p->tc_lo, rot, nbx, p->perm.as<const int>(),
+                           p->chunks.as<const int4>(), p->ctrl.as<int>(), out, out_bf16, part);
+        CMTIP_LAUNCH_CHECK();
+        return CMTIP_OK;
+    }
+};
 
 template <int W>
 struct SliceLaunch {
@@ -1625,27 +1865,52 @@ extern "C" int cmtip_slice_prepare(cmtip_nufft_plan p, const float* ac, const fl
 namespace {
 
 constexpr int kProbeOrients = 16;           // orientations sliced by the route probe
-constexpr double kFusedMinFit = 0.8;        // fused tiles win when this share fits LDS
+constexpr double kTileMinFit = 0.95;        // tile route when this share of tiles fits its margin
+constexpr int64_t kTileMinPerBin = 64;      // and the workload has this many tiles per bin
+constexpr double kFusedMinFit = 0.8;        // else fused tiles when this share of boxes fits LDS
 constexpr int64_t kBinnedBatchPts = 1 << 27;  // points per binned-route batch (8 B of sort state each)
 
-// Fraction of fused tiles whose box fits LDS, on the first orientations (writes their
-// slices to out).  One host synchronisation.
-int probe_fit(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
-              float* out, hipStream_t st, double* frac) {
-    CMTIP_TRY(p->probe.ensure(2 * sizeof(int)));
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->probe.as<int>(), (int64_t)2, 0);
+// Route choice from the first orientations (their slices are written to out): the
+// share of tiles within the tile route's footprint margin and the share whose fused
+// box fits LDS.  One host synchronisation.
+int probe_route(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
+                float* out, hipStream_t st, int* route) {
+    CMTIP_TRY(p->probe.ensure(3 * sizeof(int)));
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->probe.as<int>(), (int64_t)3, 0);
     CMTIP_LAUNCH_CHECK();
     CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P,
                                       std::min<int64_t>(n_orient, kProbeOrients), ext, out, (uint16_t*)nullptr,
                                       (double*)nullptr, p->probe.as<int>(), st));
-    int h[2] = {0, 0};
+    int h[3] = {0, 0, 0};
     if (hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(h, p->probe.p, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("slice_real: route probe failed");
         return CMTIP_ERR_HIP;
     }
-    *frac = h[1] > 0 ? (double)h[0] / (double)h[1] : 1.0;
+    const double tiles = h[1] > 0 ? (double)h[1] : 1.0;
+    const int64_t nbx = (p->n + kTileBs - 1) / kTileBs;
+    const bool tile_reuse = n_orient * ((P + 63) / 64) >= kTileMinPerBin * nbx * nbx * nbx;
+    if (h[2] >= kTileMinFit * tiles && tile_reuse)
+        *route = 3;
+    else if (h[0] >= kFusedMinFit * tiles)
+        *route = 1;
+    else
+        *route = 2;
     return CMTIP_OK;
+}
+
+// Tile-binned route over all orientations (tiles are counted in 32-bit indices).
+int slice_tiles(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
+                float* out, uint16_t* out_bf16, double* part, hipStream_t st) {
+    const int64_t tpo = (P + 63) / 64, ntiles = tpo * n_orient;
+    if (ntiles > 0x7fffffffLL) {
+        set_error("slice_real: %lld tiles is too many for the tile route", (long long)ntiles);
+        return CMTIP_ERR_ARG;
+    }
+    const TileRot rot{p->rinv.as<const double>(), s, order, P, tpo, ext};
+    p->x = p->y = p->z = nullptr;  // the bins no longer hold registered points
+    p->npts = std::max<int64_t>(p->npts, 1);
+    return dispatch_w<TileLaunch>(p->w, p, rot, ntiles, out, out_bf16, part, st);
 }
 
 // Binned route: bin sort of the rotated pixels -> LDS-subgrid interpolation, in
@@ -1676,8 +1941,8 @@ int slice_binned(cmtip_nufft_plan_s* p, const double* s, const int* order, int64
 
 extern "C" int cmtip_slice_set_route(cmtip_nufft_plan p, int route) {
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
-    if (route < 0 || route > 2) {
-        set_error("slice_set_route: route %d not in {0 auto, 1 fused, 2 binned}", route);
+    if (route < 0 || route > 3) {
+        set_error("slice_set_route: route %d not in {0 auto, 1 fused, 2 binned, 3 tiles}", route);
         return CMTIP_ERR_ARG;
     }
     p->route_req = route;
@@ -1712,9 +1977,7 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
             p->route_order == order) {
             route = p->route_auto;
         } else {
-            double frac = 1.0;
-            CMTIP_TRY(probe_fit(p, s, order, P, n_orient, recip_extent, out, st, &frac));
-            route = frac >= kFusedMinFit ? 1 : 2;
+            CMTIP_TRY(probe_route(p, s, order, P, n_orient, recip_extent, out, st, &route));
             p->route_auto = route;
             p->route_s = s;
             p->route_P = P;
@@ -1734,6 +1997,8 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
     if (route == 1) {
         CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P, n_orient, recip_extent,
                                           out, out_bf16, part, (int*)nullptr, st));
+    } else if (route == 3) {
+        CMTIP_TRY(slice_tiles(p, s, order, P, n_orient, recip_extent, out, out_bf16, part, st));
     } else {
         CMTIP_TRY(slice_binned(p, s, order, P, n_orient, recip_extent, out, st));
         if (out_bf16 || part) {

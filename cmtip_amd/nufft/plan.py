@@ -86,7 +86,7 @@ class NufftPlan:
         check(_lib.lib().cmtip_slice_prepare(self._h, ptr(ac), ptr(support), stream()), "cmtip_slice_prepare")
 
     def set_slice_route(self, route):
-        """0 automatic, 1 fused tiles, 2 binned (rotated coordinates + bin sort)."""
+        """0 automatic, 1 fused tiles, 2 binned points, 3 binned tiles."""
         check(_lib.lib().cmtip_slice_set_route(self._h, int(route)), "cmtip_slice_set_route")
 
     def slice_route(self):

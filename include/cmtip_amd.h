@@ -136,10 +136,11 @@ int cmtip_rotate_coords(const float* quats, int64_t n_orient, const double* s, i
  * nearest even) receives the same values.  moments (optional, 2 doubles, accumulated,
  * not cleared) receives sum(out) and sum(out^2) for the reference's per-batch std
  * scaling (align.py:122-124).
- * Two routes give bit-identical values: fused tiles (coordinates, LDS box and
- * interpolation in one kernel) and, for fine grids whose tile boxes do not fit LDS
- * (large M), binned (rotated coordinates into plan-owned buffers, bin sort, LDS-subgrid
- * interpolation).  The binned route replaces the plan's registered points: call
+ * Three routes compute the same sums (FP32 additions in different orders): fused
+ * tiles (coordinates, an LDS box per 64 pixels and interpolation in one kernel),
+ * binned tiles (tiles of 64 pixels sorted by fine-grid bin; one LDS subgrid serves all
+ * tiles of a bin) and binned points (pixels sorted by bin, for large M where tile
+ * footprints are wide).  The binned routes replace the plan's registered points: call
  * cmtip_nufft_setpts again before cmtip_nufft_execute.  With the automatic route the
  * first call for a detector geometry (s, P, recip_extent, order) probes the fused
  * kernel on up to 16 orientations and synchronises the stream once. */
@@ -148,8 +149,11 @@ int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient
                      const int* order, int64_t P, float recip_extent, float* out, uint16_t* out_bf16,
                      double* moments, void* stream);
 
-/* Slicing route of a plan: 0 automatic (default; fused tiles when at least 80% of the
- * probed tiles fit LDS, else binned), 1 fused tiles, 2 binned.  A fixed route never
+/* Slicing route of a plan: 0 automatic (default: binned tiles when at least 95% of the
+ * probed tiles fit the tile route's margin and the call has at least 64 tiles per bin,
+ * else fused tiles when at least 80% of their boxes fit LDS, else binned points),
+ * 1 fused tiles, 2 binned points, 3 binned tiles.
+ * A fixed route never
  * synchronises (use one under stream capture).  get_route reports the request and the
  * route the last cmtip_slice_real used (0 before the first).  No counterpart in the
  * reference, whose compute_slices (align.py:51-63) always takes the generic NUFFT. */

@@ -102,34 +102,53 @@ def _slice_route(sl, q, route):
     return out, b16, mom, sl.plan.slice_route()[1]
 
 
-@pytest.mark.parametrize("M,n_side,auto", [(41, 64, 1), (151, 128, 2), (151, 96, 2)])
-def test_slice_routes_bit_identical(M, n_side, auto):
-    # fused tiles and the binned route (rotated coordinates, bin sort, LDS subgrids)
-    # evaluate the same sums (their FP32 additions differ only in order, set by each
-    # layout's row alignment); the automatic choice follows the share of tiles whose box
-    # fits LDS (most at M = 41, none at M = 151 with 128 or 96 pixels)
+@pytest.mark.parametrize("M,n_side,n_q,auto", [(41, 64, 40, 1), (41, 64, 2200, 3), (81, 128, 40, 1),
+                                                (151, 128, 40, 2), (151, 96, 40, 2)])
+def test_slice_routes_agree(M, n_side, n_q, auto):
+    # fused tiles, binned points (rotated coordinates, bin sort, LDS subgrids) and binned
+    # tiles evaluate the same sums (their FP32 additions differ only in order, set by each
+    # layout's row alignment).  The automatic choice follows the probed share of tiles
+    # within the tile route's margin (all at M = 41 and 81), the tiles per bin of the call
+    # (enough at M = 41 with 2200 orientations) and the share of fused boxes that fit
+    # LDS (none at M = 151 with 128 or 96 pixels)
     ac = synth.gaussian_ac(M, 50, seed=5)
     s, ext = synth.detector(n_side)
-    q = torch.as_tensor(synth.random_quaternions(40, seed=6), device=dev())
+    q = torch.as_tensor(synth.random_quaternions(n_q, seed=6), device=dev())
     sl = alignment.Slicer(ac, s, ext)
     say(f"slice routes M={M} side={n_side}")
-    o1, b1, m1, u1 = _slice_route(sl, q, 1)
-    o2, b2, m2, u2 = _slice_route(sl, q, 2)
-    o0, b0, m0, u0 = _slice_route(sl, q, 0)
-    assert (u1, u2, u0) == (1, 2, auto)
-    assert rel_l2(o2, o1) < 1e-6
-    assert torch.equal(o0, o1 if auto == 1 else o2)
-    for o, b in ((o1, b1), (o2, b2), (o0, b0)):
+    res = {r: _slice_route(sl, q, r) for r in (1, 2, 3, 0)}
+    assert [res[r][3] for r in (1, 2, 3, 0)] == [1, 2, 3, auto]
+    o1, m1 = res[1][0], res[1][2]
+    for r in (2, 3):
+        assert rel_l2(res[r][0], o1) < 1e-6
+        assert torch.allclose(res[r][2], m1, rtol=1e-6, atol=0)
+    assert torch.equal(res[0][0], res[auto][0])
+    for o, b, _, _ in res.values():
         assert torch.equal(b, o.to(torch.bfloat16).view(torch.int16))
-    for m in (m2, m0):
-        assert torch.allclose(m, m1, rtol=1e-6, atol=0)
-    # spot check of the binned route against exact FP64 sums
+    # spot check of the binned routes against exact FP64 sums
     rng = np.random.default_rng(M)
-    idx = rng.choice(40 * sl.P, 100, replace=False)
+    idx = rng.choice(n_q * sl.P, 100, replace=False)
     x, y, z = (np.asarray(v).ravel()[idx] for v in geom.rotated_coords(q.cpu().numpy(), s, ext))
     from oracle import nufft as ON
     want = ON.nudft2(ac.astype(np.float64), x, y, z).real
-    assert rel_l2(o2.cpu().numpy().ravel()[idx], want) < 1e-5
+    for r in (2, 3):
+        assert rel_l2(res[r][0].cpu().numpy().ravel()[idx], want) < 1e-5
+
+
+def test_tile_route_overflow_tiles():
+    # pixels in index order (no 8 x 8 blocking): most 64-pixel tiles are detector rows,
+    # wider than the tile route's margin, and go through its direct-gather overflow bin
+    M = 41
+    ac = synth.gaussian_ac(M, 50, seed=9)
+    s, ext = synth.detector(64)
+    q = torch.as_tensor(synth.random_quaternions(24, seed=10), device=dev())
+    sl = alignment.Slicer(ac, s, ext, tile_pixels=False)
+    say("tile route with overflow tiles")
+    o3 = _slice_route(sl, q, 3)[0]
+    o1 = _slice_route(sl, q, 1)[0]
+    assert rel_l2(o3, o1) < 1e-6
+    want = OA.compute_slices(q.cpu().numpy(), s, ext, ac, eps=1e-12)
+    assert rel_l2(o3.cpu().numpy().ravel(), want) < 1e-5
 
 
 def test_binned_route_batches_and_replaces_points():
