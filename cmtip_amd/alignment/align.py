@@ -120,6 +120,7 @@ class Slicer:
         self.ext = _ext32(reciprocal_extent)
         self.order = pixel_order(self.s) if tile_pixels else None
         self.plan.slice_prepare(self.ac)
+        self.plan.set_slice_route(0)  # automatic, probed again for this detector geometry
 
     def slice(self, quats, out=None, out_bf16=None, moments=None):
         q = _arrays.to_dev(quats, torch.float32).reshape(-1, 4)

@@ -46,9 +46,11 @@ struct cmtip_nufft_plan_s {
     // requested launch settings (0 = default): bin edge, points per chunk, persistent
     // workgroups per CU of the spread / interpolation kernels
     int bs_req = 0, chunk_req = 0, wpc_req = 0;
-    // slicing route: requested (0 auto, 1 fused tiles, 2 binned) and last used; the
-    // automatic choice is cached per detector geometry
-    int route_req = 0, route_used = 0, route_auto = 0;
+    // slicing route: requested (0 auto, 1 fused tiles, 2 binned points, 3 binned tiles)
+    // and last used; the automatic choice's probe (shares of tiles whose fused box fits
+    // LDS and whose footprint fits the tile route) is cached per detector geometry
+    int route_req = 0, route_used = 0, route_probed = 0;
+    double fit_box = 0.0, fit_tile = 0.0;
     const double* route_s = nullptr;
     const int* route_order = nullptr;
     int64_t route_P = -1;
@@ -804,8 +806,25 @@ constexpr int kBoxFloats = CMTIP_BOX_FLOATS;         // per wave: 52 KB per work
 constexpr int kBoxGroups = 6;   // (y, z) pairs of an x plane: at most 64 * kBoxGroups
 // tile-binned route (see slice_tiles_kernel)
 constexpr int kTileS = 24;                    // subgrid edge (cells)
-constexpr int kTileE = 17;                    // widest tile footprint per axis, kernel width included
+constexpr int kTileE = 19;                    // widest keyed footprint per axis (kernel width and slack included)
 constexpr int kTileBs = kTileS - kTileE + 1;  // bin edge (cells)
+#ifndef CMTIP_TILE_SP_PAD
+#define CMTIP_TILE_SP_PAD 4
+#endif
+#ifndef CMTIP_TILE_SSP_PAD
+#define CMTIP_TILE_SSP_PAD 12
+#endif
+// Row / plane pitch of the tile route's subgrid: multiples of the wide-row alignment,
+// padded so that neighbouring pixels of a tile start their rows in different banks
+// (row pitch 28, plane pitch 684 = 12 mod 32; measured at M = 81 over 163.8 M pixels:
+// plane pads 4 / 8 / 12 / 16 / 20 give 6.86 / 6.06 / 6.00 / 5.95 / 5.97 ms, row pitch
+// 36 gives 7.4 ms).
+template <int W>
+__host__ __device__ inline void tile_pitch(int* SP, int* SSP) {
+    const int A = (W == 7 && CMTIP_INTERP_READS == 4) ? 4 : (CMTIP_INTERP_READS == 1 ? 1 : 2);
+    *SP = (kTileS + A - 1) / A * A + CMTIP_TILE_SP_PAD;
+    *SSP = kTileS * *SP + CMTIP_TILE_SSP_PAD;
+}
 constexpr int kTileChunk = 128;               // tiles per unit of work
 constexpr int kTileBlock = 512;               // eight waves: one tile each at a time
 constexpr int kBoxPlanes = 4;   // x planes loaded per step (loads in flight per lane)
@@ -1004,12 +1023,15 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
 //
 // A tile is 64 consecutive pixels of the pixel order (an 8 x 8 detector block) of one
 // orientation.  tile_key_kernel sorts tiles by the bin (kTileBs^3 fine cells) of their
-// footprint's low corner; every footprint no wider than kTileE cells then lies inside
-// its bin's subgrid of kTileS^3 cells.  slice_tiles_kernel stages a bin's subgrid in
-// LDS once per chunk of up to kTileChunk tiles and interpolates all of them from it
-// with wide row reads, so the staging cost is shared by many tiles of many
-// orientations.  Wider tiles go to an overflow bin whose chunks read the fine grid
-// directly.
+// footprint's low corner, estimated from the block's four corner pixels (slots 0, 21,
+// 42, 63 of its Morton order) with one cell of slack on each side: the rotated pixel
+// coordinates are affine across a block up to the Ewald curvature, far below a cell.
+// A footprint keyed no wider than kTileE cells then lies inside its bin's subgrid of
+// kTileS^3 cells.  slice_tiles_kernel stages a bin's subgrid in LDS once per chunk of
+// up to kTileChunk tiles and interpolates all of them from it with wide row reads, so
+// the staging cost is shared by many tiles of many orientations.  It checks every
+// tile's exact footprint; tiles that do not fit (and wider tiles, keyed to an overflow
+// bin) read the fine grid directly.
 
 struct TileRot {
     const double* rinv;
@@ -1051,32 +1073,48 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) tile_key_kernel(TileRot rot, int64_t ntiles, int n, float hi, float lo,
                                                           int nbx, int* __restrict__ tile_bin,
                                                           int* __restrict__ count) {
-    const int lane = threadIdx.x & 63;
-    const int64_t t = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (t >= ntiles) return;  // whole waves leave together
-    int64_t o, p;
-    const bool active = rot.pixel(t, lane, &o, &p);
-    float tx, ty, tz;
-    rot.coords(o, p, hi, lo, n, &tx, &ty, &tz);
-    int mn[3], ex[3];
-    tile_box<W>(active, first_node(tx, W), first_node(ty, W), first_node(tz, W), mn, ex);
-    if (lane == 0) {
-        const int nbins = nbx * nbx * nbx;
-        int key = nbins;  // overflow
-        if (ex[0] <= kTileE && ex[1] <= kTileE && ex[2] <= kTileE && ex[0] <= n && ex[1] <= n && ex[2] <= n)
-            key = ((wrap_n(mn[0], n) / kTileBs) * nbx + wrap_n(mn[1], n) / kTileBs) * nbx + wrap_n(mn[2], n) / kTileBs;
-        tile_bin[t] = key;
-        atomicAdd(count + key, 1);
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ntiles) return;
+    const int64_t o = t / rot.tpo, first = (t - o * rot.tpo) * 64;
+    const int last = (int)std::min<int64_t>(63, rot.P - 1 - first);
+    const int corner[4] = {0, 21, 42, 63};
+    int mn[3] = {1 << 29, 1 << 29, 1 << 29}, mx[3] = {-(1 << 29), -(1 << 29), -(1 << 29)};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int64_t slot = first + min(corner[c], last);
+        const int64_t p = rot.order ? (int64_t)rot.order[slot] : slot;
+        float tc[3];
+        rot.coords(o, p, hi, lo, n, tc, tc + 1, tc + 2);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int l = first_node(tc[d], W);
+            mn[d] = min(mn[d], l);
+            mx[d] = max(mx[d], l);
+        }
     }
+    const int nbins = nbx * nbx * nbx;
+    bool fit = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        mn[d] -= 1;  // slack
+        const int ext = mx[d] + 1 - mn[d] + W;
+        fit = fit && ext <= kTileE && ext <= n;
+    }
+    const int key = fit ? ((wrap_n(mn[0], n) / kTileBs) * nbx + wrap_n(mn[1], n) / kTileBs) * nbx +
+                              wrap_n(mn[2], n) / kTileBs
+                        : nbins;  // overflow
+    tile_bin[t] = key;
+    atomicAdd(count + key, 1);
 }
 
+// Neighbouring tiles mostly lie in different bins: plain atomics (the order of tiles
+// inside a bin does not change any result).
 __global__ void tile_scatter_kernel(int64_t ntiles, const int* __restrict__ tile_bin, const int* __restrict__ offset,
                                     int* __restrict__ cursor, int* __restrict__ tperm) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = t < ntiles;
-    const int b = valid ? tile_bin[t] : 0;
-    const int slot = wave_bin_add(cursor, b, valid);
-    if (valid) tperm[offset[b] + slot] = (int)t;
+    if (t >= ntiles) return;
+    const int b = tile_bin[t];
+    tperm[offset[b] + atomicAdd(cursor + b, 1)] = (int)t;
 }
 
 template <int W>
@@ -1087,7 +1125,7 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
     extern __shared__ float4 sub4[];  // 16-byte aligned rows
     float* sub = reinterpret_cast<float*>(sub4);
     int SP, SSP;
-    interp_pitch(kTileS, W, false, &SP, &SSP);
+    tile_pitch<W>(&SP, &SSP);
     constexpr int S3 = kTileS * kTileS * kTileS;
     __shared__ int chunk_id;
     const int lane = threadIdx.x & 63;
@@ -1109,10 +1147,7 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
             oz = (ch.x % nbx) * kTileBs;
             for (int i = threadIdx.x; i < S3; i += kTileBlock) {
                 const int ix = i / (kTileS * kTileS), iy = (i / kTileS) % kTileS, iz = i % kTileS;
-                int gx = ox + ix, gy = oy + iy, gz = oz + iz;
-                gx = gx >= n ? gx - n : gx;
-                gy = gy >= n ? gy - n : gy;
-                gz = gz >= n ? gz - n : gz;
+                const int gx = (ox + ix) % n, gy = (oy + iy) % n, gz = (oz + iz) % n;  // any n
                 sub[ix * SSP + iy * SP + iz] = g[(uint32_t)((gx * n + gy) * n + gz)];
             }
         }
@@ -1125,11 +1160,16 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
             const int lx = first_node(tx, W), ly = first_node(ty, W), lz = first_node(tz, W);
             int mn[3], ex[3];
             tile_box<W>(active, lx, ly, lz, mn, ex);
-            // local first nodes in the subgrid; a tile that does not fit (never, for tiles
-            // binned by tile_key_kernel) is read from the fine grid instead
-            const int ux = active ? lx - mn[0] + wrap_n(mn[0], n) - ox : 0;
-            const int uy = active ? ly - mn[1] + wrap_n(mn[1], n) - oy : 0;
-            const int uz = active ? lz - mn[2] + wrap_n(mn[2], n) - oz : 0;
+            // local first nodes in the subgrid: the exact low corner's offset from the bin
+            // origin, taken mod n (the keyed corner may lie one period away); a tile that
+            // does not fit is read from the fine grid instead
+            int d0 = wrap_n(mn[0], n) - ox, d1 = wrap_n(mn[1], n) - oy, d2 = wrap_n(mn[2], n) - oz;
+            d0 += d0 < 0 ? n : 0;
+            d1 += d1 < 0 ? n : 0;
+            d2 += d2 < 0 ? n : 0;
+            const int ux = active ? lx - mn[0] + d0 : 0;
+            const int uy = active ? ly - mn[1] + d1 : 0;
+            const int uz = active ? lz - mn[2] + d2 : 0;
             const int mx = __builtin_amdgcn_readfirstlane(wave_max_i(active ? max(ux, max(uy, uz)) : 0));
             const int mnl = __builtin_amdgcn_readfirstlane(wave_min_i(active ? min(ux, min(uy, uz)) : 0));
             const bool fits = staged && mnl >= 0 && mx + W <= kTileS;
@@ -1356,9 +1396,8 @@ struct TileLaunch {
         CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
         const unsigned gb = (unsigned)((nbins + 1 + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), nbins + 1, 0);
-        hipLaunchKernelGGL(tile_key_kernel<W>, dim3((unsigned)((ntiles + kBlock / 64 - 1) / (kBlock / 64))),
-                           dim3(kBlock), 0, st, rot, ntiles, p->n, p->tc_hi, p->tc_lo, nbx, p->bin_of.as<int>(),
-                           p->count.as<int>());
+        hipLaunchKernelGGL(tile_key_kernel<W>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, rot, ntiles, p->n,
+                           p->tc_hi, p->tc_lo, nbx, p->bin_of.as<int>(), p->count.as<int>());
         hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), nbins + 1, kTileChunk,
                            p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
         hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
@@ -1369,7 +1408,7 @@ struct TileLaunch {
         CMTIP_LAUNCH_CHECK();
         CMTIP_TRY(reset_work_counter(p, st));
         int SP, SSP;
-        interp_pitch(kTileS, W, false, &SP, &SSP);
+        tile_pitch<W>(&SP, &SSP);
         const size_t lds = (size_t)kTileS * SSP * sizeof(float);
         static bool attr_set = false;
         if (!attr_set && lds > kDefaultLds) {
@@ -1870,11 +1909,11 @@ constexpr int64_t kTileMinPerBin = 64;      // and the workload has this many ti
 constexpr double kFusedMinFit = 0.8;        // else fused tiles when this share of boxes fits LDS
 constexpr int64_t kBinnedBatchPts = 1 << 27;  // points per binned-route batch (8 B of sort state each)
 
-// Route choice from the first orientations (their slices are written to out): the
-// share of tiles within the tile route's footprint margin and the share whose fused
-// box fits LDS.  One host synchronisation.
+// Probe for the automatic route on the first orientations (their slices are written
+// to out): the share of tiles whose fused box fits LDS and the share within the tile
+// route's footprint margin.  One host synchronisation.
 int probe_route(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
-                float* out, hipStream_t st, int* route) {
+                float* out, hipStream_t st) {
     CMTIP_TRY(p->probe.ensure(3 * sizeof(int)));
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->probe.as<int>(), (int64_t)3, 0);
     CMTIP_LAUNCH_CHECK();
@@ -1888,15 +1927,18 @@ int probe_route(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_
         return CMTIP_ERR_HIP;
     }
     const double tiles = h[1] > 0 ? (double)h[1] : 1.0;
+    p->fit_box = h[0] / tiles;
+    p->fit_tile = h[2] / tiles;
+    return CMTIP_OK;
+}
+
+// Automatic route for a call of n_orient orientations from the probed shares.
+int auto_route(const cmtip_nufft_plan_s* p, int64_t P, int64_t n_orient) {
     const int64_t nbx = (p->n + kTileBs - 1) / kTileBs;
     const bool tile_reuse = n_orient * ((P + 63) / 64) >= kTileMinPerBin * nbx * nbx * nbx;
-    if (h[2] >= kTileMinFit * tiles && tile_reuse)
-        *route = 3;
-    else if (h[0] >= kFusedMinFit * tiles)
-        *route = 1;
-    else
-        *route = 2;
-    return CMTIP_OK;
+    if (p->fit_tile >= kTileMinFit && tile_reuse) return 3;
+    if (p->fit_box >= kFusedMinFit) return 1;
+    return 2;
 }
 
 // Tile-binned route over all orientations (tiles are counted in 32-bit indices).
@@ -1946,6 +1988,7 @@ extern "C" int cmtip_slice_set_route(cmtip_nufft_plan p, int route) {
         return CMTIP_ERR_ARG;
     }
     p->route_req = route;
+    p->route_probed = 0;  // the next automatic call probes again
     return CMTIP_OK;
 }
 
@@ -1973,17 +2016,16 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
     CMTIP_TRY(ensure_rinv(p, quats, n_orient, st));
     int route = p->route_req;
     if (route == 0) {
-        if (p->route_auto && p->route_s == s && p->route_P == P && p->route_ext == recip_extent &&
-            p->route_order == order) {
-            route = p->route_auto;
-        } else {
-            CMTIP_TRY(probe_route(p, s, order, P, n_orient, recip_extent, out, st, &route));
-            p->route_auto = route;
+        if (!(p->route_probed && p->route_s == s && p->route_P == P && p->route_ext == recip_extent &&
+              p->route_order == order)) {
+            CMTIP_TRY(probe_route(p, s, order, P, n_orient, recip_extent, out, st));
+            p->route_probed = 1;
             p->route_s = s;
             p->route_P = P;
             p->route_ext = recip_extent;
             p->route_order = order;
         }
+        route = auto_route(p, P, n_orient);
     }
     p->route_used = route;
     double* part = nullptr;

@@ -142,8 +142,9 @@ int cmtip_rotate_coords(const float* quats, int64_t n_orient, const double* s, i
  * tiles of a bin) and binned points (pixels sorted by bin, for large M where tile
  * footprints are wide).  The binned routes replace the plan's registered points: call
  * cmtip_nufft_setpts again before cmtip_nufft_execute.  With the automatic route the
- * first call for a detector geometry (s, P, recip_extent, order) probes the fused
- * kernel on up to 16 orientations and synchronises the stream once. */
+ * first call for a detector geometry (s, P, recip_extent, order: compared by pointer
+ * and value; cmtip_slice_set_route clears it) probes the fused kernel on up to 16
+ * orientations and synchronises the stream once. */
 int cmtip_slice_prepare(cmtip_nufft_plan plan, const float* ac, const float* support, void* stream);
 int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient, const double* s,
                      const int* order, int64_t P, float recip_extent, float* out, uint16_t* out_bf16,

@@ -102,14 +102,14 @@ def _slice_route(sl, q, route):
     return out, b16, mom, sl.plan.slice_route()[1]
 
 
-@pytest.mark.parametrize("M,n_side,n_q,auto", [(41, 64, 40, 1), (41, 64, 2200, 3), (81, 128, 40, 1),
+@pytest.mark.parametrize("M,n_side,n_q,auto", [(41, 64, 40, 1), (41, 64, 3500, 3), (81, 128, 40, 1),
                                                 (151, 128, 40, 2), (151, 96, 40, 2)])
 def test_slice_routes_agree(M, n_side, n_q, auto):
     # fused tiles, binned points (rotated coordinates, bin sort, LDS subgrids) and binned
     # tiles evaluate the same sums (their FP32 additions differ only in order, set by each
     # layout's row alignment).  The automatic choice follows the probed share of tiles
     # within the tile route's margin (all at M = 41 and 81), the tiles per bin of the call
-    # (enough at M = 41 with 2200 orientations) and the share of fused boxes that fit
+    # (enough at M = 41 with 3500 orientations) and the share of fused boxes that fit
     # LDS (none at M = 151 with 128 or 96 pixels)
     ac = synth.gaussian_ac(M, 50, seed=5)
     s, ext = synth.detector(n_side)
@@ -175,3 +175,17 @@ def test_binned_route_batches_and_replaces_points():
     p.setpts(x, x, x)
     p.execute(c, acd, CMTIP_GRID_REAL | CMTIP_PTS_REAL)
     assert abs(float(c[0]) - float(acd.double().sum())) <= 1e-5 * float(acd.double().abs().sum())
+
+
+def test_tile_route_small_grid():
+    # a forced tile route on a fine grid smaller than its subgrid (M = 7, n = 16): the
+    # staged subgrid wraps the grid more than once
+    M = 7
+    ac = synth.gaussian_ac(M, 5, seed=2)
+    s, ext = synth.detector(16)
+    q = torch.as_tensor(synth.random_quaternions(12, seed=3), device=dev())
+    sl = alignment.Slicer(ac, s, ext)
+    say("tile route small grid M=7")
+    o3 = _slice_route(sl, q, 3)[0]
+    want = OA.compute_slices(q.cpu().numpy(), s, ext, ac, eps=1e-12)
+    assert rel_l2(o3.cpu().numpy().ravel(), want) < 1e-5
