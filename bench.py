@@ -14,6 +14,7 @@ autocorrelation solve, a roofline entry for the dominant kernel and a CPU baseli
 (the oracle on a bounded sample, on this host's cores).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -99,6 +100,7 @@ def main():
     import torch
 
     from cmtip_amd import _lib, alignment, nufft, synth
+    from cmtip_amd._lib import check
     from cmtip_amd.alignment import align as AL
     from cmtip_amd.autocorrelation import autocorrelation as AC
 
@@ -151,17 +153,25 @@ def main():
     L.cmtip_match_topk_bf16 = _Wrapped(orig_topk)
     L.cmtip_match_argmin_f32 = _Wrapped(orig_f32)
 
+    kernel_ms = {}
+
     def timed(precision, steps, warmup):
         for _ in range(warmup):
             step(precision)
         torch.cuda.synchronize()
         slice_t.reset()
         gemm_t.reset()
+        # HIP events around the main slicing kernel, recorded by the library on its stream
+        check(L.cmtip_slice_timer(sl.plan._h, 1, None, None), "cmtip_slice_timer")
         t = time.perf_counter()
         for i in range(steps):
             q = step(precision)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
+        tot, cnt = ctypes.c_float(), ctypes.c_int()
+        check(L.cmtip_slice_timer(sl.plan._h, 0, ctypes.byref(tot), ctypes.byref(cnt)), "cmtip_slice_timer")
+        kernel_ms["slice_kernel"] = tot.value / max(cnt.value, 1)
+        kernel_ms["route"] = sl.plan.slice_route()[1]
         return el, q
 
     log(f"[bench] warmup {args.warmup}, timing {args.steps} steps ({args.precision})")
@@ -174,20 +184,25 @@ def main():
     # roofline of the dominant kernel
     N = n_ref * P
     n_fine = sl.plan.n_fine
-    if slice_ms >= gemm_ms:
-        out_b = 4
-        bytes_launch = out_b * N + 24 * P + 16 * n_ref + 4 * n_fine ** 3
-        achieved = bytes_launch / (slice_ms * 1e-3) / 1e9
-        roof = {"kernel": "slice_real_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic("slice_real_kernel")
+    route = kernel_ms["route"]
+    k_ms = kernel_ms["slice_kernel"]
+    if k_ms >= gemm_ms:
+        kname = {1: "slice_real_kernel", 2: "binned points route", 3: "slice_tiles_kernel"}[route]
+        n_tiles = n_ref * ((P + 63) // 64)
+        bytes_launch = 4 * N + 24 * P + 16 * n_ref + 4 * n_fine ** 3 + (4 * n_tiles if route == 3 else 0)
+        achieved = bytes_launch / (k_ms * 1e-3) / 1e9
+        roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic(kname)
                 if args.config == "c2" and args.nbatches == 1 else None,
-                "per_unit": f"{out_b} B per rotated pixel + 24 B per pixel + 16 B per orientation + fine grid",
-                "avg_ms": slice_ms}
+                "per_unit": "4 B per rotated pixel + 24 B per pixel + 16 B per orientation + fine grid"
+                            + (" + 4 B per tile" if route == 3 else ""),
+                "avg_ms": k_ms, "slicing_call_ms": slice_ms}
     else:
         flops = 2.0 * n_img * n_ref * P
         peak = BF16_PEAK_TFS if args.precision == "bf16" else FP32_MFMA_PEAK_TFS
         achieved = flops / (gemm_ms * 1e-3) / 1e12
         roof = {"kernel": "match_topk_bf16_kernel" if args.precision == "bf16" else "match_argmin_f32_kernel",
+                "slicing_kernel_ms": k_ms,
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": None, "per_unit": "2 flop per (image, reference, pixel)", "avg_ms": gemm_ms}
 

@@ -42,6 +42,7 @@ SIGNATURES = {
     "cmtip_nufft_autotune": [_vp, _vp, _vp, _i, _vp, _f, _vp, _i, _i, _f, _vp, _pi, _vp],
     "cmtip_slice_set_route": [_vp, _i],
     "cmtip_slice_get_route": [_vp, _pi, _pi],
+    "cmtip_slice_timer": [_vp, _i, ctypes.POINTER(ctypes.c_float), _pi],
     "cmtip_rotate_coords": [_vp, _i64, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "cmtip_slice_prepare": [_vp, _vp, _vp, _vp],
     "cmtip_slice_real": [_vp, _vp, _i64, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],

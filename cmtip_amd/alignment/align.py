@@ -10,7 +10,9 @@ Reference orientations: the reference draws them with skopi's get_uniform_quat
 (align.py:108), which is not available; pass ``ref_orientations`` to use a given
 set, otherwise a seeded uniform set (cmtip_amd.synth.random_quaternions) is drawn.
 """
+import collections
 import ctypes
+import hashlib
 
 import numpy as np
 import torch
@@ -59,6 +61,31 @@ def calc_eudist(model_slices, slices):
 def _s64(pixel_position_reciprocal):
     s = _arrays.to_dev(pixel_position_reciprocal, torch.float64)
     return s.reshape(3, -1).contiguous()
+
+
+_GEOMETRY = collections.OrderedDict()
+
+
+def _geometry(pixel_position_reciprocal, tile_pixels):
+    """Device copy of the pixel vectors (3, P) float64 and their pixel order.  Host
+    arrays are cached by content (a few detectors), so repeated alignment calls reuse
+    the same device tensors and the library's route probe for them."""
+    key = None
+    if not isinstance(pixel_position_reciprocal, torch.Tensor):
+        a = np.ascontiguousarray(pixel_position_reciprocal)
+        key = (a.shape, a.dtype.str, hashlib.blake2b(a.tobytes(), digest_size=16).digest(), bool(tile_pixels),
+               torch.cuda.current_device())
+        hit = _GEOMETRY.get(key)
+        if hit is not None:
+            _GEOMETRY.move_to_end(key)
+            return hit
+    s = _s64(pixel_position_reciprocal)
+    geo = (s, pixel_order(s) if tile_pixels else None)
+    if key is not None:
+        _GEOMETRY[key] = geo
+        while len(_GEOMETRY) > 4:
+            _GEOMETRY.popitem(last=False)
+    return geo
 
 
 def _ext32(reciprocal_extent):
@@ -115,12 +142,10 @@ class Slicer:
         self.ac = _arrays.to_dev(ac, torch.float32)
         M = self.ac.shape[0]
         self.plan = get_plan(2, M, -1, eps, tag="slice")
-        self.s = _s64(pixel_position_reciprocal)
+        self.s, self.order = _geometry(pixel_position_reciprocal, tile_pixels)
         self.P = self.s.shape[1]
         self.ext = _ext32(reciprocal_extent)
-        self.order = pixel_order(self.s) if tile_pixels else None
         self.plan.slice_prepare(self.ac)
-        self.plan.set_slice_route(0)  # automatic, probed again for this detector geometry
 
     def slice(self, quats, out=None, out_bf16=None, moments=None):
         q = _arrays.to_dev(quats, torch.float32).reshape(-1, 4)

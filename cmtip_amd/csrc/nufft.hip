@@ -55,7 +55,10 @@ struct cmtip_nufft_plan_s {
     const int* route_order = nullptr;
     int64_t route_P = -1;
     float route_ext = 0.f;
-    DevBuf probe;  // 2 int32: tiles staged in LDS, tiles probed
+    DevBuf probe;  // 3 int32: tiles staged in LDS, tiles probed, tiles within the tile margin
+    // kernel timer (cmtip_slice_timer): event pairs around the main slicing kernel
+    int timing = 0, timed = 0;
+    hipEvent_t tev[2 * 64] = {};
     DevBuf bin_of;    // N int32
     DevBuf count;     // nbins int32 (also the scatter cursors)
     DevBuf offset;    // nbins + 1 int32
@@ -1381,10 +1384,12 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     return CMTIP_OK;
 }
 
+int timer_mark(cmtip_nufft_plan_s* p, int which, hipStream_t st);
+
 template <int W>
 struct TileLaunch {
     static int run(cmtip_nufft_plan_s* p, const TileRot& rot, int64_t ntiles, float* out, uint16_t* out_bf16,
-                   double* part, hipStream_t st) {
+                   double* part, bool timed, hipStream_t st) {
         const int nbx = (p->n + kTileBs - 1) / kTileBs;
         const int64_t nbins = (int64_t)nbx * nbx * nbx;  // + 1 overflow bin
         CMTIP_TRY(p->bin_of.ensure(sizeof(int) * (size_t)ntiles));
@@ -1416,10 +1421,12 @@ struct TileLaunch {
             attr_set = true;
         }
         const unsigned grid = (unsigned)(cu_count() * std::max<size_t>(1, kMaxLds / (lds + kStaticLds)));
+        if (timed) CMTIP_TRY(timer_mark(p, 0, st));
         hipLaunchKernelGGL(slice_tiles_kernel<W>, dim3(grid), dim3(kTileBlock), lds, st, p->fine.as<const float>(),
                            p->n, p->beta_l2e, p->tc_hi, p->tc_lo, rot, nbx, p->perm.as<const int>(),
                            p->chunks.as<const int4>(), p->ctrl.as<int>(), out, out_bf16, part);
         CMTIP_LAUNCH_CHECK();
+        if (timed) CMTIP_TRY(timer_mark(p, 1, st));
         return CMTIP_OK;
     }
 };
@@ -1847,6 +1854,8 @@ extern "C" int cmtip_nufft_autotune(cmtip_nufft_plan p, void* c, void* f, int fl
 
 extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     if (!p) return CMTIP_OK;
+    for (hipEvent_t e : p->tev)
+        if (e) (void)hipEventDestroy(e);
     fft1d_free(p->fft);
     p->deconv.release();
     p->fine.release();
@@ -1941,9 +1950,19 @@ int auto_route(const cmtip_nufft_plan_s* p, int64_t P, int64_t n_orient) {
     return 2;
 }
 
+// Kernel timer: event `which` (0 start, 1 end) of the current timed call.
+int timer_mark(cmtip_nufft_plan_s* p, int which, hipStream_t st) {
+    if (hipEventRecord(p->tev[2 * p->timed + which], st) != hipSuccess) {
+        set_error("slice timer: hipEventRecord failed");
+        return CMTIP_ERR_HIP;
+    }
+    if (which == 1) ++p->timed;
+    return CMTIP_OK;
+}
+
 // Tile-binned route over all orientations (tiles are counted in 32-bit indices).
 int slice_tiles(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_t P, int64_t n_orient, float ext,
-                float* out, uint16_t* out_bf16, double* part, hipStream_t st) {
+                float* out, uint16_t* out_bf16, double* part, hipStream_t st, bool timed) {
     const int64_t tpo = (P + 63) / 64, ntiles = tpo * n_orient;
     if (ntiles > 0x7fffffffLL) {
         set_error("slice_real: %lld tiles is too many for the tile route", (long long)ntiles);
@@ -1952,7 +1971,7 @@ int slice_tiles(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_
     const TileRot rot{p->rinv.as<const double>(), s, order, P, tpo, ext};
     p->x = p->y = p->z = nullptr;  // the bins no longer hold registered points
     p->npts = std::max<int64_t>(p->npts, 1);
-    return dispatch_w<TileLaunch>(p->w, p, rot, ntiles, out, out_bf16, part, st);
+    return dispatch_w<TileLaunch>(p->w, p, rot, ntiles, out, out_bf16, part, timed, st);
 }
 
 // Binned route: bin sort of the rotated pixels -> LDS-subgrid interpolation, in
@@ -1989,6 +2008,35 @@ extern "C" int cmtip_slice_set_route(cmtip_nufft_plan p, int route) {
     }
     p->route_req = route;
     p->route_probed = 0;  // the next automatic call probes again
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_slice_timer(cmtip_nufft_plan p, int start, float* total_ms, int* count) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (start) {
+        for (hipEvent_t& e : p->tev)
+            if (!e && hipEventCreate(&e) != hipSuccess) {
+                set_error("slice timer: hipEventCreate failed");
+                return CMTIP_ERR_HIP;
+            }
+        p->timing = 1;
+        p->timed = 0;
+        return CMTIP_OK;
+    }
+    float sum = 0.f;
+    for (int i = 0; i < p->timed; ++i) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p->tev[2 * i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, p->tev[2 * i], p->tev[2 * i + 1]) != hipSuccess) {
+            set_error("slice timer: event timing failed");
+            return CMTIP_ERR_HIP;
+        }
+        sum += ms;
+    }
+    if (total_ms) *total_ms = sum;
+    if (count) *count = p->timed;
+    p->timing = 0;
+    p->timed = 0;
     return CMTIP_OK;
 }
 
@@ -2036,13 +2084,18 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
                            (int64_t)2 * kMomentSlots, 0.0);
         CMTIP_LAUNCH_CHECK();
     }
+    const bool timed = p->timing && p->timed < 64;
     if (route == 1) {
+        if (timed) CMTIP_TRY(timer_mark(p, 0, st));
         CMTIP_TRY(dispatch_w<SliceLaunch>(p->w, (const cmtip_nufft_plan_s*)p, s, order, P, n_orient, recip_extent,
                                           out, out_bf16, part, (int*)nullptr, st));
+        if (timed) CMTIP_TRY(timer_mark(p, 1, st));
     } else if (route == 3) {
-        CMTIP_TRY(slice_tiles(p, s, order, P, n_orient, recip_extent, out, out_bf16, part, st));
+        CMTIP_TRY(slice_tiles(p, s, order, P, n_orient, recip_extent, out, out_bf16, part, st, timed));
     } else {
+        if (timed) CMTIP_TRY(timer_mark(p, 0, st));
         CMTIP_TRY(slice_binned(p, s, order, P, n_orient, recip_extent, out, st));
+        if (timed) CMTIP_TRY(timer_mark(p, 1, st));
         if (out_bf16 || part) {
             const int64_t total = n_orient * P;
             hipLaunchKernelGGL(slice_finish_kernel,

@@ -161,6 +161,14 @@ int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient
 int cmtip_slice_set_route(cmtip_nufft_plan plan, int route);
 int cmtip_slice_get_route(cmtip_nufft_plan plan, int* requested, int* last_used);
 
+/* Kernel timer of a plan's slicing: start = 1 starts recording HIP events on the
+ * stream around the main kernel of each following cmtip_slice_real call (the
+ * interpolation kernel of the fused and tile routes, the whole binned-points route), up
+ * to 64 calls, without synchronising; start = 0 waits for the recorded events and
+ * returns their total time and count, then stops.  A measurement hook with no
+ * counterpart in the reference. */
+int cmtip_slice_timer(cmtip_nufft_plan plan, int start, float* total_ms, int* count);
+
 /* ---------------------------------------------------------------- matching
  * Orientation matching (cmtip/alignment/align.py:113-131): for every image i, the
  * reference r minimising the weighted Euclidean distance

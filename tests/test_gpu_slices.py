@@ -12,6 +12,17 @@ from tests.gpu_util import dev, rel_l2, say
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def _automatic_route_after():
+    # tests here force slicing routes on the shared slicing plans; later callers get the
+    # automatic choice back
+    yield
+    from cmtip_amd.nufft import plan as NP
+    for key, p in list(NP._CACHE.items()):
+        if key[-1] == "slice":
+            p.set_slice_route(0)
+
+
 def test_rotate_coords_bit_exact():
     s, ext = synth.detector(32)
     q = synth.random_quaternions(200, seed=4)
