@@ -56,6 +56,8 @@ SIGNATURES = {
     "cmtip_match_argmin_f32": [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_eudist_f32": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_match_topk_bf16": [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _i, _vp, _vp],
+    "cmtip_match_cross_bf16": [_vp, _i64, _vp, _i64, _i64, _vp, _vp],
+    "cmtip_match_topk_cross": [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _i, _vp, _vp],
     "cmtip_match_rescore": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _i, _vp, _vp, _vp],
     "cmtip_image_stats_masked": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_match_prepare_masked": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -63,6 +63,8 @@ def _s64(pixel_position_reciprocal):
     return s.reshape(3, -1).contiguous()
 
 
+CROSS_BLOCK_ELEMS = 1 << 28  # FP32 cross-term block of the library GEMM path (1 GiB)
+
 _GEOMETRY = collections.OrderedDict()
 
 
@@ -182,7 +184,9 @@ class _Matcher:
     per-image masks (1 = valid pixel) the cross term runs over 2P columns:
     [y^2, y] . [-m w^2 / 2, m w^2 (b - c)] with y = gamma a - c."""
 
-    def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk, masks=None):
+    def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk, masks=None, cross="lib"):
+        assert cross in ("lib", "fused")
+        self.cross = cross
         self.P = pixel_position_reciprocal.shape[-1]
         B = _arrays.to_dev(slices_, torch.float32).reshape(-1, self.P)
         self.B = B
@@ -242,6 +246,8 @@ class _Matcher:
             cand = torch.full((self.n_img, self.k), EMPTY_KEY, dtype=torch.int64, device=dev)
             gamma_ref = torch.empty(n_ref, dtype=torch.float64, device=dev)
             A16 = torch.empty((quot, self.K), dtype=torch.int16, device=dev)
+            blk = max(1, min(quot, CROSS_BLOCK_ELEMS // max(self.n_img, 1)))
+            Dt = None
         elif masked:
             A2 = torch.empty((quot, self.K), dtype=torch.float32, device=dev)
         for nb in range(nbatches):
@@ -261,9 +267,21 @@ class _Matcher:
                 else:
                     check(L.cmtip_center_rows(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(self.w2), None,
                                               ptr(A16), ptr(an), stream()), "cmtip_center_rows")
-                check(L.cmtip_match_topk_bf16(ptr(A16), quot, start, ptr(self.Bw16), self.n_img, self.K, ptr(an),
-                                              ptr(self.bn), None, self.k, ptr(cand), stream()),
-                      "cmtip_match_topk_bf16")
+                if self.cross == "fused":
+                    check(L.cmtip_match_topk_bf16(ptr(A16), quot, start, ptr(self.Bw16), self.n_img, self.K,
+                                                  ptr(an), ptr(self.bn), None, self.k, ptr(cand), stream()),
+                          "cmtip_match_topk_bf16")
+                else:
+                    # library GEMM into an FP32 block of at most CROSS_BLOCK_ELEMS, then top-k
+                    if Dt is None:
+                        Dt = torch.empty(blk * self.n_img, dtype=torch.float32, device=dev)
+                    for r0 in range(0, quot, blk):
+                        r1 = min(quot, r0 + blk)
+                        check(L.cmtip_match_cross_bf16(ptr(A16[r0:r1]), r1 - r0, ptr(self.Bw16), self.n_img,
+                                                       self.K, ptr(Dt), stream()), "cmtip_match_cross_bf16")
+                        check(L.cmtip_match_topk_cross(ptr(Dt), r1 - r0, start + r0, self.n_img, ptr(an[r0:r1]),
+                                                       ptr(self.bn), None, self.k, ptr(cand), stream()),
+                              "cmtip_match_topk_cross")
             else:
                 if masked:
                     check(L.cmtip_center_rows_sq(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), ptr(A2), None,
@@ -319,11 +337,13 @@ def key_d2(best):
 def match_orientations(generation, pixel_position_reciprocal, reciprocal_extent, slices_, ac, n_ref_orientations,
                        nbatches=1, order=0, use_gpu=False, true_orientations=None, *, ref_orientations=None,
                        precision="fp32", topk=8, eps=DEFAULT_EPS, seed=0, verbose=False, return_index=False,
-                       masks=None):
+                       masks=None, cross="lib"):
     """Quaternions (n_images, 4) float32 of the best-matching reference slices (align.py:68-132).
 
     precision "fp32": cross term in FP32 on MFMA, argmin fused.  "bf16": cross term in
-    BF16 on MFMA with FP32 accumulation, the ``topk`` best per image rescored in FP32.
+    BF16 on MFMA with FP32 accumulation, the ``topk`` best per image rescored in FP32;
+    ``cross`` "lib" takes the cross term from the hipBLASLt GEMM then reduces it to the
+    top-k, "fused" runs this library's MFMA kernel with the top-k in its epilogue.
     masks (n_images, P), 1 = valid pixel: distances over each image's valid pixels only
     (the data std is then taken over valid pixels); not in the reference.
     """
@@ -332,7 +352,7 @@ def match_orientations(generation, pixel_position_reciprocal, reciprocal_extent,
     ref = _ref_set(n_ref_orientations, ref_orientations, true_orientations, seed)
     as_torch = _arrays.is_torch(slices_, ac)
     slicer = Slicer(ac, pixel_position_reciprocal, reciprocal_extent, eps)
-    m = _Matcher(pixel_position_reciprocal, slices_, order, precision, topk, masks)
+    m = _Matcher(pixel_position_reciprocal, slices_, order, precision, topk, masks, cross)
     ref_dev = torch.from_numpy(ref).to(m.B.device)
     best = m.run(slicer, ref_dev, nbatches, verbose)
     index = key_index(best)

@@ -62,7 +62,9 @@ def build(verbose=False):
         if verbose and err.strip():
             print(err, file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    # hipBLASLt (the BF16 cross-term GEMM); with torch imported first, the library binds
+    # to the copy torch has loaded (same soname), as it does for the HIP runtime
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lhipblaslt"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

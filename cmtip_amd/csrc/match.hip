@@ -8,6 +8,17 @@
 // then with one 64-bit atomicMin per (image, workgroup column).  Keys pack
 // (float bits of d2 >= 0) << 32 | r, so the unsigned order is (d2, r) and ties keep
 // the smallest reference index, as np.argmin does.
+//
+// The BF16 form can also take its cross term from the hipBLASLt GEMM
+// (cmtip_match_cross_bf16, a plain library GEMM on MFMA with FP32 accumulation and
+// output) and reduce the FP32 cross-term block to per-image candidates in a separate
+// pass (cmtip_match_topk_cross).
+#include <hipblaslt/hipblaslt.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "common.h"
 
 namespace {
@@ -795,6 +806,168 @@ extern "C" int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int
     hipLaunchKernelGGL(match_argmin_f32_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
                        A, n_a, 0u, B, n_b, P, an, bn, (const double*)nullptr, (unsigned long long*)nullptr, D,
                        (int)tiles_m);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- top-k from a cross-term block
+// One wave per image i: lanes scan references r = lane, lane + 64, ... of the row
+// Dt[i, 0..n_ref) and keep their k smallest keys (d2, r) in registers; the image's
+// earlier candidates (previous reference batches) join the lists of lanes 0..k-1; k
+// rounds of a wave-wide minimum over the list heads then give the merged top-k, written
+// back without atomics (ties keep the smallest reference index).
+constexpr int kCrossWaves = 4;
+
+__global__ void __launch_bounds__(64 * kCrossWaves) topk_cross_kernel(const float* __restrict__ Dt, int64_t n_ref,
+                                                                      uint32_t ref_offset, int64_t n_img,
+                                                                      const float* __restrict__ an,
+                                                                      const float* __restrict__ bn,
+                                                                      const double* __restrict__ gamma, int k,
+                                                                      unsigned long long* __restrict__ cand) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * kCrossWaves + (threadIdx.x >> 6);
+    if (i >= n_img) return;  // whole waves leave together
+    const float g = gamma ? (float)gamma[0] : 1.0f;
+    const float g2 = g * g, m2g = -2.0f * g;
+    const float bni = bn[i];
+    unsigned long long lst[kTopK];
+#pragma unroll
+    for (int s = 0; s < kTopK; ++s) lst[s] = kEmpty;
+    unsigned long long* slots = cand + i * k;
+    if (lane < k) topk_insert(lst, slots[lane]);
+    const float* row = Dt + i * n_ref;
+    for (int64_t r = lane; r < n_ref; r += 64)
+        topk_insert(lst, pack_key(fmaf(g2, an[r], fmaf(m2g, row[r], bni)), (uint32_t)r + ref_offset));
+    // k rounds: the wave minimum of the list heads; its lane pops it
+    for (int s = 0; s < k; ++s) {
+        unsigned long long m = lst[0];
+        for (int off = 32; off > 0; off >>= 1) m = umin64(m, shfl_xor_u64(m, off));
+        if (lst[0] == m && m != kEmpty) {
+#pragma unroll
+            for (int q = 0; q < kTopK - 1; ++q) lst[q] = lst[q + 1];
+            lst[kTopK - 1] = kEmpty;
+        }
+        if (lane == 0) slots[s] = m;
+    }
+}
+
+// ---------------------------------------------------------------- hipBLASLt cross term
+struct BlasLt {
+    hipblasLtHandle_t handle = nullptr;
+    void* workspace = nullptr;
+    size_t ws_size = 0;
+    std::map<std::tuple<int64_t, int64_t, int64_t>, hipblasLtMatmulAlgo_t> algos;
+};
+std::mutex g_blas_mutex;
+std::map<int, BlasLt> g_blas;  // per device; kept until the process ends
+
+constexpr size_t kBlasWorkspace = (size_t)64 << 20;
+
+int blas_fail(const char* what, hipblasStatus_t st) {
+    set_error("match_cross_bf16: %s failed (hipblasStatus %d)", what, (int)st);
+    return CMTIP_ERR_LAUNCH;
+}
+
+}  // namespace
+
+extern "C" int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const uint16_t* B, int64_t n_img, int64_t K,
+                                      float* Dt, void* stream) {
+    if (n_ref < 0 || n_img < 0 || K < 1 || (n_ref > 0 && n_img > 0 && (!A || !B || !Dt))) {
+        set_error("match_cross_bf16: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_ref == 0 || n_img == 0) return CMTIP_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("match_cross_bf16: hipGetDevice failed");
+        return CMTIP_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_blas_mutex);
+    BlasLt& bl = g_blas[dev];
+    hipblasStatus_t st;
+    if (!bl.handle) {
+        if ((st = hipblasLtCreate(&bl.handle)) != HIPBLAS_STATUS_SUCCESS) return blas_fail("hipblasLtCreate", st);
+        if (hipMalloc(&bl.workspace, kBlasWorkspace) != hipSuccess) {
+            set_error("match_cross_bf16: workspace allocation failed");
+            return CMTIP_ERR_ALLOC;
+        }
+        bl.ws_size = kBlasWorkspace;
+    }
+    // column-major view: Dt (n_ref x n_img, ld n_ref) = A^T (n_ref x K) * B (K x n_img),
+    // i.e. row-major Dt[i][r] = <A_r, B_i> (A, B row-major with K columns)
+    hipblasLtMatmulDesc_t desc = nullptr;
+    hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
+    int rc = CMTIP_OK;
+    const hipblasOperation_t opA = HIPBLAS_OP_T, opB = HIPBLAS_OP_N;
+    do {
+        if ((st = hipblasLtMatmulDescCreate(&desc, HIPBLAS_COMPUTE_32F, HIP_R_32F)) != HIPBLAS_STATUS_SUCCESS ||
+            (st = hipblasLtMatmulDescSetAttribute(desc, HIPBLASLT_MATMUL_DESC_TRANSA, &opA, sizeof(opA))) !=
+                HIPBLAS_STATUS_SUCCESS ||
+            (st = hipblasLtMatmulDescSetAttribute(desc, HIPBLASLT_MATMUL_DESC_TRANSB, &opB, sizeof(opB))) !=
+                HIPBLAS_STATUS_SUCCESS) {
+            rc = blas_fail("matmul descriptor", st);
+            break;
+        }
+        if ((st = hipblasLtMatrixLayoutCreate(&la, HIP_R_16BF, K, n_ref, K)) != HIPBLAS_STATUS_SUCCESS ||
+            (st = hipblasLtMatrixLayoutCreate(&lb, HIP_R_16BF, K, n_img, K)) != HIPBLAS_STATUS_SUCCESS ||
+            (st = hipblasLtMatrixLayoutCreate(&lc, HIP_R_32F, n_ref, n_img, n_ref)) != HIPBLAS_STATUS_SUCCESS) {
+            rc = blas_fail("matrix layouts", st);
+            break;
+        }
+        const auto key = std::make_tuple(n_ref, n_img, K);
+        auto it = bl.algos.find(key);
+        if (it == bl.algos.end()) {
+            hipblasLtMatmulPreference_t pref = nullptr;
+            hipblasLtMatmulHeuristicResult_t heur[4];
+            int found = 0;
+            const uint64_t ws = bl.ws_size;
+            st = hipblasLtMatmulPreferenceCreate(&pref);
+            if (st == HIPBLAS_STATUS_SUCCESS)
+                st = hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &ws,
+                                                           sizeof(ws));
+            if (st == HIPBLAS_STATUS_SUCCESS)
+                st = hipblasLtMatmulAlgoGetHeuristic(bl.handle, desc, la, lb, lc, lc, pref, 4, heur, &found);
+            if (pref) (void)hipblasLtMatmulPreferenceDestroy(pref);
+            if (st != HIPBLAS_STATUS_SUCCESS || found < 1) {
+                rc = blas_fail("algorithm heuristic", st);
+                break;
+            }
+            it = bl.algos.emplace(key, heur[0].algo).first;
+        }
+        const float alpha = 1.0f, beta = 0.0f;
+        st = hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc, &it->second, bl.workspace,
+                             bl.ws_size, as_stream(stream));
+        if (st != HIPBLAS_STATUS_SUCCESS) rc = blas_fail("hipblasLtMatmul", st);
+    } while (false);
+    if (la) (void)hipblasLtMatrixLayoutDestroy(la);
+    if (lb) (void)hipblasLtMatrixLayoutDestroy(lb);
+    if (lc) (void)hipblasLtMatrixLayoutDestroy(lc);
+    if (desc) (void)hipblasLtMatmulDescDestroy(desc);
+    return rc;
+}
+
+extern "C" int cmtip_match_topk_cross(const float* Dt, int64_t n_ref, int64_t ref_offset, int64_t n_img,
+                                      const float* an, const float* bn, const double* gamma, int k,
+                                      unsigned long long* cand, void* stream) {
+    if (k < 1 || k > kTopK) {
+        set_error("match_topk_cross: k must be in [1, %d]", kTopK);
+        return CMTIP_ERR_ARG;
+    }
+    if (n_ref < 0 || n_img < 0 || ref_offset < 0 || ref_offset + n_ref > 0xffffffffLL ||
+        (n_ref > 0 && n_img > 0 && (!Dt || !an || !bn || !cand))) {
+        set_error("match_topk_cross: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_ref == 0 || n_img == 0) return CMTIP_OK;
+    const int64_t blocks = (n_img + kCrossWaves - 1) / kCrossWaves;
+    if (blocks > 0x7fffffffLL) {
+        set_error("match_topk_cross: too many images");
+        return CMTIP_ERR_ARG;
+    }
+    hipLaunchKernelGGL(topk_cross_kernel, dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0, as_stream(stream), Dt,
+                       n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

@@ -228,6 +228,16 @@ int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t ref_offset,
                           const uint16_t* Bw, int64_t n_img, int64_t P, const float* an,
                           const float* bn, const double* gamma, int k,
                           unsigned long long* cand, void* stream);
+/* The same BF16 cross term in two steps, for large blocks: cmtip_match_cross_bf16
+ * writes the FP32 cross-term block Dt[i * n_ref + r] = <A_r, B_i> (bfloat16 rows of K
+ * elements, FP32 accumulation) with the hipBLASLt GEMM; cmtip_match_topk_cross then
+ * merges d2 = gamma^2 an[r] - 2 gamma Dt[i, r] + bn[i] into the image's k candidates
+ * (one wave per image, no atomics; ties keep the smallest reference index).  Replaces
+ * the same sklearn euclidean_distances + argmin (align.py:35,128-131). */
+int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const uint16_t* B, int64_t n_img, int64_t K,
+                           float* Dt, void* stream);
+int cmtip_match_topk_cross(const float* Dt, int64_t n_ref, int64_t ref_offset, int64_t n_img, const float* an,
+                           const float* bn, const double* gamma, int k, unsigned long long* cand, void* stream);
 int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, const float* B, int64_t n_img,
                         int64_t P, const float* w2, const double* gamma_of_ref, int k,
                         const unsigned long long* cand, unsigned long long* best, void* stream);

@@ -39,13 +39,14 @@ def test_calc_eudist_vs_sklearn():
     assert np.abs(D - want).max() < 1e-5 * want.max()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("precision,cross", [("fp32", "lib"), ("bf16", "lib"), ("bf16", "fused")])
 @pytest.mark.parametrize("nbatches,order", [(1, 0), (3, 0), (2, -2)])
-def test_match_vs_oracle(precision, nbatches, order):
+def test_match_vs_oracle(precision, cross, nbatches, order):
     s, ext, ac, ref, _, imgs = _dataset()
-    say(f"match precision={precision} nbatches={nbatches} order={order}")
+    say(f"match precision={precision} cross={cross} nbatches={nbatches} order={order}")
     q, idx = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), nbatches=nbatches, order=order,
-                                         ref_orientations=ref, precision=precision, return_index=True)
+                                         ref_orientations=ref, precision=precision, return_index=True,
+                                         cross=cross)
     want, D = OA.match(ref, s, ext, imgs, ac, nbatches=nbatches, order=order)
     ok = OA.tie_band_ok(D, rel=BAND)
     assert ok.mean() > 0.5, "too few images outside the tie band for a meaningful check"
@@ -158,6 +159,66 @@ def test_match_kernels_vs_fp64(n_ref, n_img, P):
     # candidate lists are sorted and hold distinct references
     c = cand.cpu().numpy().view(np.uint64)
     assert np.all(np.diff(c.astype(np.float64), axis=1) >= 0)
+    # the same cross term from the library GEMM, in two reference blocks, then top-k
+    say(f"bf16 library cross term + top-k {n_ref}x{n_img}x{P}")
+    cand3 = torch.full((n_img, 8), -1, dtype=torch.int64, device=d)
+    half = n_ref // 2
+    Dt = torch.empty((n_img, n_ref - half), device=d)
+    for r0, r1 in ((0, half), (half, n_ref)):
+        check(L.cmtip_match_cross_bf16(ptr(A16[r0:r1]), r1 - r0, ptr(Bw16), n_img, P, ptr(Dt), stream()))
+        check(L.cmtip_match_topk_cross(ptr(Dt), r1 - r0, r0, n_img, ptr(an[r0:r1]), ptr(bn), ptr(gamma), 8,
+                                       ptr(cand3), stream()))
+    best3 = torch.full((n_img,), -1, dtype=torch.int64, device=d)
+    check(L.cmtip_match_rescore(ptr(A), n_ref, ptr(B), n_img, P, ptr(w2), ptr(gref), 8, ptr(cand3), ptr(best3),
+                                stream()))
+    got3 = (best3 & 0xFFFFFFFF).cpu().numpy()
+    assert np.array_equal(got3[ok2], want[ok2]), f"lib mismatches: {(got3[ok2] != want[ok2]).sum()} of {ok2.sum()}"
+    c3 = cand3.cpu().numpy().view(np.uint64)
+    assert np.all(np.diff(c3.astype(np.float64), axis=1) >= 0)
+    # both BF16 forms keep the same candidates apart from near-ties of the approximate distances
+    same = np.mean([len(set(a & 0xFFFFFFFF) & set(b & 0xFFFFFFFF)) / 8 for a, b in zip(c, c3)])
+    assert same > 0.9
+
+
+def test_topk_cross_exact():
+    # keys formed and merged exactly as numpy forms them in float32 (gamma = 1), over two
+    # reference batches, with repeated distances (ties keep the smaller reference index)
+    from cmtip_amd._lib import check, lib, ptr, stream
+    rng = np.random.default_rng(5)
+    d = dev()
+    n_img, n_ref, k = 37, 300, 8
+    Dt = np.round(rng.random((n_img, n_ref)) * 64).astype(np.float32) / 64  # many equal values
+    an = (rng.random(n_ref) * 2).astype(np.float32)
+    bn = (rng.random(n_img) * 2).astype(np.float32)
+    L = lib()
+    cand = torch.full((n_img, k), -1, dtype=torch.int64, device=d)
+    an_d, bn_d = torch.as_tensor(an, device=d), torch.as_tensor(bn, device=d)
+    say("top-k from a cross-term block")
+    for r0, r1 in ((0, 180), (180, 300)):
+        Db = torch.as_tensor(np.ascontiguousarray(Dt[:, r0:r1]), device=d)
+        check(L.cmtip_match_topk_cross(ptr(Db), r1 - r0, r0, n_img, ptr(an_d[r0:r1]), ptr(bn_d), None, k,
+                                       ptr(cand), stream()))
+    d2 = np.maximum(an[None, :] + (bn[:, None] - np.float32(2) * Dt), np.float32(0)).astype(np.float32)
+    keys = (d2.view(np.uint32).astype(np.uint64) << np.uint64(32)) | np.arange(n_ref, dtype=np.uint64)[None, :]
+    want = np.sort(keys, axis=1)[:, :k]
+    got = cand.cpu().numpy().view(np.uint64)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_cross_bf16_vs_fp32_product():
+    from cmtip_amd._lib import check, lib, ptr, stream
+    d = dev()
+    g = torch.Generator(device=d).manual_seed(3)
+    n_ref, n_img, K = 1000, 600, 2 * 1234
+    A16 = torch.randn(n_ref, K, device=d, generator=g).to(torch.bfloat16)
+    B16 = torch.randn(n_img, K, device=d, generator=g).to(torch.bfloat16)
+    Dt = torch.empty((n_img, n_ref), device=d)
+    say("library bf16 cross term")
+    check(lib().cmtip_match_cross_bf16(ptr(A16.view(torch.int16)), n_ref, ptr(B16.view(torch.int16)), n_img, K,
+                                       ptr(Dt), stream()))
+    want = B16.double() @ A16.double().T  # exact products of the bf16 values, FP64 sums
+    err = (Dt.double() - want).abs().max() / want.abs().max()
+    assert float(err) < 1e-5
 
 
 def test_generate_weights():
@@ -215,9 +276,9 @@ def _masked_oracle(ref, s, ext, imgs, ac, masks, nbatches):
     return D
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("precision,cross", [("fp32", "lib"), ("bf16", "lib"), ("bf16", "fused")])
 @pytest.mark.parametrize("side", [32, 24])
-def test_masked_matching_vs_oracle(precision, side):
+def test_masked_matching_vs_oracle(precision, cross, side):
     # two detector binnings with random beamstop masks of uneven size per image
     s, ext, ac, ref, _, imgs = _dataset(n_side=side, n_ref=200, n_img=40, seed=side)
     rng = np.random.default_rng(side)
@@ -231,7 +292,7 @@ def test_masked_matching_vs_oracle(precision, side):
         masks[i, rng.random(P) < 0.05] = 0
     say(f"masked matching {precision} side={side}")
     q, idx = alignment.match_orientations(0, s, ext, imgs, ac, len(ref), nbatches=2, ref_orientations=ref,
-                                         precision=precision, return_index=True, masks=masks)
+                                         precision=precision, return_index=True, masks=masks, cross=cross)
     D = _masked_oracle(ref, s, ext, imgs, ac, masks, 2)
     want = np.argmin(D, axis=0)
     srt = np.sort(D, axis=0)
