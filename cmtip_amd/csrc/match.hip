@@ -631,7 +631,86 @@ __global__ void __launch_bounds__(kThreads, 2)
 }
 
 // ---------------------------------------------------------------- FP32 rescoring
-// One workgroup per image; each wave takes candidates s = wave, wave + 4, ...
+// One workgroup per image.  rescore4_kernel (P a multiple of 4, 16-byte aligned rows)
+// sweeps the pixels once in float4 quads: each quad of the image, weights and mask is
+// loaded once and the same quad of all k candidate rows is loaded together (k loads in
+// flight), with one FP32 accumulator per candidate; then a block reduction.
+// rescore_kernel is the scalar form for other shapes: each wave takes candidates
+// s = wave, wave + 4, ...
+
+__global__ void __launch_bounds__(kThreads) rescore4_kernel(const float* __restrict__ A, int64_t n_ref_total,
+                                                            const float* __restrict__ B, int64_t P,
+                                                            const float* __restrict__ w2,
+                                                            const double* __restrict__ gamma_of_ref, int k,
+                                                            const unsigned long long* __restrict__ cand,
+                                                            unsigned long long* __restrict__ best,
+                                                            const uint8_t* __restrict__ mask) {
+    __shared__ float red[kThreads / 64][kTopK];
+    const int64_t i = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float4* a4[kTopK];
+    float g[kTopK], acc[kTopK];
+    uint32_t r_of[kTopK];
+#pragma unroll
+    for (int s = 0; s < kTopK; ++s) {
+        const unsigned long long key = s < k ? cand[i * k + s] : kEmpty;
+        const uint32_t r = (uint32_t)(key & 0xffffffffu);
+        const bool ok = key != kEmpty && (int64_t)r < n_ref_total;
+        r_of[s] = ok ? r : 0xffffffffu;  // candidates past k are never valid
+        g[s] = ok ? (float)gamma_of_ref[r] : 0.f;
+        a4[s] = reinterpret_cast<const float4*>(A + (ok ? (int64_t)r : 0) * P);
+        acc[s] = 0.f;
+    }
+    const float4* b4 = reinterpret_cast<const float4*>(B + i * P);
+    const float4* w4 = reinterpret_cast<const float4*>(w2);
+    const uchar4* m4 = mask ? reinterpret_cast<const uchar4*>(mask + i * P) : nullptr;
+    const int64_t P4 = P / 4;
+    for (int64_t q = threadIdx.x; q < P4; q += kThreads) {
+        const float4 bv = b4[q];
+        float4 wv = w4[q];
+        if (m4) {
+            const uchar4 mv = m4[q];
+            wv.x = mv.x ? wv.x : 0.f;
+            wv.y = mv.y ? wv.y : 0.f;
+            wv.z = mv.z ? wv.z : 0.f;
+            wv.w = mv.w ? wv.w : 0.f;
+        }
+        float4 av[kTopK];
+#pragma unroll
+        for (int s = 0; s < kTopK; ++s)
+            if (r_of[s] != 0xffffffffu) av[s] = a4[s][q];  // same for every thread of the block
+#pragma unroll
+        for (int s = 0; s < kTopK; ++s) {
+            if (r_of[s] == 0xffffffffu) continue;
+            float d = fmaf(g[s], av[s].x, -bv.x);
+            float t = fmaf(wv.x * d, d, acc[s]);
+            d = fmaf(g[s], av[s].y, -bv.y);
+            t = fmaf(wv.y * d, d, t);
+            d = fmaf(g[s], av[s].z, -bv.z);
+            t = fmaf(wv.z * d, d, t);
+            d = fmaf(g[s], av[s].w, -bv.w);
+            acc[s] = fmaf(wv.w * d, d, t);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kTopK; ++s) {
+        float v = acc[s];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][s] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = kEmpty;
+        for (int s = 0; s < k; ++s) {
+            if (r_of[s] == 0xffffffffu) continue;
+            float v = 0.f;
+            for (int w = 0; w < kThreads / 64; ++w) v += red[w][s];
+            m = umin64(m, pack_key(v, r_of[s]));
+        }
+        best[i] = m;
+    }
+}
+
 
 __global__ void __launch_bounds__(kThreads) rescore_kernel(const float* __restrict__ A, int64_t n_ref_total,
                                                            const float* __restrict__ B, int64_t P,
@@ -1009,8 +1088,15 @@ extern "C" int cmtip_match_rescore_masked(const float* A_all, int64_t n_ref_tota
         set_error("match_rescore: too many images");
         return CMTIP_ERR_ARG;
     }
-    hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
-                       n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);
+    const bool quads = P % 4 == 0 && ((reinterpret_cast<uintptr_t>(A_all) | reinterpret_cast<uintptr_t>(B) |
+                                        reinterpret_cast<uintptr_t>(w2)) & 15) == 0 &&
+                       (reinterpret_cast<uintptr_t>(mask) & 3) == 0;
+    if (quads)
+        hipLaunchKernelGGL(rescore4_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
+                           n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);
+    else
+        hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
+                           n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
