@@ -343,17 +343,20 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
     if args.part == "solve":
         # the autocorrelation solve of the workload: kernel and rhs spreads, then CG
         imgs = alignment.Slicer(ac, s, ext).slice(q)
-        t = time.perf_counter()
-        acs, info = AC.solve_ac(0, s, ext, imgs.reshape(n_img, 1, -1), M, orientations=q, maxiter=20, rtol=0.0,
-                                rlambda=1e-4, return_info=True)
-        torch.cuda.synchronize()
-        t_total = time.perf_counter() - t
+        t_calls = []
+        for _ in range(2):  # the first call also creates the plans and the solver (cached afterwards)
+            t = time.perf_counter()
+            acs, info = AC.solve_ac(0, s, ext, imgs.reshape(n_img, 1, -1), M, orientations=q, maxiter=20,
+                                    rtol=0.0, rlambda=1e-4, return_info=True)
+            torch.cuda.synchronize()
+            t_calls.append(time.perf_counter() - t)
+        t_total = t_calls[-1]
         solver = AC.get_solver(M)
         d = torch.rand(M ** 3, device=dev)
         xx = torch.zeros(M ** 3, device=dev)
         t_cg = tm(lambda: solver.cg(d, xx.zero_(), 1.0, 1e-4, 1e3, 20, 0.0), reps=3)
-        res = {"part": "solve", "M": M, "points": N, "solve_ms": t_total * 1e3, "iterations": info["iterations"],
-               "cg_ms_per_iteration": t_cg / 20 * 1e3}
+        res = {"part": "solve", "M": M, "points": N, "solve_ms": t_total * 1e3, "first_call_ms": t_calls[0] * 1e3,
+               "iterations": info["iterations"], "cg_ms_per_iteration": t_cg / 20 * 1e3}
         log(f"[bench] solve {res}")
         print(json.dumps(res), flush=True)
         return

@@ -135,8 +135,10 @@ __host__ __device__ inline int bin_size_for(int w) { return w <= 12 ? 17 - w : 2
 // an odd row pitch.
 __host__ __device__ inline void interp_pitch(int S, int W, bool cplx, int* SP, int* SSP) {
     if (cplx) {
-        *SP = S + 1;
-        *SSP = S * (S + 1);
+        // in complex cells; wide rows need even pitches (16-byte aligned pairs)
+        const int A = CMTIP_INTERP_READS == 1 ? 1 : 2;
+        *SP = (S + A - 1) / A * A + A;
+        *SSP = S * *SP + A;
         return;
     }
     const int A = CMTIP_INTERP_READS == 1 ? 1 : ((W == 7 && CMTIP_INTERP_READS == 4) ? 4 : 2);
@@ -144,10 +146,29 @@ __host__ __device__ inline void interp_pitch(int S, int W, bool cplx, int* SP, i
     *SSP = S * *SP + A;
 }
 
+// Row / plane pitch (in cells) of the spreading subgrid of 64-bit accumulators: odd
+// row pitch, plane pitch padded by 5 cells so that neighbouring planes start in other
+// banks (measured over 81.92 M points: type 1 at M = 81 17.2 ms with no plane pad,
+// 15.9 / 14.1 / 13.9 ms with pads 1 / 3 / 5; at 162 modes 11.6 -> 9.8 ms).
+#ifndef CMTIP_SPREAD_SP_PAD
+#define CMTIP_SPREAD_SP_PAD 1
+#endif
+#ifndef CMTIP_SPREAD_SSP_PAD
+#define CMTIP_SPREAD_SSP_PAD 5
+#endif
+__host__ __device__ inline void spread_pitch(int S, int* SP, int* SSP) {
+    *SP = S + CMTIP_SPREAD_SP_PAD;
+    *SSP = S * *SP + CMTIP_SPREAD_SSP_PAD;
+}
+
 // LDS bytes of the spread (64-bit accumulators) or interpolation (float / float2) subgrid.
 inline size_t subgrid_lds(int bs, int W, bool spread, bool cplx) {
     const int S = bs + W - 1;
-    if (spread) return (size_t)S * S * (S + 1) * 8 * (cplx ? 2 : 1);
+    if (spread) {
+        int SP, SSP;
+        spread_pitch(S, &SP, &SSP);
+        return (size_t)S * SSP * 8 * (cplx ? 2 : 1);
+    }
     int SP, SSP;
     interp_pitch(S, W, cplx, &SP, &SSP);
     return (size_t)S * SSP * (cplx ? 8 : 4);
@@ -402,6 +423,53 @@ struct WideRow {
     }
 };
 
+// Complex rows (float2 cells) with pitches multiples of two cells: every row of W cells
+// is read as 16-byte pairs plus one or two single cells whatever its alignment
+// m = start mod 2: odd W: pairs at m, m + 2, ... and a single at (m ? 0 : W - 1); even W:
+// pairs at m, m + 2, ..., singles at (m ? 0 : W - 2) and W - 1.
+template <int W>
+struct WideRow2 {
+    static constexpr int kPairs = (W & 1) ? (W - 1) / 2 : W / 2 - 1;
+    int op, os;
+    __device__ __forceinline__ explicit WideRow2(int start) {
+        const int m = start & 1;
+        op = m;
+        os = m ? 0 : W - ((W & 1) ? 1 : 2);
+    }
+    __device__ __forceinline__ void load_offsets(int* z) const {
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < kPairs; ++i) {
+            z[k++] = op + 2 * i;
+            z[k++] = op + 2 * i + 1;
+        }
+        z[k++] = os;
+        if (!(W & 1)) z[k++] = W - 1;
+    }
+    __device__ __forceinline__ float2 dot(const float2* row, const float* wl) const {
+        float qr = 0.f, qi = 0.f;
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < kPairs; ++i) {
+            const float4 v = *reinterpret_cast<const float4*>(row + op + 2 * i);
+            qr = fmaf(wl[k], v.x, qr);
+            qi = fmaf(wl[k], v.y, qi);
+            qr = fmaf(wl[k + 1], v.z, qr);
+            qi = fmaf(wl[k + 1], v.w, qi);
+            k += 2;
+        }
+        const float2 a = row[os];
+        qr = fmaf(wl[k], a.x, qr);
+        qi = fmaf(wl[k], a.y, qi);
+        if (!(W & 1)) {
+            const float2 b = row[W - 1];
+            qr = fmaf(wl[k + 1], b.x, qr);
+            qi = fmaf(wl[k + 1], b.y, qi);
+        }
+        return make_float2(qr, qi);
+    }
+};
+
 // ES weights of one axis at the load-order offsets z of a WideRow.
 template <int W>
 __device__ __forceinline__ void weights_at(float t, const int* z, float beta_l2e, float* wl) {
@@ -441,8 +509,11 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
     typedef typename Cell<CPLX>::T T;
     constexpr int NC = CPLX ? 2 : 1;  // integer accumulators per cell
     extern __shared__ unsigned long long acc[];
-    // LDS layout (x * S + y) * SP + z with an odd row pitch SP = S + 1 (bank spread)
-    const int S = bs + W - 1, SP = S + 1, SSP = S * SP, S3 = S * S * S, L3 = S * S * SP;
+    // LDS layout x * SSP + y * SP + z (spread_pitch)
+    const int S = bs + W - 1, S3 = S * S * S;
+    int SP, SSP;
+    spread_pitch(S, &SP, &SSP);
+    const int L3 = S * SSP;
     __shared__ int chunk_id;
     const int total = ctrl[0];
     const int sc = spread_scale_exp(c ? __uint_as_float(*cmax_bits) : 1.0f);
@@ -477,7 +548,7 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
             } else {
                 vr = ldexpf(c ? reinterpret_cast<const float*>(c)[perm[k]] : 1.0f, sc);
             }
-            unsigned long long* base = acc + NC * ((lx * S + ly) * SP + lz);
+            unsigned long long* base = acc + NC * (lx * SSP + ly * SP + lz);
 #pragma unroll
             for (int a = 0; a < W; ++a)
 #pragma unroll
@@ -496,7 +567,7 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
         __syncthreads();
         for (int i = threadIdx.x; i < S3; i += kSpreadBlock) {
             const int ix = i / (S * S), iy = (i / S) % S, iz = i % S;
-            const int li = (ix * S + iy) * SP + iz;
+            const int li = ix * SSP + iy * SP + iz;
             int gx = ox + ix, gy = oy + iy, gz = oz + iz;
             gx = gx >= n ? gx - n : gx;
             gy = gy >= n ? gy - n : gy;
@@ -579,7 +650,26 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
             }
             const int boff = lx * SSP + ly * SP + lz;
             float sr = 0.f, si = 0.f;
-            if (CPLX) {
+            if (CPLX && CMTIP_INTERP_READS != 1) {
+                const WideRow2<W> wr(boff);
+                int zo[W];
+                wr.load_offsets(zo);
+                float wl[W];
+                weights_at<W>(tz, zo, beta_l2e, wl);
+                const float2* base = reinterpret_cast<const float2*>(sub) + boff;
+#pragma unroll
+                for (int a = 0; a < W; ++a) {
+                    float pr = 0.f, pi = 0.f;
+#pragma unroll
+                    for (int bb = 0; bb < W; ++bb) {
+                        const float2 q = wr.dot(base + a * SSP + bb * SP, wl);
+                        pr = fmaf(wy[bb], q.x, pr);
+                        pi = fmaf(wy[bb], q.y, pi);
+                    }
+                    sr = fmaf(wx[a], pr, sr);
+                    si = fmaf(wx[a], pi, si);
+                }
+            } else if (CPLX) {
                 const T* base = sub + boff;
 #pragma unroll
                 for (int a = 0; a < W; ++a) {

@@ -81,9 +81,9 @@ def test_set_bins_rejects_and_reports():
     p1.set_bins(15, 100, 2)
     assert p1.get_bins() == (15, 100, 2)
     p2 = NufftPlan(2, 21, -1)
-    p2.set_bins(21, 0, 0)  # float subgrids fit a larger bin than 64-bit spreading ones
+    p2.set_bins(20, 0, 0)  # float2 subgrids (26^3 cells, padded pitches) fit a larger bin than spreading ones
     with pytest.raises(CmtipError):
-        p2.set_bins(22, 0, 0)
+        p2.set_bins(21, 0, 0)
 
 
 @pytest.mark.parametrize("nufft_type", [1, 2])
