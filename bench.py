@@ -252,16 +252,29 @@ def main():
             return (time.perf_counter() - t) / reps
 
         data = imgs.reshape(-1)
-        t2 = tm(lambda: nufft.forward_real(ac_d, x, y, z))
-        t2c = tm(lambda: nufft.forward_gpu(ac_d, x, y, z))
-        t1 = tm(lambda: nufft.adjoint_gpu(data, x, y, z, M))
-        t1k = tm(lambda: nufft.adjoint_ones(x, y, z, 2 * M, scale=1.0 / (2 * M) ** 3), reps=1)
-        result["nufft"] = {
-            "points": Np,
-            "type2_real_Mpts_s": Np / t2 / 1e6, "type2_complex_Mpts_s": Np / t2c / 1e6,
-            "type1_real_M81_Mpts_s": Np / t1 / 1e6, "type1_ones_M162_Mpts_s": Np / t1k / 1e6,
-            "type2_plus_type1_hbm_frac": ((20 * Np + 4 * M ** 3) * 2 / (t2c + t1) / 1e9) / HBM_PEAK_GBS,
-        }
+        # plan-level timing: setpts (binning) once per point set, execute per transform
+        from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_ONES, CMTIP_PTS_REAL
+        res_nufft = {"points": Np, "note": "Mpts_s over execute; with_setpts adds one binning of the points"}
+        for typ, modes, flags, label in ((2, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type2_real"),
+                                         (2, M, CMTIP_GRID_REAL, "type2_complex"),
+                                         (1, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type1_real_M81"),
+                                         (1, 2 * M, CMTIP_GRID_REAL | CMTIP_PTS_ONES, "type1_ones_M162")):
+            p = nufft.get_plan(typ, modes, -1 if typ == 2 else 1)
+            ts = tm(lambda: p.setpts(x, y, z))
+            if typ == 2:
+                o = torch.empty(Np, device=dev, dtype=torch.float32 if flags & CMTIP_PTS_REAL else torch.complex64)
+                te = tm(lambda: p.execute(o, ac_d, flags))
+            else:
+                o = torch.empty((modes,) * 3, device=dev)
+                te = tm(lambda: p.execute(None if flags & CMTIP_PTS_ONES else data, o, flags, None,
+                                          1.0 / modes ** 3))
+            res_nufft[label] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": Np / te / 1e6,
+                                "Mpts_s_with_setpts": Np / (te + ts) / 1e6}
+        t2 = res_nufft["type2_real"]["execute_ms"] * 1e-3
+        t1 = res_nufft["type1_real_M81"]["execute_ms"] * 1e-3
+        # 20 B per point + 4 M^3 per transform (real form), execute only
+        res_nufft["type2_plus_type1_real_hbm_frac"] = ((20 * Np + 4 * M ** 3) * 2 / (t2 + t1) / 1e9) / HBM_PEAK_GBS
+        result["nufft"] = res_nufft
         log(f"[bench] nufft {result['nufft']}")
         # CG iteration time of the autocorrelation solve (20 iterations)
         solver = AC.get_solver(M)

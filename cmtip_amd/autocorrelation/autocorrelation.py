@@ -78,7 +78,7 @@ class ACSolver:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _lib._LIB is not None:
+        if h is not None and h.value and _lib is not None and _lib._LIB is not None:
             _lib._LIB.cmtip_acsolver_destroy(h)
             self._h = None
 
@@ -234,12 +234,12 @@ def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, 
         # the reference draws sk.get_random_quat(N_images) (:208); skopi is not available
         orientations = synth.random_quaternions(N_images, seed if seed is not None else np.random.randint(2 ** 31))
     ext = np.float32(reciprocal_extent.item() if isinstance(reciprocal_extent, torch.Tensor) else reciprocal_extent)
-    H_, K_, L_ = gen_nonuniform_positions(
-        orientations if isinstance(orientations, torch.Tensor) else np.asarray(orientations),
-        pixel_position_reciprocal if isinstance(pixel_position_reciprocal, torch.Tensor)
-        else np.asarray(pixel_position_reciprocal), recip_extent=ext)
     dev = _lib.device()
-    H_, K_, L_ = (torch.as_tensor(t, device=dev).reshape(-1) for t in (H_, K_, L_))
+    # device tensors in, so the rotated coordinates (12 B per point) stay in device memory
+    H_, K_, L_ = gen_nonuniform_positions(
+        _arrays.to_dev(orientations, torch.float32),
+        _arrays.to_dev(pixel_position_reciprocal, torch.float64), recip_extent=ext)
+    H_, K_, L_ = (t.reshape(-1) for t in (H_, K_, L_))
     data = _arrays.to_dev(slices_, torch.float32).reshape(-1)
     if ac_estimate is None:
         ac_support = torch.ones((M,) * 3, dtype=torch.float32, device=dev)

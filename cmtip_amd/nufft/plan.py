@@ -103,7 +103,7 @@ class NufftPlan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _lib._LIB is not None:
+        if h is not None and h.value and _lib is not None and _lib._LIB is not None:
             _lib._LIB.cmtip_nufft_destroy(h)
             self._h = None
 
