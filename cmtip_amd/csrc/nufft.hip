@@ -303,6 +303,68 @@ __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __r
     }
 }
 
+// Workgroup-privatised binning (when the bin counters fit LDS): a workgroup takes a
+// contiguous range of points, counts them in an LDS histogram and adds only its
+// non-zero bins to the global counters; the scatter pass counts again, reserves one
+// contiguous range per (workgroup, bin) with a single global atomic and hands out the
+// slots from LDS.  Global atomics drop from one per (wave, bin) to one per
+// (workgroup, bin).
+constexpr int kBinBlock = 1024;
+constexpr int64_t kLocalBinsMax = 36864;  // 144 KB of LDS counters
+
+template <class Src>
+__global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int64_t N, int64_t per_block, int n,
+                                                                   int W, int bs, int nbx, int nbins, float hi,
+                                                                   float lo, int* __restrict__ bin_of,
+                                                                   int* __restrict__ count) {
+    extern __shared__ int hist[];
+    for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
+    __syncthreads();
+    const int64_t j0 = (int64_t)blockIdx.x * per_block, j1 = min(N, j0 + per_block);
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
+        float t, x, y, z;
+        pts.at(j, &x, &y, &z);
+        const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
+        const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
+        const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
+        const int b = (bx * nbx + by) * nbx + bz;
+        bin_of[j] = b;
+        atomicAdd(hist + b, 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += kBinBlock)
+        if (hist[i]) atomicAdd(count + i, hist[i]);
+}
+
+template <bool COPY>
+__global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int64_t N,
+    int64_t per_block, int nbins, const int* __restrict__ bin_of, const int* __restrict__ offset,
+    int* __restrict__ cursor, int* __restrict__ perm, float* __restrict__ xs, float* __restrict__ ys,
+    float* __restrict__ zs) {
+    extern __shared__ int hist[];
+    for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
+    __syncthreads();
+    const int64_t j0 = (int64_t)blockIdx.x * per_block, j1 = min(N, j0 + per_block);
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) atomicAdd(hist + bin_of[j], 1);
+    __syncthreads();
+    // the workgroup's range of each bin: hist[b] becomes its first position
+    for (int i = threadIdx.x; i < nbins; i += kBinBlock) {
+        const int c = hist[i];
+        if (c) hist[i] = offset[i] + atomicAdd(cursor + i, c);
+    }
+    __syncthreads();
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
+        const int pos = atomicAdd(hist + bin_of[j], 1);
+        perm[pos] = (int)j;
+        if (COPY) {
+            xs[pos] = x[j];
+            ys[pos] = y[j];
+            zs[pos] = z[j];
+        }
+    }
+}
+
 __global__ void chunk_fill_kernel(const int* __restrict__ offset, const int* __restrict__ nchunk, int64_t nbins,
                                   int chunk, int4* __restrict__ chunks) {
     const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -1451,6 +1513,44 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
     const unsigned gp = blocks_for(N);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), p->nbins, 0);
+    if (p->nbins <= kLocalBinsMax) {
+        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((N + 8191) / 8192, 4 * cu_count()));
+        const int64_t per_block = (N + blocks - 1) / blocks;
+        const size_t lds = sizeof(int) * (size_t)p->nbins;
+        const int nb = (int)p->nbins;
+        static bool attr_set = false;
+        if (!attr_set) {
+            allow_large_lds((const void*)bin_count_local_kernel<PtsRotated>);
+            allow_large_lds((const void*)bin_count_local_kernel<PtsArrays>);
+            allow_large_lds((const void*)bin_scatter_local_kernel<false>);
+            allow_large_lds((const void*)bin_scatter_local_kernel<true>);
+            attr_set = true;
+        }
+        if (rot)
+            hipLaunchKernelGGL(bin_count_local_kernel<PtsRotated>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
+                               *rot, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi, p->tc_lo,
+                               p->bin_of.as<int>(), p->count.as<int>());
+        else
+            hipLaunchKernelGGL(bin_count_local_kernel<PtsArrays>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
+                               PtsArrays{p->x, p->y, p->z}, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi,
+                               p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+        hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
+                           p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
+        if (rot)
+            hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
+                               (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, N, per_block, nb,
+                               p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
+                               p->perm.as<int>(), (float*)nullptr, (float*)nullptr, (float*)nullptr);
+        else
+            hipLaunchKernelGGL(bin_scatter_local_kernel<true>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st, p->x,
+                               p->y, p->z, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
+                               p->count.as<int>(), p->perm.as<int>(), p->xs.as<float>(), p->ys.as<float>(),
+                               p->zs.as<float>());
+        hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
+                           p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
+        CMTIP_LAUNCH_CHECK();
+        return CMTIP_OK;
+    }
     if (rot)
         hipLaunchKernelGGL(bin_count_kernel<PtsRotated>, dim3(gp), dim3(kBlock), 0, st, *rot, N, p->n, p->w, p->bs,
                            p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
@@ -2003,9 +2103,11 @@ extern "C" int cmtip_slice_prepare(cmtip_nufft_plan p, const float* ac, const fl
 namespace {
 
 constexpr int kProbeOrients = 16;           // orientations sliced by the route probe
-constexpr double kTileMinFit = 0.95;        // tile route when this share of tiles fits its margin
-constexpr int64_t kTileMinPerBin = 64;      // and the workload has this many tiles per bin
-constexpr double kFusedMinFit = 0.8;        // else fused tiles when this share of boxes fits LDS
+constexpr double kTileMinFit = 0.95;        // tile route when this share of tiles fits its margin,
+constexpr int64_t kTileMinPerBin = 64;      // the call has this many tiles per bin
+constexpr int kTileMinGrid = 128;           // and the fine grid is at least this wide
+constexpr double kFusedMinFit = 0.8;        // else fused tiles for small calls when this share of boxes fits LDS
+constexpr int64_t kFusedMaxPts = 1 << 20;   // (small: no sort to amortise); else binned points
 constexpr int64_t kBinnedBatchPts = 1 << 27;  // points per binned-route batch (8 B of sort state each)
 
 // Probe for the automatic route on the first orientations (their slices are written
@@ -2031,12 +2133,14 @@ int probe_route(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_
     return CMTIP_OK;
 }
 
-// Automatic route for a call of n_orient orientations from the probed shares.
+// Automatic route for a call of n_orient orientations from the probed shares
+// (measured at C1 / C2 / M = 151 scale: binned tiles win at M = 81, 128^2 pixels;
+// binned points at M = 41 and 151; fused tiles for calls of a few orientations).
 int auto_route(const cmtip_nufft_plan_s* p, int64_t P, int64_t n_orient) {
     const int64_t nbx = (p->n + kTileBs - 1) / kTileBs;
     const bool tile_reuse = n_orient * ((P + 63) / 64) >= kTileMinPerBin * nbx * nbx * nbx;
-    if (p->fit_tile >= kTileMinFit && tile_reuse) return 3;
-    if (p->fit_box >= kFusedMinFit) return 1;
+    if (p->fit_tile >= kTileMinFit && tile_reuse && p->n >= kTileMinGrid) return 3;
+    if (p->fit_box >= kFusedMinFit && n_orient * P <= kFusedMaxPts) return 1;
     return 2;
 }
 

@@ -151,9 +151,10 @@ int cmtip_slice_real(cmtip_nufft_plan plan, const float* quats, int64_t n_orient
                      double* moments, void* stream);
 
 /* Slicing route of a plan: 0 automatic (default: binned tiles when at least 95% of the
- * probed tiles fit the tile route's margin and the call has at least 64 tiles per bin,
- * else fused tiles when at least 80% of their boxes fit LDS, else binned points),
- * 1 fused tiles, 2 binned points, 3 binned tiles.
+ * probed tiles fit the tile route's margin, the call has at least 64 tiles per bin and
+ * the fine grid is at least 128 cells wide; else fused tiles for calls of at most 2^20
+ * points when at least 80% of their boxes fit LDS; else binned points), 1 fused tiles,
+ * 2 binned points, 3 binned tiles.
  * A fixed route never
  * synchronises (use one under stream capture).  get_route reports the request and the
  * route the last cmtip_slice_real used (0 before the first).  No counterpart in the

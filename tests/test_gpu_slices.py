@@ -113,15 +113,16 @@ def _slice_route(sl, q, route):
     return out, b16, mom, sl.plan.slice_route()[1]
 
 
-@pytest.mark.parametrize("M,n_side,n_q,auto", [(41, 64, 40, 1), (41, 64, 3500, 3), (81, 128, 40, 1),
-                                                (151, 128, 40, 2), (151, 96, 40, 2)])
+@pytest.mark.parametrize("M,n_side,n_q,auto", [(41, 64, 40, 1), (41, 64, 3500, 2), (81, 128, 40, 1),
+                                                (81, 128, 5000, 3), (151, 128, 40, 2), (151, 96, 40, 2)])
 def test_slice_routes_agree(M, n_side, n_q, auto):
     # fused tiles, binned points (rotated coordinates, bin sort, LDS subgrids) and binned
     # tiles evaluate the same sums (their FP32 additions differ only in order, set by each
-    # layout's row alignment).  The automatic choice follows the probed share of tiles
-    # within the tile route's margin (all at M = 41 and 81), the tiles per bin of the call
-    # (enough at M = 41 with 3500 orientations) and the share of fused boxes that fit
-    # LDS (none at M = 151 with 128 or 96 pixels)
+    # layout's row alignment).  The automatic choice: binned tiles when the probed tiles fit
+    # the tile route's margin, the call has enough tiles per bin and the fine grid is wide
+    # (M = 81 with 5000 orientations); fused tiles for small calls whose boxes fit LDS
+    # (40 orientations at M = 41 and 81); binned points otherwise (M = 41 with 3500
+    # orientations on a 90-cell grid; M = 151, where no fused box fits)
     ac = synth.gaussian_ac(M, 50, seed=5)
     s, ext = synth.detector(n_side)
     q = torch.as_tensor(synth.random_quaternions(n_q, seed=6), device=dev())
@@ -136,10 +137,17 @@ def test_slice_routes_agree(M, n_side, n_q, auto):
     assert torch.equal(res[0][0], res[auto][0])
     for o, b, _, _ in res.values():
         assert torch.equal(b, o.to(torch.bfloat16).view(torch.int16))
-    # spot check of the binned routes against exact FP64 sums
+    # spot check of the binned routes against exact FP64 sums (coordinates of the sampled
+    # orientations only)
     rng = np.random.default_rng(M)
-    idx = rng.choice(n_q * sl.P, 100, replace=False)
-    x, y, z = (np.asarray(v).ravel()[idx] for v in geom.rotated_coords(q.cpu().numpy(), s, ext))
+    idx = np.sort(rng.choice(n_q * sl.P, 100, replace=False))
+    qn = q.cpu().numpy()
+    xs, ys, zs = [], [], []
+    for o in np.unique(idx // sl.P):
+        pix = idx[idx // sl.P == o] % sl.P
+        xo, yo, zo = geom.rotated_coords(qn[o:o + 1], s, ext)
+        xs.append(xo[pix]), ys.append(yo[pix]), zs.append(zo[pix])
+    x, y, z = np.concatenate(xs), np.concatenate(ys), np.concatenate(zs)
     from oracle import nufft as ON
     want = ON.nudft2(ac.astype(np.float64), x, y, z).real
     for r in (2, 3):
