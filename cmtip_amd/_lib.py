@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libcmtip_amd.so")
 CMTIP_GRID_REAL = 1
 CMTIP_PTS_REAL = 2
 CMTIP_PTS_ONES = 4
+CMTIP_PTS_IMAG = 8
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

@@ -6,14 +6,15 @@ Forward (compute_slices, cmtip/alignment/align.py:39-65):
 Backward, for an upstream gradient g (real, one value per point):
     dL/df_k     = sum_j g_j cos(k.x_j) = Re(type 1 of g)_k          (the adjoint spread)
     dL/dx_{j,d} = g_j Im(sum_k k_d f_k exp(-i k.x_j))               (an ik-weighted type 2, d = 0, 1, 2)
-Both run on the same HIP NUFFT plans as the forward.  ``rotated_coords`` builds the
+Both run on the same HIP NUFFT plans as the forward; the coordinate gradient uses the
+type-2 imaginary-part output (real interpolation of the fine grid's imaginary part).  ``rotated_coords`` builds the
 rotated pixel coordinates from quaternions with torch operations, so gradients reach
 the quaternions for orientation refinement.
 """
 import numpy as np
 import torch
 
-from ._lib import CMTIP_GRID_REAL, CMTIP_PTS_REAL
+from ._lib import CMTIP_GRID_REAL, CMTIP_PTS_IMAG, CMTIP_PTS_REAL
 from .nufft.plan import DEFAULT_EPS, get_plan
 
 
@@ -62,9 +63,9 @@ class NufftSlice(torch.autograd.Function):
                 shape = [1, 1, 1]
                 shape[d] = M
                 kf = (ac * k.reshape(shape)).contiguous()
-                c = torch.empty(x.numel(), dtype=torch.complex64, device=ac.device)
-                p2.execute(c, kf, CMTIP_GRID_REAL)
-                outs.append((g * c.imag).reshape(ctx.shape))
+                c = torch.empty(x.numel(), dtype=torch.float32, device=ac.device)
+                p2.execute(c, kf, CMTIP_GRID_REAL | CMTIP_PTS_IMAG)  # Im(T2(k_d f))
+                outs.append((g * c).reshape(ctx.shape))
             grad_H, grad_K, grad_L = outs
         return grad_ac, grad_H, grad_K, grad_L, None
 

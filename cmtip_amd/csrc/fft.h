@@ -48,7 +48,7 @@ struct AxisDesc {
     const float* in_mask = nullptr;  // per-element input weight (float, input offsets)
 
     void* out = nullptr;
-    int out_mode = 0;  // 0: float2, 1: real part as float
+    int out_mode = 0;  // 0: float2, 1: real part as float, 2: imaginary part as float
     int64_t out_sa = 0, out_su = 0, out_sv = 0;
     int oa = 0, ob = 0, og1 = 0, og2 = 0;
     const float* out_w = nullptr;     // per-LDS-position output weight (length n)

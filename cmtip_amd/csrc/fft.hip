@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
         if (OUT_MODE == 0) {
             reinterpret_cast<float2*>(d.out)[off] = x;
         } else {
-            float y = x.x;
+            float y = OUT_MODE == 1 ? x.x : x.y;
             if (d.add_src) y += d.add_scale * d.add_src[off];
             reinterpret_cast<float*>(d.out)[off] = y;
         }
@@ -388,14 +388,18 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     if (d.in_real) {
         if (d.out_mode == 0) {
             if (mid) CMTIP_AXIS_LAUNCH(true, 0, true); else CMTIP_AXIS_LAUNCH(true, 0, false);
-        } else {
+        } else if (d.out_mode == 1) {
             if (mid) CMTIP_AXIS_LAUNCH(true, 1, true); else CMTIP_AXIS_LAUNCH(true, 1, false);
+        } else {
+            if (mid) CMTIP_AXIS_LAUNCH(true, 2, true); else CMTIP_AXIS_LAUNCH(true, 2, false);
         }
     } else {
         if (d.out_mode == 0) {
             if (mid) CMTIP_AXIS_LAUNCH(false, 0, true); else CMTIP_AXIS_LAUNCH(false, 0, false);
-        } else {
+        } else if (d.out_mode == 1) {
             if (mid) CMTIP_AXIS_LAUNCH(false, 1, true); else CMTIP_AXIS_LAUNCH(false, 1, false);
+        } else {
+            if (mid) CMTIP_AXIS_LAUNCH(false, 2, true); else CMTIP_AXIS_LAUNCH(false, 2, false);
         }
     }
 #undef CMTIP_AXIS_LAUNCH

@@ -1669,7 +1669,7 @@ void mode_segments_out(AxisDesc& d, int M) {
 // Uniform grid f (M^3, float or float2) -> fine grid (n^3), real part only when
 // real_out.  Deconvolution weights on every axis; support multiplies f on load.
 int grid_to_fine(cmtip_nufft_plan_s* p, const void* f, bool f_real, const float* support, bool real_out,
-                 hipStream_t st) {
+                 hipStream_t st, bool imag_out = false) {
     const int M = p->M, n = p->n;
     const float* dw = p->deconv.as<const float>();
     // pass 1: axis 2 (contiguous), lines (i0, i1)
@@ -1719,7 +1719,7 @@ int grid_to_fine(cmtip_nufft_plan_s* p, const void* f, bool f_real, const float*
     mode_segments_in(d3, M);
     d3.in_w = dw;
     d3.out = p->fine.p;
-    d3.out_mode = real_out ? 1 : 0;
+    d3.out_mode = imag_out ? 2 : (real_out ? 1 : 0);
     d3.out_su = n;
     d3.out_sv = 1;
     d3.out_sa = (int64_t)n * n;
@@ -1934,7 +1934,8 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     hipStream_t st = as_stream(stream);
     const bool grid_real = flags & CMTIP_GRID_REAL;
     const bool pts_ones = (flags & CMTIP_PTS_ONES) && p->type == 1;
-    const bool pts_real = (flags & CMTIP_PTS_REAL) || pts_ones;
+    const bool pts_imag = (flags & CMTIP_PTS_IMAG) && p->type == 2;
+    const bool pts_real = (flags & CMTIP_PTS_REAL) || pts_ones || pts_imag;
     if (pts_ones) c = nullptr;
     if (!f || (p->npts > 0 && !c && !pts_ones)) {
         set_error("execute: null data pointer");
@@ -1946,7 +1947,8 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     }
     if (p->type == 2) {
         p->slice_ready = 0;
-        CMTIP_TRY(grid_to_fine(p, f, grid_real, support, pts_real, st));
+        // the interpolation weights are real: Re / Im of c interpolate Re / Im of the fine grid
+        CMTIP_TRY(grid_to_fine(p, f, grid_real, support, pts_real, st, pts_imag));
         return dispatch_w<InterpLaunch>(p->w, p, !pts_real,
                                         pts_real ? (float2*)nullptr : (float2*)c,
                                         pts_real ? (float*)c : (float*)nullptr, (const PtsRotated*)nullptr, st);

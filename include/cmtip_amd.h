@@ -73,6 +73,8 @@ int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, cons
 #define CMTIP_PTS_REAL 2    /* the point values c are float32 (real); else complex64 */
 #define CMTIP_PTS_ONES 4    /* type 1 only: every strength is 1 (c is ignored), as the
                                all-ones kernel input of autocorrelation.py:143,146 */
+#define CMTIP_PTS_IMAG 8    /* type 2 only: c (float) receives Im(T2(f)), e.g. the
+                               coordinate gradient of a real slice */
 
 /* Type 2: c = T2(support (.) f); the support (float32 M^3, may be NULL) multiplies f
  * on load (forward.py:34-35); with CMTIP_PTS_REAL only Re(c) is written.

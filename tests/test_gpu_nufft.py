@@ -135,3 +135,23 @@ def test_out_of_range_asserts():
     x = np.array([3 * np.pi + 0.1], np.float32)
     with pytest.raises(AssertionError):
         nufft.forward_gpu(f, x, x, x)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_type2_imag_output(cplx):
+    # Im(T2(f)) from the real interpolation of the fine grid's imaginary part
+    from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_IMAG
+    M, N = 21, 4000
+    rng = np.random.default_rng(40 + cplx)
+    f = rng.standard_normal((M, M, M)).astype(np.float32)
+    if cplx:
+        f = (f + 1j * rng.standard_normal((M, M, M))).astype(np.complex64)
+    x, y, z = points(rng, N)
+    want = O.nudft2(f.astype(np.complex128), x, y, z).imag
+    say(f"type2 imaginary-part output cplx={cplx}")
+    p = nufft.NufftPlan(2, M, -1)
+    xd, yd, zd = (torch.as_tensor(v, device=dev()) for v in (x, y, z))
+    p.setpts(xd, yd, zd)
+    out = torch.empty(N, dtype=torch.float32, device=dev())
+    p.execute(out, torch.as_tensor(f, device=dev()), (0 if cplx else CMTIP_GRID_REAL) | CMTIP_PTS_IMAG)
+    assert rel_l2(out, want) < TOL
