@@ -232,6 +232,9 @@ def main():
         log(f"[bench] {other} form, 2 steps")
         el2, q2 = timed(other, 2, 1)
         result[f"{other}_images_per_s"] = n_img * 2 / el2
+        # the BF16 form against the plain FP32 form of the same step (same slicing)
+        result["bf16_over_fp32_form"] = (value / result["fp32_images_per_s"] if other == "fp32"
+                                         else result["bf16_images_per_s"] / value)
         agree = float((q2 == q).all(dim=1).float().mean())
         result["assignment_agreement_fp32_vs_bf16"] = agree
         sl.plan.slice_real = plan_slice
