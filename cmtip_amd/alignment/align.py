@@ -181,8 +181,9 @@ class _Matcher:
 
     Model slices and images are centred on the mean image c before the cross term
     (d2 is unchanged; the FP32 / BF16 expanded form then does not cancel).  With
-    per-image masks (1 = valid pixel) the cross term runs over 2P columns:
-    [y^2, y] . [-m w^2 / 2, m w^2 (b - c)] with y = gamma a - c."""
+    per-image masks (1 = valid pixel) the cross term runs over 2P columns,
+    [y^2, y] . [-m w^2 / 2, m w^2 (b - c)] with y = gamma a - c, or 3P in BF16 with y^2
+    split into two bfloat16 parts."""
 
     def __init__(self, pixel_position_reciprocal, slices_, order, precision, topk, masks=None, cross="lib"):
         assert cross in ("lib", "fused")
@@ -209,7 +210,8 @@ class _Matcher:
             count = mom[2].clamp(min=1.0)
             mean = mom[0] / count
             self.std_data = torch.sqrt(torch.clamp(mom[1] / count - mean * mean, min=0.0))
-            self.K = 2 * self.P
+            # float32 cross term over [y^2, y]; bfloat16 over [hi(y^2), lo(y^2), y]
+            self.K = (2 if precision == "fp32" else 3) * self.P
             self.Bw = torch.empty((self.n_img, self.K), dtype=torch.float32, device=dev) \
                 if precision == "fp32" else None
             self.Bw16 = torch.empty((self.n_img, self.K), dtype=torch.int16, device=dev) \

@@ -288,14 +288,17 @@ __global__ void prepare_masked_kernel(const float* __restrict__ B, const uint8_t
         const float d = B[o] - c[p];
         const float lo = -0.5f * m, hi = m * d;
         s += (double)m * (double)d * (double)d;
-        const int64_t q = row * 2 * P;
-        if (Bw) {
+        if (Bw) {  // [-m w^2 / 2, m w^2 (b - c)]
+            const int64_t q = row * 2 * P;
             Bw[q + p] = lo;
             Bw[q + P + p] = hi;
         }
-        if (Bw16) {
-            Bw16[q + p] = f32_to_bf16(lo);
-            Bw16[q + P + p] = f32_to_bf16(hi);
+        if (Bw16) {  // [-m w^2 / 2, -m w^2 / 2, m w^2 (b - c)]: pairs with the split y^2
+            const int64_t q = row * 3 * P;
+            const uint16_t l16 = f32_to_bf16(lo);
+            Bw16[q + p] = l16;
+            Bw16[q + P + p] = l16;
+            Bw16[q + 2 * P + p] = f32_to_bf16(hi);
         }
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -308,21 +311,29 @@ __global__ void prepare_masked_kernel(const float* __restrict__ B, const uint8_t
     }
 }
 
-// model rows -> [y^2, y] with y = gamma a - c (2P columns, float32 and/or bf16)
+// model rows -> [y^2, y] with y = gamma a - c (2P float32 columns) and/or
+// [hi(y^2), lo(y^2), y] (3P bfloat16 columns): y^2 carried as the sum of two bfloat16
+// values (hi = bf16(y^2), lo = bf16(y^2 - hi)), which keeps the masked squared term to
+// ~16 significant bits; with beamstop masks the best-to-second distance gaps are small
+// enough that a single bfloat16 y^2 misplaces the best reference outside the top 8
 __global__ void center_rows_sq_kernel(const float* __restrict__ X, int64_t P, const double* __restrict__ gamma,
                                       const float* __restrict__ c, float* __restrict__ Y, uint16_t* __restrict__ Y16) {
     const float g = gamma ? (float)gamma[0] : 1.0f;
     const int64_t row = blockIdx.x;
     for (int64_t p = threadIdx.x; p < P; p += kThreads) {
         const float y = fmaf(g, X[row * P + p], -c[p]);
-        const int64_t q = row * 2 * P;
+        const float y2 = y * y;
         if (Y) {
-            Y[q + p] = y * y;
+            const int64_t q = row * 2 * P;
+            Y[q + p] = y2;
             Y[q + P + p] = y;
         }
         if (Y16) {
-            Y16[q + p] = f32_to_bf16(y * y);
-            Y16[q + P + p] = f32_to_bf16(y);
+            const int64_t q = row * 3 * P;
+            const uint16_t h = f32_to_bf16(y2);
+            Y16[q + p] = h;
+            Y16[q + P + p] = f32_to_bf16(y2 - __uint_as_float((uint32_t)h << 16));
+            Y16[q + 2 * P + p] = f32_to_bf16(y);
         }
     }
 }

@@ -247,9 +247,11 @@ int cmtip_match_rescore(const float* A_all, int64_t n_ref_total, const float* B,
 
 /* Per-image pixel masks (uint8, n_img x P, 1 = valid; an extension for detectors with
  * beamstops, absent from the reference).  d2 = sum_p m_ip w_p^2 (gamma a - b)^2 is one
- * cross term over 2P columns: model rows [y^2, y] with y = gamma a - c
- * (cmtip_center_rows_sq), image rows [-m w^2 / 2, m w^2 (b - c)] and bn = sum m w^2 (b - c)^2
- * (cmtip_match_prepare_masked); the matching kernels then run with K = 2P and an = 0.
+ * cross term: float32 rows over 2P columns, model [y^2, y] with y = gamma a - c
+ * (cmtip_center_rows_sq) and image [-m w^2 / 2, m w^2 (b - c)] (cmtip_match_prepare_masked),
+ * K = 2P; bfloat16 rows over 3P columns, model [hi(y^2), lo(y^2), y] (y^2 split into two
+ * bfloat16 parts) and image [-m w^2 / 2, -m w^2 / 2, m w^2 (b - c)], K = 3P.  bn =
+ * sum m w^2 (b - c)^2; the matching kernels run with an = 0.
  * cmtip_image_stats_masked: colsum (3P doubles workspace), c = masked column mean,
  * mom = (sum m x, sum m x^2, sum m). */
 int cmtip_image_stats_masked(const float* X, const uint8_t* mask, int64_t rows, int64_t P, double* colsum,

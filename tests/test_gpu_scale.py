@@ -73,11 +73,11 @@ def test_bf16_top8_at_full_size():
 
 @pytest.mark.parametrize("side", [128, 96])
 def test_masked_matching_m151(side):
-    # M = 151 grid (5000-atom density), per-image beamstop masks of uneven size.  The FP32
-    # form must give the FP64 assignment outside the tie band.  Masking the bright centre
-    # shrinks the best-to-second gaps ~100 x (median 5.6e-4 relative at 128 x 128), into
-    # the range of the BF16 cross term's error: its agreement is reported (0.93 at
-    # 128 x 128) and guarded against regressions, not required to be exact
+    # M = 151 grid (5000-atom density), per-image beamstop masks of uneven size.  Masking
+    # the bright centre shrinks the best-to-second gaps ~100 x (median 5.6e-4 relative at
+    # 128 x 128); with y^2 split into two bfloat16 parts the BF16 top-8 still holds the
+    # best reference (a single bfloat16 y^2 missed it for 7 % of the images), so both
+    # forms must give the FP64 assignment outside the tie band
     d = dev()
     M, n_ref, n_img = 151, 3000, 800
     s, ext = synth.detector(side)
@@ -111,9 +111,8 @@ def test_masked_matching_m151(side):
     got, w = idx.cpu().numpy(), want.cpu().numpy()
     assert ok.mean() > 0.9
     assert np.array_equal(got[ok], w[ok]), f"{(got[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
-    agree16 = float((idx16.cpu().numpy()[ok] == w[ok]).mean())
-    say(f"masked BF16 agreement with FP64 outside the tie band: {agree16:.4f}")
-    assert agree16 > 0.85
+    got16 = idx16.cpu().numpy()
+    assert np.array_equal(got16[ok], w[ok]), f"bf16: {(got16[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
 
 
 def test_solve_m151_autotuned():
