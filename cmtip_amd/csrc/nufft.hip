@@ -106,7 +106,22 @@ __device__ __forceinline__ float fine_coord(float x, float hi, float lo, int n) 
 // interpolation (type 2) kernels, which stage the chunk's subgrid in LDS.
 
 constexpr int kChunk = 4096;  // largest chunk (the fixed-point spreading margin assumes it)
-constexpr int kSpreadBlock = 256;
+constexpr int kSpreadBlock = 256;  // interpolation workgroups
+// Spreading workgroups of 512 threads (three per CU with their LDS) and points dealt
+// to lanes by bank class: measured over 81.92 M points, type 1 at M = 81 13.7 ms
+// (256 threads, no dealing) -> 11.2 (256, dealt) -> 9.25 ms (512, dealt); unit
+// strengths at 162 modes 9.8 -> 9.1 ms.
+#ifndef CMTIP_SPREAD_THREADS
+#define CMTIP_SPREAD_THREADS 512
+#endif
+constexpr int kSpreadThreads = CMTIP_SPREAD_THREADS;  // spreading workgroups
+#ifndef CMTIP_SPREAD_DEAL
+#define CMTIP_SPREAD_DEAL 1
+#endif
+constexpr bool kSpreadDeal = CMTIP_SPREAD_DEAL;  // deal points to lanes by LDS bank class
+constexpr int kSpreadClasses = 32;  // LDS bank pairs of 64-bit accumulators
+// static LDS of the spreading kernel (point dealing), beside its dynamic subgrid
+constexpr size_t kSpreadStaticLds = 3 * kChunk + 4 * 3 * kSpreadClasses + 64;
 constexpr size_t kMaxLds = 160 * 1024;     // LDS per workgroup on gfx950
 constexpr size_t kDefaultLds = 64 * 1024;  // dynamic LDS allowed without raising the limit
 constexpr size_t kStaticLds = 256;         // reserve for the binned kernels' static LDS
@@ -563,7 +578,7 @@ __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsign
 }
 
 template <int W, bool CPLX>
-__global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
+__global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     typename Cell<CPLX>::T* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx,
     const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
     const int* __restrict__ perm, const typename Cell<CPLX>::T* __restrict__ c, const unsigned* __restrict__ cmax_bits,
@@ -576,7 +591,10 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
     int SP, SSP;
     spread_pitch(S, &SP, &SSP);
     const int L3 = S * SSP;
-    __shared__ int chunk_id;
+    __shared__ int chunk_id, rounds;
+    __shared__ int cls_count[kSpreadClasses], cls_start[kSpreadClasses], cls_fill[kSpreadClasses];
+    __shared__ uint16_t dealt[kChunk];  // chunk-local point indices grouped by class
+    __shared__ uint8_t cls_of[kChunk];
     const int total = ctrl[0];
     const int sc = spread_scale_exp(c ? __uint_as_float(*cmax_bits) : 1.0f);
     const double down = ldexp(1.0, -sc);
@@ -588,9 +606,55 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
         const int4 ch = chunks[cid];
         const int b = ch.x;
         const int ox = (b / (nbx * nbx)) * bs, oy = ((b / nbx) % nbx) * bs, oz = (b % nbx) * bs;
-        for (int i = threadIdx.x; i < NC * L3; i += kSpreadBlock) acc[i] = 0ull;
+        for (int i = threadIdx.x; i < NC * L3; i += kSpreadThreads) acc[i] = 0ull;
+        if (threadIdx.x < kSpreadClasses) {
+            cls_count[threadIdx.x] = 0;
+            cls_fill[threadIdx.x] = 0;
+        }
         __syncthreads();
-        for (int k = ch.y + threadIdx.x; k < ch.z; k += kSpreadBlock) {
+        // deal the chunk's points to lanes by the LDS bank pair of their first accumulator
+        // (class = its 64-bit cell index mod 32): the two lanes of a wave that share a
+        // class sit 32 lanes apart, so every atomic of the 343-add sequence, which adds
+        // the same offset for all lanes, hits distinct bank pairs within a 32-lane pass
+        const int npts = kSpreadDeal ? ch.z - ch.y : 0;  // <= kChunk
+        for (int i = threadIdx.x; i < npts; i += kSpreadThreads) {
+            float t;
+            const int k = ch.y + i;
+            const int ux = first_node_wrapped(xs[k], hi, lo, n, W, &t) - ox;
+            const int uy = first_node_wrapped(ys[k], hi, lo, n, W, &t) - oy;
+            const int uz = first_node_wrapped(zs[k], hi, lo, n, W, &t) - oz;
+            const int cl = (NC * (ux * SSP + uy * SP + uz)) & (kSpreadClasses - 1);
+            cls_of[i] = (uint8_t)cl;
+            atomicAdd(cls_count + cl, 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s0 = 0, mx = 0;
+            for (int q = 0; q < kSpreadClasses; ++q) {
+                cls_start[q] = s0;
+                s0 += cls_count[q];
+                mx = max(mx, cls_count[q]);
+            }
+            rounds = (mx + kSpreadThreads / kSpreadClasses - 1) / (kSpreadThreads / kSpreadClasses);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < npts; i += kSpreadThreads) {
+            const int cl = cls_of[i];
+            dealt[cls_start[cl] + atomicAdd(cls_fill + cl, 1)] = (uint16_t)i;
+        }
+        __syncthreads();
+        const int my_cls = threadIdx.x & (kSpreadClasses - 1), my_sub = threadIdx.x / kSpreadClasses;
+        const int nrounds = kSpreadDeal ? rounds : (ch.z - ch.y + kSpreadThreads - 1) / kSpreadThreads;
+        for (int r = 0; r < nrounds; ++r) {
+            int k;
+            if (kSpreadDeal) {
+                const int e = r * (kSpreadThreads / kSpreadClasses) + my_sub;
+                if (e >= cls_count[my_cls]) continue;
+                k = ch.y + dealt[cls_start[my_cls] + e];
+            } else {
+                k = ch.y + r * kSpreadThreads + threadIdx.x;
+                if (k >= ch.z) continue;
+            }
             float tx, ty, tz;
             const int lxw = first_node_wrapped(xs[k], hi, lo, n, W, &tx);
             const int lyw = first_node_wrapped(ys[k], hi, lo, n, W, &ty);
@@ -627,7 +691,7 @@ __global__ void __launch_bounds__(kSpreadBlock) spread_binned_kernel(
                 }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < S3; i += kSpreadBlock) {
+        for (int i = threadIdx.x; i < S3; i += kSpreadThreads) {
             const int ix = i / (S * S), iy = (i / S) % S, iz = i % S;
             const int li = ix * SSP + iy * SP + iz;
             int gx = ox + ix, gy = oy + iy, gz = oz + iz;
@@ -1465,7 +1529,8 @@ struct SpreadLaunch {
                                dim3(kBlock), 0, st, reinterpret_cast<const float*>(c), cnt, cmax);
         }
         const size_t lds = subgrid_lds(p->bs, W, true, cplx);
-        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : (lds > 40000 ? 2 : 4)));
+        const int fit = (int)std::max<size_t>(1, std::min<size_t>(4, kMaxLds / (lds + kSpreadStaticLds)));
+        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : fit));
         const int4* ch = p->chunks.as<const int4>();
         static bool attr_set[2] = {false, false};
         if (!attr_set[cplx] && lds > kDefaultLds) {
@@ -1473,13 +1538,13 @@ struct SpreadLaunch {
             attr_set[cplx] = true;
         }
         if (cplx)
-            hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadBlock), lds, st,
+            hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadThreads), lds, st,
                                p->fine.as<float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
                                p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
                                p->perm.as<const int>(), reinterpret_cast<const float2*>(c), cmax, ch,
                                p->ctrl.as<int>());
         else
-            hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadBlock), lds, st,
+            hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
                                p->fine.as<float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
                                p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
                                p->perm.as<const int>(), reinterpret_cast<const float*>(c), cmax, ch,
@@ -1883,7 +1948,8 @@ extern "C" int cmtip_nufft_set_bins(cmtip_nufft_plan p, int bin_size, int chunk,
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
     const int bs = bin_size > 0 ? bin_size : bin_size_for(p->w);
     // the subgrid must fit LDS for every execute form of this plan type (complex worst case)
-    if (bin_size < 0 || bs > p->n || subgrid_lds(bs, p->w, p->type == 1, true) + kStaticLds > kMaxLds) {
+    const size_t static_lds = p->type == 1 ? kSpreadStaticLds : kStaticLds;
+    if (bin_size < 0 || bs > p->n || subgrid_lds(bs, p->w, p->type == 1, true) + static_lds > kMaxLds) {
         set_error("set_bins: bin size %d does not fit (w=%d, n=%d)", bin_size, p->w, p->n);
         return CMTIP_ERR_ARG;
     }

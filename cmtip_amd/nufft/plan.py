@@ -114,9 +114,10 @@ def spread_candidates(w, nufft_type):
     the plan type."""
     out = []
     per_cell = 16 if nufft_type == 1 else 8  # complex: 64-bit pairs (spread) / float2 (interp)
+    reserve = 16384 if nufft_type == 1 else 256  # the kernels' static LDS
     for S in range(10, 25, 2):
         bs = S - w + 1
-        if bs >= 2 and S * S * (S + 1) * per_cell + 256 <= 160 * 1024:
+        if bs >= 2 and S * S * (S + 1) * per_cell + reserve <= 160 * 1024:
             out.append((bs, 0, 0))
     return out
 

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 # (bin_size, chunk, wg_per_cu); 0 = default.  M = 21 has a 48^3 fine grid and w = 7.
-SETTINGS = [(2, 64, 1), (4, 1024, 2), (9, 0, 0), (15, 4096, 4), (0, 128, 8), (12, 333, 3)]
+SETTINGS = [(2, 64, 1), (4, 1024, 2), (9, 0, 0), (14, 4096, 4), (0, 128, 8), (12, 333, 3)]
 
 
 def _tensors(*arrs):
@@ -78,8 +78,10 @@ def test_set_bins_rejects_and_reports():
         with pytest.raises(CmtipError):
             p1.set_bins(*bad)
     assert p1.get_bins() == (10, 4096, 0)  # rejected settings leave the plan unchanged
-    p1.set_bins(15, 100, 2)
-    assert p1.get_bins() == (15, 100, 2)
+    p1.set_bins(14, 100, 2)  # 20^3 complex subgrid: the largest that fits beside the dealing arrays
+    assert p1.get_bins() == (14, 100, 2)
+    with pytest.raises(CmtipError):
+        p1.set_bins(15, 0, 0)
     p2 = NufftPlan(2, 21, -1)
     p2.set_bins(20, 0, 0)  # float2 subgrids (26^3 cells, padded pitches) fit a larger bin than spreading ones
     with pytest.raises(CmtipError):

@@ -70,7 +70,7 @@ def test_spread_candidates_fit_lds():
             for bs, chunk, wpc in cands:
                 S = bs + w - 1
                 assert bs >= 2 and chunk == 0 and wpc == 0
-                assert S * S * (S + 1) * per_cell + 256 <= 160 * 1024
+                assert S * S * (S + 1) * per_cell + (16384 if typ == 1 else 256) <= 160 * 1024
     # the default subgrid (16^3 cells at w = 7) is among the candidates
     assert (10, 0, 0) in spread_candidates(7, 1)
 
