@@ -143,19 +143,11 @@ __host__ __device__ inline int bin_size_for(int w) { return w <= 12 ? 17 - w : 2
 #define CMTIP_SLICE_READS 2
 #endif
 
-// Row pitch SP and plane pitch SSP of the interpolation subgrid (S^3 cells, layout
-// x * SSP + y * SP + z).  Real subgrids are read in wide rows: both pitches are
+// Row pitch SP and plane pitch SSP of the interpolation subgrid (S^3 real cells,
+// layout x * SSP + y * SP + z).  Subgrids are read in wide rows: both pitches are
 // multiples of the row alignment (4 for w = 7, else 2), padded by one alignment unit so
-// that consecutive rows and planes start in different LDS banks.  Complex subgrids keep
-// an odd row pitch.
-__host__ __device__ inline void interp_pitch(int S, int W, bool cplx, int* SP, int* SSP) {
-    if (cplx) {
-        // in complex cells; wide rows need even pitches (16-byte aligned pairs)
-        const int A = CMTIP_INTERP_READS == 1 ? 1 : 2;
-        *SP = (S + A - 1) / A * A + A;
-        *SSP = S * *SP + A;
-        return;
-    }
+// that consecutive rows and planes start in different LDS banks.
+__host__ __device__ inline void interp_pitch(int S, int W, int* SP, int* SSP) {
     const int A = CMTIP_INTERP_READS == 1 ? 1 : ((W == 7 && CMTIP_INTERP_READS == 4) ? 4 : 2);
     *SP = (S + A - 1) / A * A + A;
     *SSP = S * *SP + A;
@@ -176,17 +168,17 @@ __host__ __device__ inline void spread_pitch(int S, int* SP, int* SSP) {
     *SSP = S * *SP + CMTIP_SPREAD_SSP_PAD;
 }
 
-// LDS bytes of the spread (64-bit accumulators) or interpolation (float / float2) subgrid.
+// LDS bytes of the spread (64-bit accumulators, two per cell when complex) or the
+// interpolation (float) subgrid.
 inline size_t subgrid_lds(int bs, int W, bool spread, bool cplx) {
     const int S = bs + W - 1;
+    int SP, SSP;
     if (spread) {
-        int SP, SSP;
         spread_pitch(S, &SP, &SSP);
         return (size_t)S * SSP * 8 * (cplx ? 2 : 1);
     }
-    int SP, SSP;
-    interp_pitch(S, W, cplx, &SP, &SSP);
-    return (size_t)S * SSP * (cplx ? 8 : 4);
+    interp_pitch(S, W, &SP, &SSP);
+    return (size_t)S * SSP * 4;
 }
 
 __device__ __forceinline__ int first_node_wrapped(float x, float hi, float lo, int n, int W, float* t_out) {
@@ -500,53 +492,6 @@ struct WideRow {
     }
 };
 
-// Complex rows (float2 cells) with pitches multiples of two cells: every row of W cells
-// is read as 16-byte pairs plus one or two single cells whatever its alignment
-// m = start mod 2: odd W: pairs at m, m + 2, ... and a single at (m ? 0 : W - 1); even W:
-// pairs at m, m + 2, ..., singles at (m ? 0 : W - 2) and W - 1.
-template <int W>
-struct WideRow2 {
-    static constexpr int kPairs = (W & 1) ? (W - 1) / 2 : W / 2 - 1;
-    int op, os;
-    __device__ __forceinline__ explicit WideRow2(int start) {
-        const int m = start & 1;
-        op = m;
-        os = m ? 0 : W - ((W & 1) ? 1 : 2);
-    }
-    __device__ __forceinline__ void load_offsets(int* z) const {
-        int k = 0;
-#pragma unroll
-        for (int i = 0; i < kPairs; ++i) {
-            z[k++] = op + 2 * i;
-            z[k++] = op + 2 * i + 1;
-        }
-        z[k++] = os;
-        if (!(W & 1)) z[k++] = W - 1;
-    }
-    __device__ __forceinline__ float2 dot(const float2* row, const float* wl) const {
-        float qr = 0.f, qi = 0.f;
-        int k = 0;
-#pragma unroll
-        for (int i = 0; i < kPairs; ++i) {
-            const float4 v = *reinterpret_cast<const float4*>(row + op + 2 * i);
-            qr = fmaf(wl[k], v.x, qr);
-            qi = fmaf(wl[k], v.y, qi);
-            qr = fmaf(wl[k + 1], v.z, qr);
-            qi = fmaf(wl[k + 1], v.w, qi);
-            k += 2;
-        }
-        const float2 a = row[os];
-        qr = fmaf(wl[k], a.x, qr);
-        qi = fmaf(wl[k], a.y, qi);
-        if (!(W & 1)) {
-            const float2 b = row[W - 1];
-            qr = fmaf(wl[k + 1], b.x, qr);
-            qi = fmaf(wl[k + 1], b.y, qi);
-        }
-        return make_float2(qr, qi);
-    }
-};
-
 // ES weights of one axis at the load-order offsets z of a WideRow.
 template <int W>
 __device__ __forceinline__ void weights_at(float t, const int* z, float beta_l2e, float* wl) {
@@ -726,18 +671,19 @@ struct SortedCopies {
     }
 };
 
-template <int W, bool CPLX, class Sorted>
+// Real interpolation: out[j * ostride] = sum of the w^3 real-weighted taps of the
+// (real) fine grid.  A complex result is two such passes, over the real and the
+// imaginary part of the fine grid, into the interleaved output (ostride 2).
+template <int W, class Sorted>
 __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
-    const typename Cell<CPLX>::T* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx,
-    Sorted pts, float2* __restrict__ out_c, float* __restrict__ out_r, const int4* __restrict__ chunks,
-    int* __restrict__ ctrl) {
-    typedef typename Cell<CPLX>::T T;
+    const float* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx, Sorted pts,
+    float* __restrict__ out, int ostride, const int4* __restrict__ chunks, int* __restrict__ ctrl) {
     extern __shared__ float4 sub_raw[];  // 16-byte aligned for the wide row reads
-    T* sub = reinterpret_cast<T*>(sub_raw);
+    float* sub = reinterpret_cast<float*>(sub_raw);
     // LDS layout x * SSP + y * SP + z (interp_pitch)
     const int S = bs + W - 1, S3 = S * S * S;
     int SP, SSP;
-    interp_pitch(S, W, CPLX, &SP, &SSP);
+    interp_pitch(S, W, &SP, &SSP);
     __shared__ int chunk_id;
     const int total = ctrl[0];
     for (;;) {
@@ -775,64 +721,21 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
                 lz = 0;
             }
             const int boff = lx * SSP + ly * SP + lz;
-            float sr = 0.f, si = 0.f;
-            if (CPLX && CMTIP_INTERP_READS != 1) {
-                const WideRow2<W> wr(boff);
-                int zo[W];
-                wr.load_offsets(zo);
-                float wl[W];
-                weights_at<W>(tz, zo, beta_l2e, wl);
-                const float2* base = reinterpret_cast<const float2*>(sub) + boff;
+            const WideRow<W, CMTIP_INTERP_READS> wr(boff);
+            int zo[W];
+            wr.load_offsets(zo);
+            float wl[W];
+            weights_at<W>(tz, zo, beta_l2e, wl);
+            const float* base = sub + boff;
+            float sr = 0.f;
 #pragma unroll
-                for (int a = 0; a < W; ++a) {
-                    float pr = 0.f, pi = 0.f;
+            for (int a = 0; a < W; ++a) {
+                float pr = 0.f;
 #pragma unroll
-                    for (int bb = 0; bb < W; ++bb) {
-                        const float2 q = wr.dot(base + a * SSP + bb * SP, wl);
-                        pr = fmaf(wy[bb], q.x, pr);
-                        pi = fmaf(wy[bb], q.y, pi);
-                    }
-                    sr = fmaf(wx[a], pr, sr);
-                    si = fmaf(wx[a], pi, si);
-                }
-            } else if (CPLX) {
-                const T* base = sub + boff;
-#pragma unroll
-                for (int a = 0; a < W; ++a) {
-                    float pr = 0.f, pi = 0.f;
-#pragma unroll
-                    for (int bb = 0; bb < W; ++bb) {
-                        const T* row = base + a * SSP + bb * SP;
-                        float qr = 0.f, qi = 0.f;
-#pragma unroll
-                        for (int cc = 0; cc < W; ++cc) {
-                            const float2 v = *reinterpret_cast<const float2*>(row + cc);
-                            qr = fmaf(wz[cc], v.x, qr);
-                            qi = fmaf(wz[cc], v.y, qi);
-                        }
-                        pr = fmaf(wy[bb], qr, pr);
-                        pi = fmaf(wy[bb], qi, pi);
-                    }
-                    sr = fmaf(wx[a], pr, sr);
-                    si = fmaf(wx[a], pi, si);
-                }
-            } else {
-                const WideRow<W, CMTIP_INTERP_READS> wr(boff);
-                int zo[W];
-                wr.load_offsets(zo);
-                float wl[W];
-                weights_at<W>(tz, zo, beta_l2e, wl);
-                const float* base = reinterpret_cast<const float*>(sub) + boff;
-#pragma unroll
-                for (int a = 0; a < W; ++a) {
-                    float pr = 0.f;
-#pragma unroll
-                    for (int bb = 0; bb < W; ++bb) pr = fmaf(wy[bb], wr.dot(base + a * SSP + bb * SP, wl), pr);
-                    sr = fmaf(wx[a], pr, sr);
-                }
+                for (int bb = 0; bb < W; ++bb) pr = fmaf(wy[bb], wr.dot(base + a * SSP + bb * SP, wl), pr);
+                sr = fmaf(wx[a], pr, sr);
             }
-            if (out_c) out_c[j] = make_float2(sr, si);
-            if (out_r) out_r[j] = sr;
+            out[j * ostride] = sr;
         }
         __syncthreads();
     }
@@ -1482,34 +1385,30 @@ int reset_work_counter(cmtip_nufft_plan_s* p, hipStream_t st) {
 
 template <int W>
 struct InterpLaunch {
-    template <bool CPLX, class Sorted>
-    static void launch(cmtip_nufft_plan_s* p, Sorted src, float2* out_c, float* out_r, hipStream_t st) {
-        const size_t lds = subgrid_lds(p->bs, W, false, CPLX);
-        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : (CPLX ? 2 : 4)));
+    template <class Sorted>
+    static void launch(cmtip_nufft_plan_s* p, Sorted src, float* out, int ostride, hipStream_t st) {
+        const size_t lds = subgrid_lds(p->bs, W, false, false);
+        const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : 4));
         static bool attr_set = false;
         if (!attr_set && lds > kDefaultLds) {
-            allow_large_lds((const void*)interp_binned_kernel<W, CPLX, Sorted>);
+            allow_large_lds((const void*)interp_binned_kernel<W, Sorted>);
             attr_set = true;
         }
-        hipLaunchKernelGGL((interp_binned_kernel<W, CPLX, Sorted>), dim3(grid), dim3(kSpreadBlock), lds, st,
-                           p->fine.as<const typename Cell<CPLX>::T>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs,
-                           p->nbx, src, out_c, out_r, p->chunks.as<const int4>(), p->ctrl.as<int>());
+        hipLaunchKernelGGL((interp_binned_kernel<W, Sorted>), dim3(grid), dim3(kSpreadBlock), lds, st,
+                           p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx, src, out,
+                           ostride, p->chunks.as<const int4>(), p->ctrl.as<int>());
     }
-    // rot: the slicing route's rotated pixels (real output only), else the setpts points
-    static int run(cmtip_nufft_plan_s* p, bool cplx, float2* out_c, float* out_r, const PtsRotated* rot,
-                   hipStream_t st) {
+    // real interpolation of the fine grid into out[j * ostride]; rot: the slicing
+    // route's rotated pixels, else the setpts points
+    static int run(cmtip_nufft_plan_s* p, float* out, int ostride, const PtsRotated* rot, hipStream_t st) {
         if (p->npts == 0) return CMTIP_OK;
         CMTIP_TRY(reset_work_counter(p, st));
-        if (rot) {
-            launch<false>(p, SortedRotated{*rot, p->perm.as<const int>()}, out_c, out_r, st);
-        } else {
-            const SortedCopies src{p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
-                                   p->perm.as<const int>()};
-            if (cplx)
-                launch<true>(p, src, out_c, out_r, st);
-            else
-                launch<false>(p, src, out_c, out_r, st);
-        }
+        if (rot)
+            launch(p, SortedRotated{*rot, p->perm.as<const int>()}, out, ostride, st);
+        else
+            launch(p, SortedCopies{p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
+                                   p->perm.as<const int>()},
+                   out, ostride, st);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -2013,11 +1912,18 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     }
     if (p->type == 2) {
         p->slice_ready = 0;
-        // the interpolation weights are real: Re / Im of c interpolate Re / Im of the fine grid
-        CMTIP_TRY(grid_to_fine(p, f, grid_real, support, pts_real, st, pts_imag));
-        return dispatch_w<InterpLaunch>(p->w, p, !pts_real,
-                                        pts_real ? (float2*)nullptr : (float2*)c,
-                                        pts_real ? (float*)c : (float*)nullptr, (const PtsRotated*)nullptr, st);
+        // the interpolation weights are real: Re / Im of c interpolate Re / Im of the fine
+        // grid, so every form is one or two real passes (complex c: real part into the
+        // even floats, then the imaginary part into the odd ones)
+        if (pts_real) {
+            CMTIP_TRY(grid_to_fine(p, f, grid_real, support, true, st, pts_imag));
+            return dispatch_w<InterpLaunch>(p->w, p, (float*)c, 1, (const PtsRotated*)nullptr, st);
+        }
+        for (int part = 0; part < 2; ++part) {
+            CMTIP_TRY(grid_to_fine(p, f, grid_real, support, true, st, part == 1));
+            CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, (float*)c + part, 2, (const PtsRotated*)nullptr, st));
+        }
+        return CMTIP_OK;
     }
     // type 1
     p->slice_ready = 0;
@@ -2254,7 +2160,7 @@ int slice_binned(cmtip_nufft_plan_s* p, const double* s, const int* order, int64
         const PtsRotated rot{p->rinv.as<const double>() + 9 * o0, s, order, P, ext};
         p->npts = N;
         CMTIP_TRY(bin_points(p, st, &rot));
-        CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, false, (float2*)nullptr, out + o0 * P, &rot, st));
+        CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, out + o0 * P, 1, &rot, st));
     }
     // npts > 0 with no coordinates: execute reports that setpts is needed
     return CMTIP_OK;

@@ -113,7 +113,7 @@ def spread_candidates(w, nufft_type):
     subgrid edge - w + 1) whose LDS subgrid fits a workgroup in every execute form of
     the plan type."""
     out = []
-    per_cell = 16 if nufft_type == 1 else 8  # complex: 64-bit pairs (spread) / float2 (interp)
+    per_cell = 16 if nufft_type == 1 else 4  # 64-bit pairs (complex spread) / float (interpolation)
     reserve = 16384 if nufft_type == 1 else 256  # the kernels' static LDS
     for S in range(10, 25, 2):
         bs = S - w + 1

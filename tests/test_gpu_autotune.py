@@ -83,9 +83,9 @@ def test_set_bins_rejects_and_reports():
     with pytest.raises(CmtipError):
         p1.set_bins(15, 0, 0)
     p2 = NufftPlan(2, 21, -1)
-    p2.set_bins(20, 0, 0)  # float2 subgrids (26^3 cells, padded pitches) fit a larger bin than spreading ones
+    p2.set_bins(26, 0, 0)  # float interpolation subgrids (32^3 cells) fit a larger bin than spreading ones
     with pytest.raises(CmtipError):
-        p2.set_bins(21, 0, 0)
+        p2.set_bins(27, 0, 0)
 
 
 @pytest.mark.parametrize("nufft_type", [1, 2])

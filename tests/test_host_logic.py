@@ -66,7 +66,7 @@ def test_spread_candidates_fit_lds():
         for typ in (1, 2):
             cands = spread_candidates(w, typ)
             assert cands, (w, typ)
-            per_cell = 16 if typ == 1 else 8
+            per_cell = 16 if typ == 1 else 4
             for bs, chunk, wpc in cands:
                 S = bs + w - 1
                 assert bs >= 2 and chunk == 0 and wpc == 0
