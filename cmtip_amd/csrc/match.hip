@@ -954,6 +954,7 @@ std::mutex g_blas_mutex;
 std::map<int, BlasLt> g_blas;  // per device; kept until the process ends
 
 constexpr size_t kBlasWorkspace = (size_t)64 << 20;
+constexpr int kBlasCandidates = 8;
 
 int blas_fail(const char* what, hipblasStatus_t st) {
     set_error("match_cross_bf16: %s failed (hipblasStatus %d)", what, (int)st);
@@ -1006,11 +1007,14 @@ extern "C" int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const ui
             rc = blas_fail("matrix layouts", st);
             break;
         }
+        const float alpha = 1.0f, beta = 0.0f;
         const auto key = std::make_tuple(n_ref, n_img, K);
         auto it = bl.algos.find(key);
         if (it == bl.algos.end()) {
+            // first call for this shape: time up to kBlasCandidates heuristic algorithms
+            // once each (HIP events on the stream; one synchronisation) and keep the fastest
             hipblasLtMatmulPreference_t pref = nullptr;
-            hipblasLtMatmulHeuristicResult_t heur[4];
+            hipblasLtMatmulHeuristicResult_t heur[kBlasCandidates];
             int found = 0;
             const uint64_t ws = bl.ws_size;
             st = hipblasLtMatmulPreferenceCreate(&pref);
@@ -1018,15 +1022,44 @@ extern "C" int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const ui
                 st = hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &ws,
                                                            sizeof(ws));
             if (st == HIPBLAS_STATUS_SUCCESS)
-                st = hipblasLtMatmulAlgoGetHeuristic(bl.handle, desc, la, lb, lc, lc, pref, 4, heur, &found);
+                st = hipblasLtMatmulAlgoGetHeuristic(bl.handle, desc, la, lb, lc, lc, pref, kBlasCandidates, heur,
+                                                     &found);
             if (pref) (void)hipblasLtMatmulPreferenceDestroy(pref);
             if (st != HIPBLAS_STATUS_SUCCESS || found < 1) {
                 rc = blas_fail("algorithm heuristic", st);
                 break;
             }
-            it = bl.algos.emplace(key, heur[0].algo).first;
+            int pick = 0;
+            if (found > 1) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                float best_ms = 0.f;
+                bool have = false;
+                // warm-up launch (code loading), then one timed launch per candidate
+                (void)hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc, &heur[0].algo,
+                                      bl.workspace, bl.ws_size, as_stream(stream));
+                if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+                    for (int a = 0; a < found; ++a) {
+                        if (heur[a].state != HIPBLAS_STATUS_SUCCESS || heur[a].workspaceSize > bl.ws_size) continue;
+                        float ms = 0.f;
+                        if (hipEventRecord(e0, as_stream(stream)) != hipSuccess ||
+                            hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc,
+                                            &heur[a].algo, bl.workspace, bl.ws_size,
+                                            as_stream(stream)) != HIPBLAS_STATUS_SUCCESS ||
+                            hipEventRecord(e1, as_stream(stream)) != hipSuccess ||
+                            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                            continue;
+                        if (!have || ms < best_ms) {
+                            have = true;
+                            best_ms = ms;
+                            pick = a;
+                        }
+                    }
+                }
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+            }
+            it = bl.algos.emplace(key, heur[pick].algo).first;
         }
-        const float alpha = 1.0f, beta = 0.0f;
         st = hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc, &it->second, bl.workspace,
                              bl.ws_size, as_stream(stream));
         if (st != HIPBLAS_STATUS_SUCCESS) rc = blas_fail("hipblasLtMatmul", st);
