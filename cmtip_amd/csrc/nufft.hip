@@ -1191,14 +1191,11 @@ __device__ __forceinline__ void tile_box(bool active, int lx, int ly, int lz, in
     ex[2] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lz : -big) + W - mn[2]);
 }
 
+// Bin key of tile t (see above); nbins = the overflow bin.
 template <int W>
-__global__ void __launch_bounds__(kBlock) tile_key_kernel(TileRot rot, int64_t ntiles, int n, float hi, float lo,
-                                                          int nbx, int* __restrict__ tile_bin,
-                                                          int* __restrict__ count) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= ntiles) return;
+__device__ __forceinline__ int tile_key(const TileRot& rot, int64_t t, int n, float hi, float lo, int nbx) {
     const int64_t o = t / rot.tpo, first = (t - o * rot.tpo) * 64;
-    const int last = (int)std::min<int64_t>(63, rot.P - 1 - first);
+    const int last = (int)min((int64_t)63, rot.P - 1 - first);
     const int corner[4] = {0, 21, 42, 63};
     int mn[3] = {1 << 29, 1 << 29, 1 << 29}, mx[3] = {-(1 << 29), -(1 << 29), -(1 << 29)};
 #pragma unroll
@@ -1214,7 +1211,6 @@ __global__ void __launch_bounds__(kBlock) tile_key_kernel(TileRot rot, int64_t n
             mx[d] = max(mx[d], l);
         }
     }
-    const int nbins = nbx * nbx * nbx;
     bool fit = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -1222,11 +1218,39 @@ __global__ void __launch_bounds__(kBlock) tile_key_kernel(TileRot rot, int64_t n
         const int ext = mx[d] + 1 - mn[d] + W;
         fit = fit && ext <= kTileE && ext <= n;
     }
-    const int key = fit ? ((wrap_n(mn[0], n) / kTileBs) * nbx + wrap_n(mn[1], n) / kTileBs) * nbx +
-                              wrap_n(mn[2], n) / kTileBs
-                        : nbins;  // overflow
+    return fit ? ((wrap_n(mn[0], n) / kTileBs) * nbx + wrap_n(mn[1], n) / kTileBs) * nbx + wrap_n(mn[2], n) / kTileBs
+               : nbx * nbx * nbx;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) tile_key_kernel(TileRot rot, int64_t ntiles, int n, float hi, float lo,
+                                                          int nbx, int* __restrict__ tile_bin,
+                                                          int* __restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ntiles) return;
+    const int key = tile_key<W>(rot, t, n, hi, lo, nbx);
     tile_bin[t] = key;
     atomicAdd(count + key, 1);
+}
+
+// Workgroup-privatised form (the bins, overflow included, fit an LDS histogram).
+template <int W>
+__global__ void __launch_bounds__(kBinBlock) tile_key_local_kernel(TileRot rot, int64_t ntiles, int64_t per_block,
+                                                                  int n, float hi, float lo, int nbx, int nkeys,
+                                                                  int* __restrict__ tile_bin,
+                                                                  int* __restrict__ count) {
+    extern __shared__ int hist[];
+    for (int i = threadIdx.x; i < nkeys; i += kBinBlock) hist[i] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * per_block, t1 = min(ntiles, t0 + per_block);
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBinBlock) {
+        const int key = tile_key<W>(rot, t, n, hi, lo, nbx);
+        tile_bin[t] = key;
+        atomicAdd(hist + key, 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nkeys; i += kBinBlock)
+        if (hist[i]) atomicAdd(count + i, hist[i]);
 }
 
 // Neighbouring tiles mostly lie in different bins: plain atomics (the order of tiles
@@ -1555,13 +1579,36 @@ struct TileLaunch {
         CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
         const unsigned gb = (unsigned)((nbins + 1 + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), nbins + 1, 0);
-        hipLaunchKernelGGL(tile_key_kernel<W>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, rot, ntiles, p->n,
-                           p->tc_hi, p->tc_lo, nbx, p->bin_of.as<int>(), p->count.as<int>());
+        const bool local = nbins + 1 <= kLocalBinsMax;
+        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((ntiles + 8191) / 8192, 4 * cu_count()));
+        const int64_t per_block = (ntiles + blocks - 1) / blocks;
+        const size_t hlds = sizeof(int) * (size_t)(nbins + 1);
+        if (local) {
+            static bool attr_set = false;
+            if (!attr_set) {
+                allow_large_lds((const void*)tile_key_local_kernel<W>);
+                allow_large_lds((const void*)bin_scatter_local_kernel<false>);
+                attr_set = true;
+            }
+            hipLaunchKernelGGL(tile_key_local_kernel<W>, dim3((unsigned)blocks), dim3(kBinBlock), hlds, st, rot,
+                               ntiles, per_block, p->n, p->tc_hi, p->tc_lo, nbx, (int)(nbins + 1),
+                               p->bin_of.as<int>(), p->count.as<int>());
+        } else {
+            hipLaunchKernelGGL(tile_key_kernel<W>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, rot, ntiles, p->n,
+                               p->tc_hi, p->tc_lo, nbx, p->bin_of.as<int>(), p->count.as<int>());
+        }
         hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), nbins + 1, kTileChunk,
                            p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
-        hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
-                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                           p->perm.as<int>());
+        if (local)
+            hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), hlds, st,
+                               (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, ntiles, per_block,
+                               (int)(nbins + 1), p->bin_of.as<const int>(), p->offset.as<const int>(),
+                               p->count.as<int>(), p->perm.as<int>(), (float*)nullptr, (float*)nullptr,
+                               (float*)nullptr);
+        else
+            hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
+                               p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
+                               p->perm.as<int>());
         hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                            p->nchunk.as<const int>(), nbins + 1, kTileChunk, p->chunks.as<int4>());
         CMTIP_LAUNCH_CHECK();
