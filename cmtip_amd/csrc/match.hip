@@ -31,6 +31,9 @@ constexpr int BM = 128;  // references per workgroup tile
 constexpr int BN = 128;  // images per workgroup tile
 constexpr unsigned long long kEmpty = ~0ull;
 
+// host: null or 16-byte aligned (the float4 forms of the row kernels)
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 __device__ __forceinline__ unsigned long long pack_key(float d2, uint32_t r) {
     d2 = fmaxf(d2, 0.0f);  // sklearn clamps negative squared distances to 0
     return ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned long long)r;
@@ -101,6 +104,35 @@ __global__ void prepare_images_kernel(const float* __restrict__ B, int64_t total
     const float v = w2[p] * (c ? B[i] - c[p] : B[i]);
     if (Bw) Bw[i] = v;
     if (Bw_bf16) Bw_bf16[i] = f32_to_bf16(v);
+}
+
+// The same per row (one workgroup per image) for P a multiple of 4 and 16-byte aligned
+// rows: float4 loads, one 8-byte store of four bfloat16 values.
+__global__ void __launch_bounds__(kThreads) prepare_images4_kernel(const float* __restrict__ B, int64_t P,
+                                                                   const float* __restrict__ w2,
+                                                                   const float* __restrict__ c,
+                                                                   float* __restrict__ Bw,
+                                                                   uint16_t* __restrict__ Bw_bf16) {
+    const int64_t base = (int64_t)blockIdx.x * P, P4 = P >> 2;
+    const float4* B4 = reinterpret_cast<const float4*>(B + base);
+    const float4* w4 = reinterpret_cast<const float4*>(w2);
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+#pragma unroll 4
+    for (int64_t j = threadIdx.x; j < P4; j += kThreads) {
+        const float4 b = B4[j], w = w4[j];
+        float4 v;
+        if (c) {
+            const float4 cc = c4[j];
+            v = make_float4(w.x * (b.x - cc.x), w.y * (b.y - cc.y), w.z * (b.z - cc.z), w.w * (b.w - cc.w));
+        } else {
+            v = make_float4(w.x * b.x, w.y * b.y, w.z * b.z, w.w * b.w);
+        }
+        if (Bw) reinterpret_cast<float4*>(Bw + base)[j] = v;
+        if (Bw_bf16)
+            reinterpret_cast<uint2*>(Bw_bf16 + base)[j] =
+                make_uint2((uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16),
+                           (uint32_t)f32_to_bf16(v.z) | ((uint32_t)f32_to_bf16(v.w) << 16));
+    }
 }
 
 // Column sums of X (rows x P) in FP64: block (pixel group of 64, kStatWaves waves)
@@ -185,6 +217,46 @@ __global__ void center_rows_kernel(const float* X, int64_t P, const double* __re
         s += (double)w2[p] * (double)y * (double)y;
         if (Y) Y[base + p] = y;
         if (Y16) Y16[base + p] = f32_to_bf16(y);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+        norms[blockIdx.x] = (float)t;
+    }
+}
+
+// center_rows_kernel for P a multiple of 4 and 16-byte aligned rows (Y16 rows 8-byte
+// aligned): float4 loads, float4 / 8-byte bfloat16 stores; y and its roundings are the
+// same values (fmaf(g, x, -0) = g x), only the FP64 norm adds in another order.
+__global__ void __launch_bounds__(kThreads) center_rows4_kernel(const float* X, int64_t P,
+                                                                const double* __restrict__ gamma,
+                                                                const float* __restrict__ c,
+                                                                const float* __restrict__ w2, float* Y,
+                                                                uint16_t* __restrict__ Y16,
+                                                                float* __restrict__ norms) {
+    __shared__ double red[kThreads / 64];
+    const float g = gamma ? (float)gamma[0] : 1.0f;
+    const int64_t base = (int64_t)blockIdx.x * P, P4 = P >> 2;
+    const float4* X4 = reinterpret_cast<const float4*>(X + base);
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+    const float4* w4 = reinterpret_cast<const float4*>(w2);
+    double s = 0.0;
+#pragma unroll 4
+    for (int64_t j = threadIdx.x; j < P4; j += kThreads) {
+        const float4 x = X4[j], w = w4[j];
+        const float4 cc = c ? c4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 y = make_float4(fmaf(g, x.x, -cc.x), fmaf(g, x.y, -cc.y), fmaf(g, x.z, -cc.z),
+                                     fmaf(g, x.w, -cc.w));
+        s += ((double)w.x * (double)y.x * (double)y.x + (double)w.y * (double)y.y * (double)y.y) +
+             ((double)w.z * (double)y.z * (double)y.z + (double)w.w * (double)y.w * (double)y.w);
+        if (Y) reinterpret_cast<float4*>(Y + base)[j] = y;
+        if (Y16)
+            reinterpret_cast<uint2*>(Y16 + base)[j] =
+                make_uint2((uint32_t)f32_to_bf16(y.x) | ((uint32_t)f32_to_bf16(y.y) << 16),
+                           (uint32_t)f32_to_bf16(y.z) | ((uint32_t)f32_to_bf16(y.w) << 16));
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -821,8 +893,14 @@ extern "C" int cmtip_center_rows(const float* X, int64_t rows, int64_t P, const 
         return CMTIP_ERR_ARG;
     }
     if (rows == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(center_rows_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P, gamma,
-                       c, w2, Y, Y16, norms);
+    const bool vec = P % 4 == 0 && aligned16(X) && aligned16(Y) && aligned16(c) && aligned16(w2) &&
+                     (reinterpret_cast<uintptr_t>(Y16) & 7) == 0;
+    if (vec)
+        hipLaunchKernelGGL(center_rows4_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P,
+                           gamma, c, w2, Y, Y16, norms);
+    else
+        hipLaunchKernelGGL(center_rows_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P,
+                           gamma, c, w2, Y, Y16, norms);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -840,8 +918,13 @@ extern "C" int cmtip_match_prepare_images_centered(const float* B, int64_t n_img
     }
     const int64_t total = n_img * P;
     if (total == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(prepare_images_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads),
-                       0, as_stream(stream), B, total, P, w2, c, Bw, Bw_bf16);
+    if (P % 4 == 0 && n_img <= 0x7fffffffLL && aligned16(B) && aligned16(w2) && aligned16(c) && aligned16(Bw) &&
+        (reinterpret_cast<uintptr_t>(Bw_bf16) & 7) == 0)
+        hipLaunchKernelGGL(prepare_images4_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), B, P,
+                           w2, c, Bw, Bw_bf16);
+    else
+        hipLaunchKernelGGL(prepare_images_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, as_stream(stream), B, total, P, w2, c, Bw, Bw_bf16);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -909,7 +992,12 @@ namespace {
 // rounds of a wave-wide minimum over the list heads then give the merged top-k, written
 // back without atomics (ties keep the smallest reference index).
 constexpr int kCrossWaves = 4;
+constexpr int kCrossBatch = 4;  // loads in flight per lane before the inserts
 
+// VEC: n_ref a multiple of 4 and 16-byte aligned Dt and an (float4 loads).  The lists
+// hold the lane's smallest keys whatever the order the references arrive in, so both
+// forms give the same top-k.
+template <bool VEC>
 __global__ void __launch_bounds__(64 * kCrossWaves) topk_cross_kernel(const float* __restrict__ Dt, int64_t n_ref,
                                                                       uint32_t ref_offset, int64_t n_img,
                                                                       const float* __restrict__ an,
@@ -928,8 +1016,47 @@ __global__ void __launch_bounds__(64 * kCrossWaves) topk_cross_kernel(const floa
     unsigned long long* slots = cand + i * k;
     if (lane < k) topk_insert(lst, slots[lane]);
     const float* row = Dt + i * n_ref;
-    for (int64_t r = lane; r < n_ref; r += 64)
-        topk_insert(lst, pack_key(fmaf(g2, an[r], fmaf(m2g, row[r], bni)), (uint32_t)r + ref_offset));
+    if (VEC) {
+        const float4* row4 = reinterpret_cast<const float4*>(row);
+        const float4* an4 = reinterpret_cast<const float4*>(an);
+        const int64_t n4 = n_ref >> 2;
+        for (int64_t b0 = 0; b0 < n4; b0 += 64 * kCrossBatch) {
+            float4 d[kCrossBatch], a[kCrossBatch];
+#pragma unroll
+            for (int u = 0; u < kCrossBatch; ++u) {
+                const int64_t j = b0 + u * 64 + lane;
+                d[u] = j < n4 ? row4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                a[u] = j < n4 ? an4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int u = 0; u < kCrossBatch; ++u) {
+                const int64_t j = b0 + u * 64 + lane;
+                if (j < n4) {
+                    const uint32_t r = (uint32_t)(4 * j) + ref_offset;
+                    topk_insert(lst, pack_key(fmaf(g2, a[u].x, fmaf(m2g, d[u].x, bni)), r));
+                    topk_insert(lst, pack_key(fmaf(g2, a[u].y, fmaf(m2g, d[u].y, bni)), r + 1));
+                    topk_insert(lst, pack_key(fmaf(g2, a[u].z, fmaf(m2g, d[u].z, bni)), r + 2));
+                    topk_insert(lst, pack_key(fmaf(g2, a[u].w, fmaf(m2g, d[u].w, bni)), r + 3));
+                }
+            }
+        }
+    } else {
+        constexpr int U = 4 * kCrossBatch;
+        for (int64_t b0 = 0; b0 < n_ref; b0 += 64 * U) {
+            float d[U], a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t r = b0 + u * 64 + lane;
+                d[u] = r < n_ref ? row[r] : 0.0f;
+                a[u] = r < n_ref ? an[r] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t r = b0 + u * 64 + lane;
+                if (r < n_ref) topk_insert(lst, pack_key(fmaf(g2, a[u], fmaf(m2g, d[u], bni)), (uint32_t)r + ref_offset));
+            }
+        }
+    }
     // k rounds: the wave minimum of the list heads; its lane pops it
     for (int s = 0; s < k; ++s) {
         unsigned long long m = lst[0];
@@ -1089,8 +1216,12 @@ extern "C" int cmtip_match_topk_cross(const float* Dt, int64_t n_ref, int64_t re
         set_error("match_topk_cross: too many images");
         return CMTIP_ERR_ARG;
     }
-    hipLaunchKernelGGL(topk_cross_kernel, dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0, as_stream(stream), Dt,
-                       n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand);
+    if (n_ref % 4 == 0 && aligned16(Dt) && aligned16(an))
+        hipLaunchKernelGGL(topk_cross_kernel<true>, dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0,
+                           as_stream(stream), Dt, n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand);
+    else
+        hipLaunchKernelGGL(topk_cross_kernel<false>, dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0,
+                           as_stream(stream), Dt, n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

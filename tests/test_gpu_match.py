@@ -180,13 +180,16 @@ def test_match_kernels_vs_fp64(n_ref, n_img, P):
     assert same > 0.9
 
 
-def test_topk_cross_exact():
+@pytest.mark.parametrize("split,n_ref", [(180, 300), (181, 300), (1000, 2601)])
+def test_topk_cross_exact(split, n_ref):
     # keys formed and merged exactly as numpy forms them in float32 (gamma = 1), over two
-    # reference batches, with repeated distances (ties keep the smaller reference index)
+    # reference batches, with repeated distances (ties keep the smaller reference index);
+    # batches of a multiple of 4 references on 16-byte aligned rows take the float4 form,
+    # the others the scalar form
     from cmtip_amd._lib import check, lib, ptr, stream
     rng = np.random.default_rng(5)
     d = dev()
-    n_img, n_ref, k = 37, 300, 8
+    n_img, k = 37, 8
     Dt = np.round(rng.random((n_img, n_ref)) * 64).astype(np.float32) / 64  # many equal values
     an = (rng.random(n_ref) * 2).astype(np.float32)
     bn = (rng.random(n_img) * 2).astype(np.float32)
@@ -194,7 +197,7 @@ def test_topk_cross_exact():
     cand = torch.full((n_img, k), -1, dtype=torch.int64, device=d)
     an_d, bn_d = torch.as_tensor(an, device=d), torch.as_tensor(bn, device=d)
     say("top-k from a cross-term block")
-    for r0, r1 in ((0, 180), (180, 300)):
+    for r0, r1 in ((0, split), (split, n_ref)):
         Db = torch.as_tensor(np.ascontiguousarray(Dt[:, r0:r1]), device=d)
         check(L.cmtip_match_topk_cross(ptr(Db), r1 - r0, r0, n_img, ptr(an_d[r0:r1]), ptr(bn_d), None, k,
                                        ptr(cand), stream()))
@@ -203,6 +206,56 @@ def test_topk_cross_exact():
     want = np.sort(keys, axis=1)[:, :k]
     got = cand.cpu().numpy().view(np.uint64)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("P", [1024, 1023])
+def test_row_kernels_float4_and_scalar_forms(P):
+    # centring (y = gamma x - c, its bfloat16 copy, sum w2 y^2) and image preparation
+    # (w2 (b - c)) give the same values from their float4 form (P a multiple of 4, aligned
+    # rows) and their scalar form (rows one float off alignment)
+    from cmtip_amd._lib import check, lib, ptr, stream
+    d = dev()
+    g = torch.Generator(device=d).manual_seed(11)
+    rows = 300
+    L = lib()
+    X = torch.rand((rows, P), device=d, generator=g) * 100
+    c = torch.rand(P, device=d, generator=g) * 50
+    w2 = torch.rand(P, device=d, generator=g)
+    gamma = torch.tensor([0.37], dtype=torch.float64, device=d)
+    outs = []
+    say(f"row kernels P={P}")
+    for off in (0, 1):
+        Xb = torch.zeros(rows * P + 1, device=d)
+        Xb[off:off + rows * P] = X.reshape(-1)
+        cb, wb = torch.zeros(P + 1, device=d), torch.zeros(P + 1, device=d)
+        cb[off:off + P], wb[off:off + P] = c, w2
+        Xo, co, wo = Xb[off:off + rows * P], cb[off:off + P], wb[off:off + P]
+        Yb = torch.empty(rows * P + 1, device=d)
+        Y16 = torch.empty(rows * P + 4, dtype=torch.int16, device=d)
+        norms = torch.empty(rows, device=d)
+        check(L.cmtip_center_rows(ptr(Xo), rows, P, ptr(gamma), ptr(co), ptr(wo), ptr(Yb[off:]),
+                                  ptr(Y16[4 * off:]), ptr(norms), stream()))
+        Bw = torch.empty(rows * P + 1, device=d)
+        Bw16 = torch.empty(rows * P + 4, dtype=torch.int16, device=d)
+        check(L.cmtip_match_prepare_images_centered(ptr(Xo), rows, P, ptr(wo), ptr(co), ptr(Bw[off:]),
+                                                    ptr(Bw16[4 * off:]), stream()))
+        outs.append((Yb[off:off + rows * P].clone(), Y16[4 * off:4 * off + rows * P].clone(), norms.clone(),
+                     Bw[off:off + rows * P].clone(), Bw16[4 * off:4 * off + rows * P].clone(), co, wo))
+    (y0, h0, n0, b0, bh0, c0, w0), (y1, h1, n1, b1, bh1, _, _) = outs
+    assert torch.equal(y0, y1) and torch.equal(h0, h1) and torch.equal(b0, b1) and torch.equal(bh0, bh1)
+    assert float(((n0 - n1).abs() / n1.abs()).max()) < 1e-6
+    # against torch: w2 (b - c) exactly; bfloat16 copies rounded to nearest even; y and its
+    # FP64 norm to float32 rounding
+    xb = X.reshape(-1)
+    want_b = (X - c0[None, :]) * w0[None, :]
+    assert torch.equal(b0, want_b.reshape(-1))
+    assert torch.equal(bh0, want_b.reshape(-1).to(torch.bfloat16).view(torch.int16))
+    assert torch.equal(h0, y0.to(torch.bfloat16).view(torch.int16))
+    g32 = float(np.float32(0.37))  # the kernels scale by gamma rounded to float32
+    y64 = g32 * xb.double() - c0.double().repeat(rows)  # g x is exact in FP64
+    assert bool(((y0.double() - y64).abs() <= 1e-7 * y64.abs() + 1e-12).all())
+    n64 = (w0.double()[None, :] * y0.double().reshape(rows, P) ** 2).sum(1)
+    assert float(((n0.double() - n64).abs() / n64).max()) < 1e-6
 
 
 def test_cross_bf16_vs_fp32_product():
