@@ -417,11 +417,43 @@ def cpu_baseline(ac, s, ext, ref, imgs, n_img, n_ref, P):
     per_pair = t_match / (n_s * m_s)
     total = per_slice * n_ref + per_pair * n_ref * n_img
     log(f"[bench] cpu: slice {per_slice * 1e3:.1f} ms/ref, match {per_pair * 1e9:.1f} ns/pair")
+    parts = cpu_solve_parts(ac.shape[0], s, ext, n_s)
     return {"value": n_img / total, "unit": "images/s", "cores": cores, "kind": "port",
             "sample": f"oracle FP64 ES type-2 (eps 1e-6, C/OpenMP gridding + numpy FFT) for {n_s} references "
                       f"and sklearn euclidean_distances + argmin of those {n_s} x {m_s} images, extrapolated "
                       f"linearly to {n_ref} references x {n_img} images",
-            "threads_openmp": O.num_threads()}
+            "threads_openmp": O.num_threads(), "solve_parts": parts}
+
+
+def cpu_solve_parts(M, s, ext, n_o):
+    """The oracle's type-1 (real strengths, M modes, FP64 ES at eps 1e-6) over the
+    q-vectors of n_o image orientations, and one application of the normal operator of
+    the CG solve (Toeplitz embedding at 2M + the corner projection, numpy complex128
+    FFTs), beside the GPU's type1_real_M81 and cg_ms_per_iteration."""
+    from cmtip_amd import synth
+    from oracle import ac as OAC
+    from oracle import geom
+    from oracle import nufft as O
+
+    q = synth.random_quaternions(n_o, seed=1)
+    x, y, z = geom.rotated_coords(q, s, ext)
+    c = np.random.default_rng(0).random(len(x))
+    t = time.perf_counter()
+    O.nufft1(c, x, y, z, M, eps=1e-6)
+    t1 = time.perf_counter() - t
+    F = OAC.kernel_spectrum(x[:65536], y[:65536], z[:65536], M, eps=1e-6)
+    anti = OAC.gen_F_antisupport(M)
+    sup = np.ones((M,) * 3)
+    u = np.random.default_rng(1).random(M ** 3)
+    t = time.perf_counter()
+    for _ in range(2):
+        OAC.core_problem_convolution(u, M, F, 2 * M, sup) + 1e-4 * u + 1e3 * OAC.fourier_reg(u, sup, anti, M)
+    t_mv = (time.perf_counter() - t) / 2
+    log(f"[bench] cpu: type-1 {len(x) / t1 / 1e6:.2f} Mpts/s, normal operator {t_mv * 1e3:.0f} ms")
+    return {"type1_real_Mpts_s": len(x) / t1 / 1e6, "type1_points": len(x),
+            "normal_operator_ms": t_mv * 1e3,
+            "note": "type-1 over the q-vectors of the first image orientations; normal operator = the CG "
+                    "iteration's matrix-vector product (its vector updates excluded)"}
 
 
 if __name__ == "__main__":
