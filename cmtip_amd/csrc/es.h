@@ -7,6 +7,7 @@
 #pragma once
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 namespace cmtip {
@@ -30,6 +31,73 @@ inline double es_phi_host(double z, int w, double beta) {
     const double s = 1.0 - t * t;
     return s < 0.0 ? 0.0 : exp(beta * (sqrt(s) - 1.0));
 }
+
+// Polynomial form of the w taps of one axis (w <= kEsPolyMaxW).  With tau = t - l0 in
+// (w/2 - 1, w/2] and y = tau - (w - 1)/2 in (-1/2, 1/2], tap a weighs
+// f_a(y) = phi(a - tau); the mirror tap satisfies f_{w-1-a}(y) = f_a(-y).  Each f_a
+// (a < w/2) is the degree-(w + 1) polynomial interpolating it at the Chebyshev points of
+// [-1/2, 1/2]; it is stored as its even part E(y^2) and odd part y O(y^2), so the pair
+// is E + y O and E - y O.  Odd w adds the even part of the middle tap.  Interpolation
+// error against phi (max over y, double precision): w = 7, 5e-8 at the two outer taps
+// (the square-root edge of phi) and 2e-9 at the inner ones; at most ~eps / 20 for
+// every w <= 8.
+constexpr int kEsPolyMaxW = 8;
+constexpr int kEsPolyCoefs = 40;  // w = 8: 4 pairs x 10
+
+inline int es_poly_even(int w) { return (w + 1) / 2 + 1; }  // coefficients of y^0, y^2, ...
+inline int es_poly_odd(int w) { return (w + 2) / 2; }        // coefficients of y^1, y^3, ...
+
+// Monomial coefficients (y^0 .. y^D) of the interpolant of tap a at D + 1 Chebyshev points.
+inline void es_tap_fit(int w, double beta, int a, int D, double* c) {
+    const int n = D + 1;
+    std::vector<double> A((size_t)n * (n + 1));
+    for (int k = 0; k < n; ++k) {
+        const double y = 0.5 * cos((2.0 * k + 1.0) * M_PI / (2.0 * n));
+        const double tau = y + 0.5 * (w - 1);
+        double pw = 1.0;
+        for (int j = 0; j < n; ++j, pw *= y) A[(size_t)k * (n + 1) + j] = pw;
+        A[(size_t)k * (n + 1) + n] = es_phi_host(a - tau, w, beta);
+    }
+    // Gaussian elimination with partial pivoting
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (fabs(A[(size_t)r * (n + 1) + col]) > fabs(A[(size_t)piv * (n + 1) + col])) piv = r;
+        for (int j = 0; j <= n; ++j) std::swap(A[(size_t)col * (n + 1) + j], A[(size_t)piv * (n + 1) + j]);
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = A[(size_t)r * (n + 1) + col] / A[(size_t)col * (n + 1) + col];
+            for (int j = col; j <= n; ++j) A[(size_t)r * (n + 1) + j] -= f * A[(size_t)col * (n + 1) + j];
+        }
+    }
+    for (int j = 0; j < n; ++j) c[j] = A[(size_t)j * (n + 1) + n] / A[(size_t)j * (n + 1) + j];
+}
+
+// Fills c (kEsPolyCoefs floats) in the layout described above; false for w > kEsPolyMaxW.
+inline bool es_poly(int w, double beta, float* c) {
+    for (int i = 0; i < kEsPolyCoefs; ++i) c[i] = 0.0f;
+    if (w < 2 || w > kEsPolyMaxW) return false;
+    const int D = w + 1, ne = es_poly_even(w), no = es_poly_odd(w);
+    double m[kEsPolyMaxW + 2];
+    int k = 0;
+    for (int a = 0; a < w / 2; ++a) {
+        es_tap_fit(w, beta, a, D, m);
+        for (int j = 0; j < ne; ++j) c[k++] = (float)(2 * j <= D ? m[2 * j] : 0.0);
+        for (int j = 0; j < no; ++j) c[k++] = (float)(2 * j + 1 <= D ? m[2 * j + 1] : 0.0);
+    }
+    if (w & 1) {
+        es_tap_fit(w, beta, w / 2, D, m);
+        for (int j = 0; j < ne; ++j) c[k++] = (float)(2 * j <= D ? m[2 * j] : 0.0);
+    }
+    return true;
+}
+
+// The ES kernel as the kernels take it (by value): the exponent scale beta log2(e) of
+// the direct form and the tap polynomials of es_poly.
+struct EsKernel {
+    float beta_l2e;
+    float c[kEsPolyCoefs];
+};
 
 // Smallest even 2,3,5-smooth n >= max(2M, 2w).
 inline int fine_size(int M, int w) {

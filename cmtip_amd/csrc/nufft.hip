@@ -27,6 +27,7 @@ struct cmtip_nufft_plan_s {
     double beta = 0.0, eps = 0.0;
     float tc_hi = 0.f, tc_lo = 0.f;  // n / (2 pi) split in two floats
     float beta_l2e = 0.f;            // beta * log2(e), the kernels' exponent scale
+    EsKernel esk{};                  // beta_l2e and the tap polynomials, passed to the kernels
     Fft1D fft;
     DevBuf deconv;   // n floats
     DevBuf fine;     // n^3 float2 (used as n^3 float when real)
@@ -85,6 +86,43 @@ __device__ __forceinline__ float es_phi(float z, float two_over_w, float beta_l2
     const float r = __builtin_amdgcn_sqrtf(fmaxf(s, 0.0f));
     const float e = __builtin_amdgcn_exp2f(beta_l2e * (r - 1.0f));
     return s > 0.0f ? e : 0.0f;
+}
+
+// Tap weights of one axis in tap order, wt[a] = phi(a - tau) with tau = t - l0 in
+// (W/2 - 1, W/2]: for W <= kEsPolyMaxW from the tap polynomials (even part E(y^2) and
+// odd part y O(y^2) per mirror pair, y = tau - (W - 1)/2; full-rate FMAs only), else
+// from es_phi.
+template <int W>
+__device__ __forceinline__ void es_taps(float tau, const EsKernel& K, float* wt) {
+    if constexpr (W <= kEsPolyMaxW) {
+        constexpr int NE = (W + 1) / 2 + 1, NO = (W + 2) / 2;
+        const float y = tau - 0.5f * (float)(W - 1);
+        const float y2 = y * y;
+#pragma unroll
+        for (int j = 0; j < W / 2; ++j) {
+            const float* c = K.c + j * (NE + NO);
+            float e = c[NE - 1];
+#pragma unroll
+            for (int k = NE - 2; k >= 0; --k) e = fmaf(e, y2, c[k]);
+            float o = c[NE + NO - 1];
+#pragma unroll
+            for (int k = NO - 2; k >= 0; --k) o = fmaf(o, y2, c[NE + k]);
+            o *= y;
+            wt[j] = e + o;
+            wt[W - 1 - j] = e - o;
+        }
+        if constexpr ((W & 1) != 0) {
+            const float* c = K.c + (W / 2) * (NE + NO);
+            float e = c[NE - 1];
+#pragma unroll
+            for (int k = NE - 2; k >= 0; --k) e = fmaf(e, y2, c[k]);
+            wt[W / 2] = e;
+        }
+    } else {
+        const float tw = 2.0f / (float)W;
+#pragma unroll
+        for (int a = 0; a < W; ++a) wt[a] = es_phi((float)a - tau, tw, K.beta_l2e);
+    }
 }
 
 // Fine-grid coordinate t = x n / (2 pi) with the constant split in two floats, so the
@@ -401,14 +439,12 @@ struct Cell<true> {
 };
 
 template <int W>
-__device__ __forceinline__ void local_weights(float t, int l0w, int origin, float beta_l2e, int* loc, float* wt) {
+__device__ __forceinline__ void local_weights(float t, int l0w, int origin, const EsKernel& K, int* loc,
+                                              float* wt) {
     // l0w: wrapped first node; t: fine coordinate (unwrapped); origin: bin origin cell
     const float l0 = ceilf(t - 0.5f * (float)W);
-    const float d = l0 - t;
     *loc = l0w - origin;
-    const float tw = 2.0f / (float)W;
-#pragma unroll
-    for (int a = 0; a < W; ++a) wt[a] = es_phi(d + (float)a, tw, beta_l2e);
+    es_taps<W>(t - l0, K, wt);
 }
 
 // ---------------------------------------------------------------- wide LDS row reads
@@ -462,6 +498,25 @@ struct WideRow {
         z[k++] = os;
         if (!kQuad && !(W & 1)) z[k++] = W - 1;
     }
+    // W = 7 quad rows: tap-order weights wt into load order (the quad is taps oq .. oq + 3,
+    // oq in {0, 1, 2, 3}; the pair op, op + 1 with op in {0, 1, 4, 5}; the single 0 or 6)
+    __device__ __forceinline__ void pick(const float* wt, float* wl) const {
+        const bool q1 = (oq & 1) != 0, q2 = (oq & 2) != 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float s0 = q1 ? wt[i + 1] : wt[i];
+            const float s1 = q1 ? wt[i + 3] : wt[i + 2];
+            wl[i] = q2 ? s1 : s0;
+        }
+        const bool p1 = (op & 1) != 0, p4 = (op & 4) != 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float s0 = p1 ? wt[i + 1] : wt[i];
+            const float s1 = p1 ? wt[i + 5] : wt[i + 4];
+            wl[4 + i] = p4 ? s1 : s0;
+        }
+        wl[6] = os ? wt[6] : wt[0];
+    }
     // sum_k wl[k] * value_k over the row starting at row (16-byte aligned base + start)
     __device__ __forceinline__ float dot(const float* row, const float* wl) const {
         float q = 0.f;
@@ -492,13 +547,22 @@ struct WideRow {
     }
 };
 
-// ES weights of one axis at the load-order offsets z of a WideRow.
-template <int W>
-__device__ __forceinline__ void weights_at(float t, const int* z, float beta_l2e, float* wl) {
-    const float d = ceilf(t - 0.5f * (float)W) - t;
-    const float tw = 2.0f / (float)W;
+// ES weights of one axis in the load order of a WideRow: the tap polynomials picked
+// into that order (W = 7 quad rows), else es_phi at the load-order offsets.
+template <int W, int READS>
+__device__ __forceinline__ void weights_at(float t, const WideRow<W, READS>& wr, const EsKernel& K, float* wl) {
+    const float tau = t - ceilf(t - 0.5f * (float)W);
+    if constexpr (WideRow<W, READS>::kQuad && W <= kEsPolyMaxW) {
+        float wt[W];
+        es_taps<W>(tau, K, wt);
+        wr.pick(wt, wl);
+    } else {
+        int z[W];
+        wr.load_offsets(z);
+        const float tw = 2.0f / (float)W;
 #pragma unroll
-    for (int k = 0; k < W; ++k) wl[k] = es_phi(d + (float)z[k], tw, beta_l2e);
+        for (int k = 0; k < W; ++k) wl[k] = es_phi((float)z[k] - tau, tw, K.beta_l2e);
+    }
 }
 
 // Spreading accumulates in LDS in 64-bit fixed point: integer LDS atomics are an
@@ -524,11 +588,10 @@ __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsign
 
 template <int W, bool CPLX>
 __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
-    typename Cell<CPLX>::T* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx,
+    typename Cell<CPLX>::T* __restrict__ grid, int n, const EsKernel esk, float hi, float lo, int bs, int nbx,
     const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
     const int* __restrict__ perm, const typename Cell<CPLX>::T* __restrict__ c, const unsigned* __restrict__ cmax_bits,
     const int4* __restrict__ chunks, int* __restrict__ ctrl) {
-    typedef typename Cell<CPLX>::T T;
     constexpr int NC = CPLX ? 2 : 1;  // integer accumulators per cell
     extern __shared__ unsigned long long acc[];
     // LDS layout x * SSP + y * SP + z (spread_pitch)
@@ -606,9 +669,9 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             const int lzw = first_node_wrapped(zs[k], hi, lo, n, W, &tz);
             int lx, ly, lz;
             float wx[W], wy[W], wz[W];
-            local_weights<W>(tx, lxw, ox, beta_l2e, &lx, wx);
-            local_weights<W>(ty, lyw, oy, beta_l2e, &ly, wy);
-            local_weights<W>(tz, lzw, oz, beta_l2e, &lz, wz);
+            local_weights<W>(tx, lxw, ox, esk, &lx, wx);
+            local_weights<W>(ty, lyw, oy, esk, &ly, wy);
+            local_weights<W>(tz, lzw, oz, esk, &lz, wz);
             // lx, ly, lz lie in [0, bs) by construction of the bins; never index outside LDS
             if ((unsigned)lx >= (unsigned)bs || (unsigned)ly >= (unsigned)bs || (unsigned)lz >= (unsigned)bs) continue;
             float vr, vi = 0.f;
@@ -676,7 +739,7 @@ struct SortedCopies {
 // imaginary part of the fine grid, into the interleaved output (ostride 2).
 template <int W, class Sorted>
 __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
-    const float* __restrict__ grid, int n, float beta_l2e, float hi, float lo, int bs, int nbx, Sorted pts,
+    const float* __restrict__ grid, int n, const EsKernel esk, float hi, float lo, int bs, int nbx, Sorted pts,
     float* __restrict__ out, int ostride, const int4* __restrict__ chunks, int* __restrict__ ctrl) {
     extern __shared__ float4 sub_raw[];  // 16-byte aligned for the wide row reads
     float* sub = reinterpret_cast<float*>(sub_raw);
@@ -711,9 +774,9 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
             const int lzw = first_node_wrapped(pz, hi, lo, n, W, &tz);
             int lx, ly, lz;
             float wx[W], wy[W], wz[W];
-            local_weights<W>(tx, lxw, ox, beta_l2e, &lx, wx);
-            local_weights<W>(ty, lyw, oy, beta_l2e, &ly, wy);
-            local_weights<W>(tz, lzw, oz, beta_l2e, &lz, wz);
+            local_weights<W>(tx, lxw, ox, esk, &lx, wx);
+            local_weights<W>(ty, lyw, oy, esk, &ly, wy);
+            local_weights<W>(tz, lzw, oz, esk, &lz, wz);
             // lx, ly, lz lie in [0, bs) by construction of the bins; never index outside LDS
             if ((unsigned)lx >= (unsigned)bs || (unsigned)ly >= (unsigned)bs || (unsigned)lz >= (unsigned)bs) {
                 lx = 0;
@@ -722,10 +785,8 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
             }
             const int boff = lx * SSP + ly * SP + lz;
             const WideRow<W, CMTIP_INTERP_READS> wr(boff);
-            int zo[W];
-            wr.load_offsets(zo);
             float wl[W];
-            weights_at<W>(tz, zo, beta_l2e, wl);
+            weights_at<W>(tz, wr, esk, wl);
             const float* base = sub + boff;
             float sr = 0.f;
 #pragma unroll
@@ -967,11 +1028,8 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 }
 
 template <int W>
-__device__ __forceinline__ void row_weights(float t, int l0, float beta, float* wt) {
-    const float d = (float)l0 - t;
-    const float tw = 2.0f / (float)W;
-#pragma unroll
-    for (int a = 0; a < W; ++a) wt[a] = es_phi(d + (float)a, tw, beta);
+__device__ __forceinline__ void row_weights(float t, int l0, const EsKernel& K, float* wt) {
+    es_taps<W>(t - (float)l0, K, wt);
 }
 
 __device__ __forceinline__ int first_node(float t, int W) { return (int)ceilf(t - 0.5f * (float)W); }
@@ -985,9 +1043,9 @@ __device__ __forceinline__ int wrap_n(int l, int n) {
 // nodes lx, ly, lz (unwrapped), x / y weights given, z weights from tz.
 template <int W>
 __device__ __forceinline__ float direct_gather(const float* __restrict__ g, int n, float tz, int lx, int ly, int lz,
-                                               const float* wx, const float* wy, float beta) {
+                                               const float* wx, const float* wy, const EsKernel& K) {
     float wz[W];
-    row_weights<W>(tz, lz, beta, wz);
+    row_weights<W>(tz, lz, K, wz);
     int ix[W], iy[W], iz[W];
     const int bx = wrap_n(lx, n), by = wrap_n(ly, n), bz = wrap_n(lz, n);
 #pragma unroll
@@ -1013,7 +1071,7 @@ __device__ __forceinline__ float direct_gather(const float* __restrict__ g, int 
 }
 
 template <int W>
-__global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __restrict__ g, int n, float beta,
+__global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __restrict__ g, int n, const EsKernel esk,
                                                             float hi, float lo,
                                                             const double* __restrict__ rinv,
                                                             const double* __restrict__ s,
@@ -1101,16 +1159,14 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
     float val = 0.f;
     if (active) {
         float wx[W], wy[W];
-        row_weights<W>(tx, lx, beta, wx);
-        row_weights<W>(ty, ly, beta, wy);
+        row_weights<W>(tx, lx, esk, wx);
+        row_weights<W>(ty, ly, esk, wy);
         float acc = 0.f;
         if (use_lds) {
             const int boff = (lx - mnx) * eyzp + (ly - mny) * ezp + (lz - mnz);
             const WideRow<W, CMTIP_SLICE_READS> wr(boff);
-            int zo[W];
-            wr.load_offsets(zo);
             float wl[W];
-            weights_at<W>(tz, zo, beta, wl);
+            weights_at<W>(tz, wr, esk, wl);
             const float* base = box + boff;
 #pragma unroll
             for (int a = 0; a < W; ++a) {
@@ -1120,7 +1176,7 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
                 acc = fmaf(wx[a], pa, acc);
             }
         } else {
-            acc = direct_gather<W>(g, n, tz, lx, ly, lz, wx, wy, beta);
+            acc = direct_gather<W>(g, n, tz, lx, ly, lz, wx, wy, esk);
         }
         val = acc;
         const int64_t j = o * P + p;
@@ -1265,7 +1321,7 @@ __global__ void tile_scatter_kernel(int64_t ntiles, const int* __restrict__ tile
 
 template <int W>
 __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
-    const float* __restrict__ g, int n, float beta, float hi, float lo, TileRot rot, int nbx,
+    const float* __restrict__ g, int n, const EsKernel esk, float hi, float lo, TileRot rot, int nbx,
     const int* __restrict__ tperm, const int4* __restrict__ chunks, int* __restrict__ ctrl, float* __restrict__ out,
     uint16_t* __restrict__ out_bf16, double* __restrict__ part) {
     extern __shared__ float4 sub4[];  // 16-byte aligned rows
@@ -1320,16 +1376,14 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
             const int mnl = __builtin_amdgcn_readfirstlane(wave_min_i(active ? min(ux, min(uy, uz)) : 0));
             const bool fits = staged && mnl >= 0 && mx + W <= kTileS;
             float wx[W], wy[W];
-            row_weights<W>(tx, lx, beta, wx);
-            row_weights<W>(ty, ly, beta, wy);
+            row_weights<W>(tx, lx, esk, wx);
+            row_weights<W>(ty, ly, esk, wy);
             float acc = 0.f;
             if (fits) {
                 const int boff = ux * SSP + uy * SP + uz;
                 const WideRow<W, CMTIP_INTERP_READS> wr(boff);
-                int zo[W];
-                wr.load_offsets(zo);
                 float wl[W];
-                weights_at<W>(tz, zo, beta, wl);
+                weights_at<W>(tz, wr, esk, wl);
                 const float* base = sub + boff;
 #pragma unroll
                 for (int a = 0; a < W; ++a) {
@@ -1339,7 +1393,7 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
                     acc = fmaf(wx[a], pa, acc);
                 }
             } else {
-                acc = direct_gather<W>(g, n, tz, lx, ly, lz, wx, wy, beta);
+                acc = direct_gather<W>(g, n, tz, lx, ly, lz, wx, wy, esk);
             }
             if (active) {
                 const int64_t j = o * rot.P + p;
@@ -1419,7 +1473,7 @@ struct InterpLaunch {
             attr_set = true;
         }
         hipLaunchKernelGGL((interp_binned_kernel<W, Sorted>), dim3(grid), dim3(kSpreadBlock), lds, st,
-                           p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx, src, out,
+                           p->fine.as<const float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx, src, out,
                            ostride, p->chunks.as<const int4>(), p->ctrl.as<int>());
     }
     // real interpolation of the fine grid into out[j * ostride]; rot: the slicing
@@ -1462,13 +1516,13 @@ struct SpreadLaunch {
         }
         if (cplx)
             hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadThreads), lds, st,
-                               p->fine.as<float2>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->fine.as<float2>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
                                p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
                                p->perm.as<const int>(), reinterpret_cast<const float2*>(c), cmax, ch,
                                p->ctrl.as<int>());
         else
             hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
-                               p->fine.as<float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
                                p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
                                p->perm.as<const int>(), reinterpret_cast<const float*>(c), cmax, ch,
                                p->ctrl.as<int>());
@@ -1624,7 +1678,7 @@ struct TileLaunch {
         const unsigned grid = (unsigned)(cu_count() * std::max<size_t>(1, kMaxLds / (lds + kStaticLds)));
         if (timed) CMTIP_TRY(timer_mark(p, 0, st));
         hipLaunchKernelGGL(slice_tiles_kernel<W>, dim3(grid), dim3(kTileBlock), lds, st, p->fine.as<const float>(),
-                           p->n, p->beta_l2e, p->tc_hi, p->tc_lo, rot, nbx, p->perm.as<const int>(),
+                           p->n, p->esk, p->tc_hi, p->tc_lo, rot, nbx, p->perm.as<const int>(),
                            p->chunks.as<const int4>(), p->ctrl.as<int>(), out, out_bf16, part);
         CMTIP_LAUNCH_CHECK();
         if (timed) CMTIP_TRY(timer_mark(p, 1, st));
@@ -1644,7 +1698,7 @@ struct SliceLaunch {
             return CMTIP_ERR_ARG;
         }
         hipLaunchKernelGGL((slice_real_kernel<W>), dim3((unsigned)blocks), dim3(kSliceBlock), 0, st,
-                           p->fine.as<const float>(), p->n, p->beta_l2e, p->tc_hi, p->tc_lo,
+                           p->fine.as<const float>(), p->n, p->esk, p->tc_hi, p->tc_lo,
                            p->rinv.as<const double>(), s, order, P, tpo, tiles, ext, out, out_bf16, moments,
                            fit_count);
         CMTIP_LAUNCH_CHECK();
@@ -1851,6 +1905,8 @@ extern "C" int cmtip_nufft_makeplan(int type, int n_modes, int iflag, double eps
     p->w = es.w;
     p->beta = es.beta;
     p->beta_l2e = (float)(es.beta / M_LN2);
+    p->esk.beta_l2e = p->beta_l2e;
+    es_poly(p->w, p->beta, p->esk.c);
     p->n = fine_size(n_modes, es.w);
     if ((int64_t)p->n * p->n * p->n >= ((int64_t)1 << 31)) {
         // kernels index the fine grid with 32-bit offsets
