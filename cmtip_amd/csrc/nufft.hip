@@ -1235,18 +1235,6 @@ struct TileRot {
     }
 };
 
-// Low corner (unwrapped first nodes) and footprint extents of the wave's tile.
-template <int W>
-__device__ __forceinline__ void tile_box(bool active, int lx, int ly, int lz, int* mn, int* ex) {
-    const int big = 1 << 29;
-    mn[0] = __builtin_amdgcn_readfirstlane(wave_min_i(active ? lx : big));
-    mn[1] = __builtin_amdgcn_readfirstlane(wave_min_i(active ? ly : big));
-    mn[2] = __builtin_amdgcn_readfirstlane(wave_min_i(active ? lz : big));
-    ex[0] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lx : -big) + W - mn[0]);
-    ex[1] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? ly : -big) + W - mn[1]);
-    ex[2] = __builtin_amdgcn_readfirstlane(wave_max_i(active ? lz : -big) + W - mn[2]);
-}
-
 // Bin key of tile t (see above); nbins = the overflow bin.
 template <int W>
 __device__ __forceinline__ int tile_key(const TileRot& rot, int64_t t, int n, float hi, float lo, int nbx) {
@@ -1360,21 +1348,18 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
             float tx, ty, tz;
             rot.coords(o, p, hi, lo, n, &tx, &ty, &tz);
             const int lx = first_node(tx, W), ly = first_node(ty, W), lz = first_node(tz, W);
-            int mn[3], ex[3];
-            tile_box<W>(active, lx, ly, lz, mn, ex);
-            // local first nodes in the subgrid: the exact low corner's offset from the bin
-            // origin, taken mod n (the keyed corner may lie one period away); a tile that
-            // does not fit is read from the fine grid instead
-            int d0 = wrap_n(mn[0], n) - ox, d1 = wrap_n(mn[1], n) - oy, d2 = wrap_n(mn[2], n) - oz;
-            d0 += d0 < 0 ? n : 0;
-            d1 += d1 < 0 ? n : 0;
-            d2 += d2 < 0 ? n : 0;
-            const int ux = active ? lx - mn[0] + d0 : 0;
-            const int uy = active ? ly - mn[1] + d1 : 0;
-            const int uz = active ? lz - mn[2] + d2 : 0;
-            const int mx = __builtin_amdgcn_readfirstlane(wave_max_i(active ? max(ux, max(uy, uz)) : 0));
-            const int mnl = __builtin_amdgcn_readfirstlane(wave_min_i(active ? min(ux, min(uy, uz)) : 0));
-            const bool fits = staged && mnl >= 0 && mx + W <= kTileS;
+            // local first nodes in the subgrid, anchored at lane 0 (the tile's first pixel,
+            // always present): its offset from the bin origin taken mod n on the scalar
+            // unit, the other lanes by their distance to it.  A tile with any footprint
+            // outside the subgrid is read from the fine grid instead.
+            const int rx = __builtin_amdgcn_readfirstlane(lx), ry = __builtin_amdgcn_readfirstlane(ly);
+            const int rz = __builtin_amdgcn_readfirstlane(lz);
+            const int d0 = wrap_n(rx - ox, n), d1 = wrap_n(ry - oy, n), d2 = wrap_n(rz - oz, n);
+            const int vx = lx - rx + d0, vy = ly - ry + d1, vz = lz - rz + d2;
+            const bool lane_fits = vx >= 0 && vy >= 0 && vz >= 0 && vx + W <= kTileS && vy + W <= kTileS &&
+                                   vz + W <= kTileS;
+            const bool fits = staged && __ballot(active && !lane_fits) == 0;
+            const int ux = active ? vx : 0, uy = active ? vy : 0, uz = active ? vz : 0;
             float wx[W], wy[W];
             row_weights<W>(tx, lx, esk, wx);
             row_weights<W>(ty, ly, esk, wy);
