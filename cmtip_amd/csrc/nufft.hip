@@ -1008,8 +1008,12 @@ __host__ __device__ inline void tile_pitch(int* SP, int* SSP) {
     *SP = (kTileS + A - 1) / A * A + CMTIP_TILE_SP_PAD;
     *SSP = kTileS * *SP + CMTIP_TILE_SSP_PAD;
 }
-constexpr int kTileChunk = 128;               // tiles per unit of work
+#ifndef CMTIP_TILE_CHUNK
+#define CMTIP_TILE_CHUNK 128
+#endif
+constexpr int kTileChunk = CMTIP_TILE_CHUNK;  // tiles per unit of work
 constexpr int kTileBlock = 512;               // eight waves: one tile each at a time
+constexpr int kStageRows = kTileBlock / kTileS;  // subgrid rows staged per pass
 constexpr int kBoxPlanes = 4;   // x planes loaded per step (loads in flight per lane)
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -1316,7 +1320,6 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
     float* sub = reinterpret_cast<float*>(sub4);
     int SP, SSP;
     tile_pitch<W>(&SP, &SSP);
-    constexpr int S3 = kTileS * kTileS * kTileS;
     __shared__ int chunk_id;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1335,10 +1338,27 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
             ox = (ch.x / (nbx * nbx)) * kTileBs;
             oy = ((ch.x / nbx) % nbx) * kTileBs;
             oz = (ch.x % nbx) * kTileBs;
-            for (int i = threadIdx.x; i < S3; i += kTileBlock) {
-                const int ix = i / (kTileS * kTileS), iy = (i / kTileS) % kTileS, iz = i % kTileS;
-                const int gx = (ox + ix) % n, gy = (oy + iy) % n, gz = (oz + iz) % n;  // any n
-                sub[ix * SSP + iy * SP + iz] = g[(uint32_t)((gx * n + gy) * n + gz)];
+            // thread (iz, j) copies z-cell iz of the rows j, j + kStageRows, ... of the
+            // kTileS^2 (x, y) rows; indices wrap mod n (one subtraction when n >= kTileS)
+            const int iz = threadIdx.x % kTileS, j = threadIdx.x / kTileS;
+            if (j < kStageRows) {
+                const int gz = (oz + iz) % n;
+                if (n >= kTileS) {
+#pragma unroll 4
+                    for (int r = j; r < kTileS * kTileS; r += kStageRows) {
+                        const int ix = r / kTileS, iy = r - ix * kTileS;
+                        int gx = ox + ix, gy = oy + iy;
+                        gx -= gx >= n ? n : 0;
+                        gy -= gy >= n ? n : 0;
+                        sub[ix * SSP + iy * SP + iz] = g[(uint32_t)((gx * n + gy) * n + gz)];
+                    }
+                } else {
+                    for (int r = j; r < kTileS * kTileS; r += kStageRows) {
+                        const int ix = r / kTileS, iy = r - ix * kTileS;
+                        const int gx = (ox + ix) % n, gy = (oy + iy) % n;
+                        sub[ix * SSP + iy * SP + iz] = g[(uint32_t)((gx * n + gy) * n + gz)];
+                    }
+                }
             }
         }
         __syncthreads();
