@@ -59,9 +59,10 @@ def parse():
     p.add_argument("--nbatches", type=int, default=1)
     p.add_argument("--no-extras", action="store_true", help="time the step only")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic", "solve"],
+    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic", "solve", "c4", "c5"],
                    help="slice: one slicing pass of the workload (for counter runs); nufft: setpts / execute "
-                        "timing of both transforms; traffic: a 1 GiB copy (counter calibration) then one slicing pass")
+                        "timing of both transforms; traffic: a 1 GiB copy (counter calibration) then one slicing "
+                        "pass; c4 / c5: matching throughput at the larger workloads (see large_main)")
     return p.parse_args()
 
 
@@ -108,6 +109,8 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     M, atoms, side, n_img, n_ref = CONFIGS[args.config]
+    if args.part in ("c4", "c5"):
+        return large_main(args, dev)
     if args.part != "all":
         return part_main(args, M, atoms, side, n_img, n_ref, dev)
     P = side * side
@@ -392,6 +395,68 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         res[label] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": N / te / 1e6}
         log(f"[bench] {label}: setpts {ts * 1e3:.2f} ms, execute {te * 1e3:.2f} ms, {N / te / 1e6:.0f} Mpts/s")
     print(json.dumps({"part": "nufft", "M": M, **res}), flush=True)
+
+
+def large_main(args, dev):
+    """Whole match_orientations calls at the larger workloads, BF16 cross term with top-8
+    FP32 rescoring; one warm-up call, then timed calls.  c4: M = 81 (1000 atoms), 20000
+    images x 50000 references x 128 x 128 pixels, Poisson mean 5, log1p-scaled.  c5:
+    M = 151 (5000 atoms), 100000 images x 100000 references with a random circular
+    beamstop mask per image, at 128 x 128 and at 96 x 96 pixels, Poisson mean 10."""
+    import torch
+
+    from cmtip_amd import alignment, synth
+
+    if args.part == "c4":
+        cases, steps = [(81, 1000, 128, 20000, 50000, False)], max(1, args.steps)
+    else:
+        cases, steps = [(151, 5000, 128, 100000, 100000, True), (151, 5000, 96, 100000, 100000, True)], 1
+    out = {"part": args.part, "precision": "bf16", "steps": steps, "cases": []}
+    for M, atoms, side, n_img, n_ref, masked in cases:
+        P = side * side
+        log(f"[bench] {args.part}: M={M} side={side} images={n_img} references={n_ref} masked={masked}")
+        s, ext = synth.detector(side)
+        ac = torch.from_numpy(synth.gaussian_ac(M, atoms, seed=0)).to(dev)
+        ref = synth.random_quaternions(n_ref, seed=0)
+        q_img = torch.from_numpy(synth.random_quaternions(n_img, seed=1)).to(dev)
+        clean = alignment.Slicer(ac, s, ext).slice(q_img).clamp_(min=0)
+        torch.manual_seed(2)
+        masks = None
+        if masked:
+            imgs = torch.poisson(clean * (10.0 / float(clean.mean())))
+            g = torch.Generator(device=dev).manual_seed(side)
+            cy = torch.randint(side // 3, 2 * side // 3, (n_img, 1), device=dev, generator=g)
+            cx = torch.randint(side // 3, 2 * side // 3, (n_img, 1), device=dev, generator=g)
+            r = torch.rand((n_img, 1), device=dev, generator=g) * (side / 5 - 2) + 2
+            pix = torch.arange(P, device=dev)
+            masks = (((pix // side)[None, :] - cy) ** 2 + ((pix % side)[None, :] - cx) ** 2 > r * r).to(torch.uint8)
+            del cy, cx, r, pix
+        else:
+            imgs = torch.log1p(torch.poisson(clean * (5.0 / float(clean.mean()))))
+        del clean
+        torch.cuda.synchronize()
+
+        def call():
+            return alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref, precision="bf16",
+                                                masks=masks)
+        call()
+        torch.cuda.synchronize()
+        log(f"[bench] {args.part}: warm-up call done")
+        t = time.perf_counter()
+        for _ in range(steps):
+            call()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t) / steps
+        K = 3 * P if masked else P
+        case = {"M": M, "pixels": P, "images": n_img, "references": n_ref, "masked": masked,
+                "ms_per_call": el * 1e3, "images_per_s": n_img / el,
+                "cross_term_tflop": 2.0 * n_img * n_ref * K / 1e12,
+                "whole_call_tflop_per_s": 2.0 * n_img * n_ref * K / el / 1e12}
+        log(f"[bench] {args.part}: {case}")
+        out["cases"].append(case)
+        del imgs, masks
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(ac, s, ext, ref, imgs, n_img, n_ref, P):
