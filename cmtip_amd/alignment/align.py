@@ -90,6 +90,31 @@ def _geometry(pixel_position_reciprocal, tile_pixels):
     return geo
 
 
+_WEIGHTS2 = collections.OrderedDict()
+
+
+def _weights2(pixel_position_reciprocal, order):
+    """Squared pixel weights w^2 (P,) float32 on the device (generate_weights squared).
+    Host arrays are cached by content and order like _geometry, so repeated calls make
+    no host-to-device copy and no reduction for them."""
+    key = None
+    if not isinstance(pixel_position_reciprocal, torch.Tensor):
+        a = np.ascontiguousarray(pixel_position_reciprocal)
+        key = (a.shape, a.dtype.str, hashlib.blake2b(a.tobytes(), digest_size=16).digest(), float(order),
+               torch.cuda.current_device())
+        hit = _WEIGHTS2.get(key)
+        if hit is not None:
+            _WEIGHTS2.move_to_end(key)
+            return hit
+    w = generate_weights(_arrays.to_dev(pixel_position_reciprocal, torch.float32), order=order).reshape(-1)
+    w2 = (w * w).contiguous()
+    if key is not None:
+        _WEIGHTS2[key] = w2
+        while len(_WEIGHTS2) > 4:
+            _WEIGHTS2.popitem(last=False)
+    return w2
+
+
 def _ext32(reciprocal_extent):
     if isinstance(reciprocal_extent, torch.Tensor):
         reciprocal_extent = reciprocal_extent.item()
@@ -167,11 +192,6 @@ def compute_slices(orientations, pixel_position_reciprocal, reciprocal_extent, a
     return _arrays.out(out.reshape(-1), as_torch)
 
 
-def _std_from_moments(mom, count):
-    mean = mom[0] / count
-    return torch.sqrt(torch.clamp(mom[1] / count - mean * mean, min=0.0))
-
-
 def reference_orientations(n_ref_orientations, seed=0):
     return synth.random_quaternions(n_ref_orientations, seed)
 
@@ -193,8 +213,7 @@ class _Matcher:
         self.B = B
         self.n_img = B.shape[0]
         L = _lib.lib()
-        w = generate_weights(_arrays.to_dev(pixel_position_reciprocal, torch.float32), order=order)
-        self.w2 = (w.reshape(-1) * w.reshape(-1)).contiguous()
+        self.w2 = _weights2(pixel_position_reciprocal, order)
         self.precision = precision
         self.k = int(topk)
         dev = B.device
@@ -207,9 +226,7 @@ class _Matcher:
             mom = torch.empty(3, dtype=torch.float64, device=dev)
             check(L.cmtip_image_stats_masked(ptr(B), ptr(self.mask), self.n_img, self.P, ptr(colsum), ptr(self.c),
                                              ptr(mom), stream()), "cmtip_image_stats_masked")
-            count = mom[2].clamp(min=1.0)
-            mean = mom[0] / count
-            self.std_data = torch.sqrt(torch.clamp(mom[1] / count - mean * mean, min=0.0))
+            self.img_mom, self.img_count = mom, 0.0  # the valid-pixel count is mom[2]
             # float32 cross term over [y^2, y]; bfloat16 over [hi(y^2), lo(y^2), y]
             self.K = (2 if precision == "fp32" else 3) * self.P
             self.Bw = torch.empty((self.n_img, self.K), dtype=torch.float32, device=dev) \
@@ -232,7 +249,7 @@ class _Matcher:
               "cmtip_match_prepare_images_centered")
         check(L.cmtip_center_rows(ptr(B), self.n_img, self.P, None, ptr(self.c), ptr(self.w2), None, None,
                                   ptr(self.bn), stream()), "cmtip_center_rows")
-        self.std_data = _std_from_moments(mom, B.numel())
+        self.img_mom, self.img_count = mom, float(B.numel())
 
     def run(self, slicer, ref, nbatches, verbose=False):
         """Per-image best key over all references (int64 holding the unsigned key)."""
@@ -252,17 +269,19 @@ class _Matcher:
             Dt = None
         elif masked:
             A2 = torch.empty((quot, self.K), dtype=torch.float32, device=dev)
+        gamma_one = torch.empty(1, dtype=torch.float64, device=dev)
         for nb in range(nbatches):
             start, end = nb * quot, (nb + 1) * quot
             mom = torch.zeros(2, dtype=torch.float64, device=dev)
             Ab = A_all[start:end] if self.precision == "bf16" else A_all
             slicer.slice(ref[start:end], out=Ab, moments=mom)
             # data/model std ratio (align.py:122-124), rounded to float32 as numpy keeps it
-            gamma = (self.std_data / _std_from_moments(mom, Ab.numel())).float().double().reshape(1)
+            gamma = gamma_ref[start:end] if self.precision == "bf16" else gamma_one
+            check(L.cmtip_match_gamma(ptr(self.img_mom), self.img_count, ptr(mom), float(Ab.numel()),
+                                      gamma.numel(), ptr(gamma), stream()), "cmtip_match_gamma")
             if verbose:
-                print(f"Data/Model std ratio: {float(gamma):.7g}.", flush=True)
+                print(f"Data/Model std ratio: {float(gamma[0]):.7g}.", flush=True)
             if self.precision == "bf16":
-                gamma_ref[start:end] = gamma
                 if masked:
                     check(L.cmtip_center_rows_sq(ptr(Ab), quot, self.P, ptr(gamma), ptr(self.c), None, ptr(A16),
                                                  stream()), "cmtip_center_rows_sq")

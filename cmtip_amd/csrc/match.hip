@@ -268,6 +268,24 @@ __global__ void __launch_bounds__(kThreads) center_rows4_kernel(const float* X, 
     }
 }
 
+// gamma[0..n) = float32(std_img / std_model); one thread computes, the block writes
+__global__ void match_gamma_kernel(const double* __restrict__ img_mom, double img_count,
+                                   const double* __restrict__ model_mom, double model_count, int64_t n,
+                                   double* __restrict__ gamma) {
+#pragma clang fp contract(off)
+    __shared__ double g;
+    if (threadIdx.x == 0) {
+        const double ci = img_count > 0.0 ? img_count : fmax(img_mom[2], 1.0);
+        const double mi = img_mom[0] / ci;
+        const double si = sqrt(fmax(img_mom[1] / ci - mi * mi, 0.0));
+        const double mm = model_mom[0] / model_count;
+        const double sm = sqrt(fmax(model_mom[1] / model_count - mm * mm, 0.0));
+        g = (double)(float)(si / sm);
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += kThreads) gamma[i] = g;
+}
+
 __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t count, unsigned long long v) {
     const int64_t stride = (int64_t)gridDim.x * kThreads;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) p[i] = v;
@@ -882,6 +900,18 @@ extern "C" int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double
     hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64 * kStatWaves), 0, st, X, rows, P,
                        colsum);
     hipLaunchKernelGGL(finish_stats_kernel, dim3(1), dim3(kThreads), 0, st, (const double*)colsum, rows, P, c, mom);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_gamma(const double* img_mom, double img_count, const double* model_mom,
+                                 double model_count, int64_t n, double* gamma, void* stream) {
+    if (!img_mom || !model_mom || !(model_count > 0.0) || n < 1 || !gamma) {
+        set_error("match_gamma: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    hipLaunchKernelGGL(match_gamma_kernel, dim3(1), dim3(kThreads), 0, as_stream(stream), img_mom, img_count,
+                       model_mom, model_count, n, gamma);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

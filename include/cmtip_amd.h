@@ -195,6 +195,14 @@ int cmtip_slice_timer(cmtip_nufft_plan plan, int start, float* total_ms, int* co
  * FP64 with a fixed summation order. */
 int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double* colsum, float* c, double* mom,
                       void* stream);
+/* Data/model standard-deviation ratio of one reference batch (align.py:122-124: both
+ * population stds, the ratio rounded to float32 as numpy keeps it), written to the n
+ * doubles of gamma.  img_mom = (sum, sum of squares[, count]) of the images with
+ * img_count entries, or with the count in img_mom[2] (at least 1) when img_count <= 0
+ * (masked images); model_mom = (sum, sum of squares) of model_count slice entries.
+ * FP64 in the order sqrt(max(s2 / n - (s1 / n)^2, 0)), without contraction. */
+int cmtip_match_gamma(const double* img_mom, double img_count, const double* model_mom, double model_count,
+                      int64_t n, double* gamma, void* stream);
 /* y = gamma x - c per row (c may be NULL), norms = sum_p w2 y^2; y written to Y (float32,
  * may alias X) and/or Y16 (bfloat16 bits) when given. */
 int cmtip_center_rows(const float* X, int64_t rows, int64_t P, const double* gamma, const float* c,
