@@ -54,6 +54,7 @@ SIGNATURES = {
     "cmtip_match_gamma": [_vp, ctypes.c_double, _vp, ctypes.c_double, _i64, _vp, _vp],
     "cmtip_center_rows": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cmtip_match_prepare_images_centered": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "cmtip_match_prepare_images_norms": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "cmtip_fill_u64": [_vp, _i64, _u64, _vp],
     "cmtip_match_argmin_f32": [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_eudist_f32": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp],

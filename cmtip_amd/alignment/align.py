@@ -244,11 +244,9 @@ class _Matcher:
         self.K = self.P
         self.Bw = torch.empty_like(B) if precision == "fp32" else None
         self.Bw16 = torch.empty(B.shape, dtype=torch.int16, device=dev) if precision == "bf16" else None
-        check(L.cmtip_match_prepare_images_centered(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.c),
-                                                    ptr(self.Bw), ptr(self.Bw16), stream()),
-              "cmtip_match_prepare_images_centered")
-        check(L.cmtip_center_rows(ptr(B), self.n_img, self.P, None, ptr(self.c), ptr(self.w2), None, None,
-                                  ptr(self.bn), stream()), "cmtip_center_rows")
+        check(L.cmtip_match_prepare_images_norms(ptr(B), self.n_img, self.P, ptr(self.w2), ptr(self.c),
+                                                 ptr(self.Bw), ptr(self.Bw16), ptr(self.bn), stream()),
+              "cmtip_match_prepare_images_norms")
         self.img_mom, self.img_count = mom, float(B.numel())
 
     def run(self, slicer, ref, nbatches, verbose=False):

@@ -107,31 +107,44 @@ __global__ void prepare_images_kernel(const float* __restrict__ B, int64_t total
 }
 
 // The same per row (one workgroup per image) for P a multiple of 4 and 16-byte aligned
-// rows: float4 loads, one 8-byte store of four bfloat16 values.
+// rows: float4 loads, one 8-byte store of four bfloat16 values.  With norms, also
+// norms[row] = sum w2 (b - c)^2 in FP64, the same values and summation order as
+// center_rows4_kernel with gamma = 1 (one pass over the images instead of two).
 __global__ void __launch_bounds__(kThreads) prepare_images4_kernel(const float* __restrict__ B, int64_t P,
                                                                    const float* __restrict__ w2,
                                                                    const float* __restrict__ c,
                                                                    float* __restrict__ Bw,
-                                                                   uint16_t* __restrict__ Bw_bf16) {
+                                                                   uint16_t* __restrict__ Bw_bf16,
+                                                                   float* __restrict__ norms) {
+    __shared__ double red[kThreads / 64];
     const int64_t base = (int64_t)blockIdx.x * P, P4 = P >> 2;
     const float4* B4 = reinterpret_cast<const float4*>(B + base);
     const float4* w4 = reinterpret_cast<const float4*>(w2);
     const float4* c4 = reinterpret_cast<const float4*>(c);
+    double s = 0.0;
 #pragma unroll 4
     for (int64_t j = threadIdx.x; j < P4; j += kThreads) {
         const float4 b = B4[j], w = w4[j];
-        float4 v;
-        if (c) {
-            const float4 cc = c4[j];
-            v = make_float4(w.x * (b.x - cc.x), w.y * (b.y - cc.y), w.z * (b.z - cc.z), w.w * (b.w - cc.w));
-        } else {
-            v = make_float4(w.x * b.x, w.y * b.y, w.z * b.z, w.w * b.w);
-        }
+        const float4 cc = c ? c4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 y = make_float4(b.x - cc.x, b.y - cc.y, b.z - cc.z, b.w - cc.w);
+        const float4 v = make_float4(w.x * y.x, w.y * y.y, w.z * y.z, w.w * y.w);
+        if (norms)
+            s += ((double)w.x * (double)y.x * (double)y.x + (double)w.y * (double)y.y * (double)y.y) +
+                 ((double)w.z * (double)y.z * (double)y.z + (double)w.w * (double)y.w * (double)y.w);
         if (Bw) reinterpret_cast<float4*>(Bw + base)[j] = v;
         if (Bw_bf16)
             reinterpret_cast<uint2*>(Bw_bf16 + base)[j] =
                 make_uint2((uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16),
                            (uint32_t)f32_to_bf16(v.z) | ((uint32_t)f32_to_bf16(v.w) << 16));
+    }
+    if (!norms) return;  // uniform over the workgroup
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+        norms[blockIdx.x] = (float)t;
     }
 }
 
@@ -176,12 +189,15 @@ __global__ void __launch_bounds__(64 * kStatWaves) column_sums_kernel(const floa
 }
 
 // c = colsum / rows (float32) and mom = (sum of all, sum of squares) in a fixed order
-__global__ void finish_stats_kernel(const double* __restrict__ colsum, int64_t rows, int64_t P,
-                                    float* __restrict__ c, double* __restrict__ mom) {
-    __shared__ double red[2][kThreads];
+constexpr int kFinishThreads = 1024;
+
+__global__ void __launch_bounds__(kFinishThreads) finish_stats_kernel(const double* __restrict__ colsum,
+                                                                      int64_t rows, int64_t P, float* __restrict__ c,
+                                                                      double* __restrict__ mom) {
+    __shared__ double red[2][kFinishThreads];
     double s1 = 0.0, s2 = 0.0;
     const double inv = 1.0 / (double)(rows > 0 ? rows : 1);
-    for (int64_t p = threadIdx.x; p < P; p += kThreads) {
+    for (int64_t p = threadIdx.x; p < P; p += kFinishThreads) {
         const double a = colsum[p];
         s1 += a;
         s2 += colsum[P + p];
@@ -190,7 +206,7 @@ __global__ void finish_stats_kernel(const double* __restrict__ colsum, int64_t r
     red[0][threadIdx.x] = s1;
     red[1][threadIdx.x] = s2;
     __syncthreads();
-    for (int h = kThreads / 2; h > 0; h >>= 1) {
+    for (int h = kFinishThreads / 2; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) {
             red[0][threadIdx.x] += red[0][threadIdx.x + h];
             red[1][threadIdx.x] += red[1][threadIdx.x + h];
@@ -899,7 +915,7 @@ extern "C" int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64 * kStatWaves), 0, st, X, rows, P,
                        colsum);
-    hipLaunchKernelGGL(finish_stats_kernel, dim3(1), dim3(kThreads), 0, st, (const double*)colsum, rows, P, c, mom);
+    hipLaunchKernelGGL(finish_stats_kernel, dim3(1), dim3(kFinishThreads), 0, st, (const double*)colsum, rows, P, c, mom);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -951,12 +967,32 @@ extern "C" int cmtip_match_prepare_images_centered(const float* B, int64_t n_img
     if (P % 4 == 0 && n_img <= 0x7fffffffLL && aligned16(B) && aligned16(w2) && aligned16(c) && aligned16(Bw) &&
         (reinterpret_cast<uintptr_t>(Bw_bf16) & 7) == 0)
         hipLaunchKernelGGL(prepare_images4_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), B, P,
-                           w2, c, Bw, Bw_bf16);
+                           w2, c, Bw, Bw_bf16, (float*)nullptr);
     else
         hipLaunchKernelGGL(prepare_images_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)),
                            dim3(kThreads), 0, as_stream(stream), B, total, P, w2, c, Bw, Bw_bf16);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
+}
+
+extern "C" int cmtip_match_prepare_images_norms(const float* B, int64_t n_img, int64_t P, const float* w2,
+                                                const float* c, float* Bw, uint16_t* Bw_bf16, float* norms,
+                                                void* stream) {
+    if (n_img < 0 || P < 1 || (n_img > 0 && (!B || !w2 || !norms || (!Bw && !Bw_bf16)))) {
+        set_error("match_prepare_images_norms: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_img == 0) return CMTIP_OK;
+    if (P % 4 == 0 && n_img <= 0x7fffffffLL && aligned16(B) && aligned16(w2) && aligned16(c) && aligned16(Bw) &&
+        (reinterpret_cast<uintptr_t>(Bw_bf16) & 7) == 0) {
+        hipLaunchKernelGGL(prepare_images4_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), B, P,
+                           w2, c, Bw, Bw_bf16, norms);
+        CMTIP_LAUNCH_CHECK();
+        return CMTIP_OK;
+    }
+    const int rc = cmtip_match_prepare_images_centered(B, n_img, P, w2, c, Bw, Bw_bf16, stream);
+    if (rc != CMTIP_OK) return rc;
+    return cmtip_center_rows(B, n_img, P, nullptr, c, w2, nullptr, nullptr, norms, stream);
 }
 
 extern "C" int cmtip_fill_u64(unsigned long long* dst, int64_t count, unsigned long long value, void* stream) {
@@ -1191,14 +1227,16 @@ extern "C" int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const ui
                 hipEvent_t e0 = nullptr, e1 = nullptr;
                 float best_ms = 0.f;
                 bool have = false;
-                // warm-up launch (code loading), then one timed launch per candidate
-                (void)hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc, &heur[0].algo,
-                                      bl.workspace, bl.ws_size, as_stream(stream));
+                // per candidate: an untimed launch (its first launch loads its code), then a
+                // timed one
                 if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
                     for (int a = 0; a < found; ++a) {
                         if (heur[a].state != HIPBLAS_STATUS_SUCCESS || heur[a].workspaceSize > bl.ws_size) continue;
                         float ms = 0.f;
-                        if (hipEventRecord(e0, as_stream(stream)) != hipSuccess ||
+                        if (hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc,
+                                            &heur[a].algo, bl.workspace, bl.ws_size,
+                                            as_stream(stream)) != HIPBLAS_STATUS_SUCCESS ||
+                            hipEventRecord(e0, as_stream(stream)) != hipSuccess ||
                             hipblasLtMatmul(bl.handle, desc, &alpha, A, la, B, lb, &beta, Dt, lc, Dt, lc,
                                             &heur[a].algo, bl.workspace, bl.ws_size,
                                             as_stream(stream)) != HIPBLAS_STATUS_SUCCESS ||

@@ -282,48 +282,65 @@ __global__ void bin_count_kernel(Src pts, int64_t N, int n, int W, int bs, int n
 
 // One workgroup: offset = exclusive scan of count, nchunk = exclusive scan of
 // ceil(count / chunk); ctrl[0] = total chunks.  Counts are then cleared for use as
-// scatter cursors.
+// scatter cursors.  Each thread sums a contiguous segment of the bins, one shuffle scan
+// over the 1024 segment sums gives each segment its start, and the thread writes its
+// segment's prefixes (integer sums: the result does not depend on the split).
 __global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count, int64_t nbins, int chunk,
                                                         int* __restrict__ offset, int* __restrict__ nchunk,
                                                         int* __restrict__ ctrl) {
-    __shared__ int sa[1024], sb[1024];
-    __shared__ int carry_a, carry_b;
-    if (threadIdx.x == 0) {
-        carry_a = 0;
-        carry_b = 0;
+    __shared__ int wa[16], wb[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t per = (nbins + 1023) / 1024;
+    const int64_t b0 = min(nbins, (int64_t)t * per), b1 = min(nbins, b0 + per);
+    int sa = 0, sb = 0;
+#pragma unroll 8
+    for (int64_t i = b0; i < b1; ++i) {
+        const int c = count[i];
+        sa += c;
+        sb += (c + chunk - 1) / chunk;
+    }
+    int ia = sa, ib = sb;  // inclusive scan over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const int ta = __shfl_up(ia, off, 64), tb = __shfl_up(ib, off, 64);
+        if (lane >= off) {
+            ia += ta;
+            ib += tb;
+        }
+    }
+    if (lane == 63) {
+        wa[wave] = ia;
+        wb[wave] = ib;
     }
     __syncthreads();
-    for (int64_t base = 0; base < nbins; base += 1024) {
-        const int64_t i = base + threadIdx.x;
-        const int c = i < nbins ? count[i] : 0;
-        int va = c, vb = (c + chunk - 1) / chunk;
-        sa[threadIdx.x] = va;
-        sb[threadIdx.x] = vb;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int ta = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0;
-            const int tb = threadIdx.x >= (unsigned)off ? sb[threadIdx.x - off] : 0;
-            __syncthreads();
-            sa[threadIdx.x] += ta;
-            sb[threadIdx.x] += tb;
-            __syncthreads();
+    if (wave == 0) {  // inclusive scan of the 16 wave totals
+        int xa = lane < 16 ? wa[lane] : 0, xb = lane < 16 ? wb[lane] : 0;
+        for (int off = 1; off < 16; off <<= 1) {
+            const int ta = __shfl_up(xa, off, 64), tb = __shfl_up(xb, off, 64);
+            if (lane >= off) {
+                xa += ta;
+                xb += tb;
+            }
         }
-        if (i < nbins) {
-            offset[i] = carry_a + sa[threadIdx.x] - va;
-            nchunk[i] = carry_b + sb[threadIdx.x] - vb;
-            count[i] = 0;
+        if (lane < 16) {
+            wa[lane] = xa;
+            wb[lane] = xb;
         }
-        __syncthreads();
-        if (threadIdx.x == 1023) {
-            carry_a += sa[1023];
-            carry_b += sb[1023];
-        }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        offset[nbins] = carry_a;
-        nchunk[nbins] = carry_b;
-        ctrl[0] = carry_b;
+    __syncthreads();
+    int ea = ia - sa + (wave ? wa[wave - 1] : 0), eb = ib - sb + (wave ? wb[wave - 1] : 0);
+#pragma unroll 8
+    for (int64_t i = b0; i < b1; ++i) {
+        const int c = count[i];
+        offset[i] = ea;
+        nchunk[i] = eb;
+        count[i] = 0;
+        ea += c;
+        eb += (c + chunk - 1) / chunk;
+    }
+    if (t == 1023) {  // its running prefix ends at the total whether or not its segment is empty
+        offset[nbins] = ea;
+        nchunk[nbins] = eb;
+        ctrl[0] = eb;
     }
 }
 
@@ -1639,7 +1656,8 @@ struct TileLaunch {
         const unsigned gb = (unsigned)((nbins + 1 + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), nbins + 1, 0);
         const bool local = nbins + 1 <= kLocalBinsMax;
-        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((ntiles + 8191) / 8192, 4 * cu_count()));
+        // about two tiles per thread: each tile's corner loads form a dependent chain
+        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((ntiles + 2047) / 2048, 4 * cu_count()));
         const int64_t per_block = (ntiles + blocks - 1) / blocks;
         const size_t hlds = sizeof(int) * (size_t)(nbins + 1);
         if (local) {

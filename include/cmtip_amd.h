@@ -210,6 +210,11 @@ int cmtip_center_rows(const float* X, int64_t rows, int64_t P, const double* gam
 /* Bw = w2 (.) (B - c) as float32 and/or bfloat16 (c may be NULL). */
 int cmtip_match_prepare_images_centered(const float* B, int64_t n_img, int64_t P, const float* w2,
                                         const float* c, float* Bw, uint16_t* Bw_bf16, void* stream);
+/* cmtip_match_prepare_images_centered and the image norms of cmtip_center_rows (gamma
+ * = 1: norms = sum w2 (b - c)^2) in one pass over the images when P is a multiple of 4
+ * and the rows are aligned (two passes otherwise); the same values either way. */
+int cmtip_match_prepare_images_norms(const float* B, int64_t n_img, int64_t P, const float* w2,
+                                     const float* c, float* Bw, uint16_t* Bw_bf16, float* norms, void* stream);
 int cmtip_weighted_norms(const float* X, int64_t rows, int64_t P, const float* w2, float* out,
                          void* stream);
 int cmtip_moments(const float* X, int64_t count, double* out2, void* stream);

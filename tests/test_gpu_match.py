@@ -258,6 +258,30 @@ def test_row_kernels_float4_and_scalar_forms(P):
     assert float(((n0.double() - n64).abs() / n64).max()) < 1e-6
 
 
+@pytest.mark.parametrize("P", [1024, 1023])
+def test_prepare_images_norms_one_pass(P):
+    # the one-pass image preparation gives the bfloat16 images and the norms of the two
+    # separate calls bit for bit (P = 1023 takes the two-call form inside)
+    from cmtip_amd._lib import check, lib, ptr, stream
+    d = dev()
+    g = torch.Generator(device=d).manual_seed(12)
+    n, L = 257, lib()
+    B = torch.rand((n, P), device=d, generator=g) * 30
+    c = torch.rand(P, device=d, generator=g) * 10
+    w2 = torch.rand(P, device=d, generator=g)
+    say(f"prepare_images_norms P={P}")
+    h1 = torch.empty((n, P), dtype=torch.int16, device=d)
+    n1 = torch.empty(n, device=d)
+    check(L.cmtip_match_prepare_images_norms(ptr(B), n, P, ptr(w2), ptr(c), None, ptr(h1), ptr(n1), stream()))
+    h2 = torch.empty_like(h1)
+    n2 = torch.empty_like(n1)
+    check(L.cmtip_match_prepare_images_centered(ptr(B), n, P, ptr(w2), ptr(c), None, ptr(h2), stream()))
+    check(L.cmtip_center_rows(ptr(B), n, P, None, ptr(c), ptr(w2), None, None, ptr(n2), stream()))
+    assert torch.equal(h1, h2) and torch.equal(n1, n2)
+    want = ((B.double() - c.double()) ** 2 * w2.double()).sum(1)
+    assert float(((n1.double() - want).abs() / want).max()) < 1e-6
+
+
 def test_cross_bf16_vs_fp32_product():
     from cmtip_amd._lib import check, lib, ptr, stream
     d = dev()
