@@ -207,6 +207,15 @@ def main():
                 "per_unit": "4 B per rotated pixel + 24 B per pixel + 16 B per orientation + fine grid"
                             + (" + 4 B per tile" if route == 3 else ""),
                 "avg_ms": k_ms, "slicing_call_ms": slice_ms}
+        if route == 3 and sl.plan.w == 7:
+            # the kernel's real limit: LDS reads, 49 rows of 7 taps per pixel read as
+            # b128 + b64 + b32 (28 B), against this GPU's conflict-free b128 read rate
+            lds_bytes = 49 * 28 * N
+            lds_peak = lds_read_ceiling(dev)
+            roof["lds"] = {"achieved": lds_bytes / (k_ms * 1e-3) / 1e12, "peak": lds_peak, "unit": "TB/s",
+                           "frac": lds_bytes / (k_ms * 1e-3) / 1e12 / lds_peak, "bytes_per_launch": lds_bytes,
+                           "per_unit": "49 x 28 B of LDS reads per rotated pixel",
+                           "peak_source": "cmtip_lds_read_probe: conflict-free ds_read_b128 on this GPU"}
     else:
         flops = 2.0 * n_img * n_ref * P
         peak = BF16_PEAK_TFS if args.precision == "bf16" else FP32_MFMA_PEAK_TFS
@@ -395,6 +404,31 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         res[label] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": N / te / 1e6}
         log(f"[bench] {label}: setpts {ts * 1e3:.2f} ms, execute {te * 1e3:.2f} ms, {N / te / 1e6:.0f} Mpts/s")
     print(json.dumps({"part": "nufft", "M": M, **res}), flush=True)
+
+
+def lds_read_ceiling(dev):
+    """TB/s of conflict-free ds_read_b128 over the whole GPU (cmtip_amd's probe kernel:
+    2 workgroups of 512 threads per CU, 64 KB of LDS each), best of 3 timed launches."""
+    import torch
+
+    from cmtip_amd import _lib
+    from cmtip_amd._lib import check, ptr, stream
+
+    L = _lib.lib()
+    wg = 2 * torch.cuda.get_device_properties(dev).multi_processor_count
+    iters = 20000
+    out = torch.empty(wg * 512, device=dev)
+    check(L.cmtip_lds_read_probe(wg, iters, ptr(out), stream()), "cmtip_lds_read_probe")
+    best = None
+    for _ in range(3):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        check(L.cmtip_lds_read_probe(wg, iters, ptr(out), stream()), "cmtip_lds_read_probe")
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        best = ms if best is None else min(best, ms)
+    return wg * 512 * 16.0 * iters / (best * 1e-3) / 1e12
 
 
 def large_main(args, dev):

@@ -331,6 +331,13 @@ int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* out, double a
 int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
                       double phi, int maxiter, double rtol, int* iters, int* info, void* stream);
 
+/* ---------------------------------------------------------------- measurement
+ * No counterpart in the reference.  workgroups x 512 threads each read a 64 KB LDS
+ * buffer iters times with conflict-free ds_read_b128 (16 B per lane per read) and
+ * write one float per thread to out (workgroups x 512 floats): the LDS read ceiling
+ * that the interpolation kernels are measured against, timed by the caller. */
+int cmtip_lds_read_probe(int workgroups, int iters, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

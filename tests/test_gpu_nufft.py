@@ -155,3 +155,18 @@ def test_type2_imag_output(cplx):
     out = torch.empty(N, dtype=torch.float32, device=dev())
     p.execute(out, torch.as_tensor(f, device=dev()), (0 if cplx else CMTIP_GRID_REAL) | CMTIP_PTS_IMAG)
     assert rel_l2(out, want) < TOL
+
+
+def test_lds_read_probe_values():
+    # the LDS ceiling probe reads what it wrote: thread t sums the float4 entries
+    # (t + 64 it) mod 4096, entry i holding (i, i + 1, i + 2, i + 3)
+    from cmtip_amd._lib import check, lib, ptr, stream
+    wg, iters = 3, 100
+    out = torch.empty(wg * 512, device=dev())
+    say("lds read probe")
+    check(lib().cmtip_lds_read_probe(wg, iters, ptr(out), stream()))
+    t = np.arange(512)
+    idx = (t[:, None] + 64 * np.arange(iters)[None, :]) & 4095
+    want = (4 * idx + 6).astype(np.float64).sum(1)
+    got = out.cpu().numpy().reshape(wg, 512)
+    np.testing.assert_allclose(got, np.broadcast_to(want, (wg, 512)), rtol=1e-6)
