@@ -27,18 +27,31 @@ class NufftPlan:
         self.iflag = int(iflag)
         self.eps = float(eps)
         self._pts = None
+        self._pts_key = None
         w, n, beta = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(L.cmtip_nufft_info(h, ctypes.byref(w), ctypes.byref(n), ctypes.byref(beta)), "cmtip_nufft_info")
         self.w, self.n_fine, self.beta = w.value, n.value, beta.value
 
     def setpts(self, x, y, z):
-        """x, y, z: float32 device tensors of equal length (kept alive by the plan)."""
+        """x, y, z: float32 device tensors of equal length (kept alive by the plan).
+
+        Registering the points this plan already holds is skipped: same storage (the plan
+        keeps the previous tensors alive, so an equal address is the same memory), same
+        length and the same version counter (no in-place write since)."""
         n = x.numel()
         assert y.numel() == n and z.numel() == n
         for t in (x, y, z):
             assert t.dtype == torch.float32 and t.is_cuda and t.is_contiguous()
+        try:
+            key = tuple((t.data_ptr(), t.numel(), t._version) for t in (x, y, z))
+        except RuntimeError:  # inference tensors carry no version counter
+            key = None
+        if key is not None and key == self._pts_key:
+            return
         self._pts = (x, y, z)
+        self._pts_key = None
         check(_lib.lib().cmtip_nufft_setpts(self._h, n, ptr(x), ptr(y), ptr(z), stream()), "cmtip_nufft_setpts")
+        self._pts_key = key
 
     def execute(self, c, f, flags, support=None, scale=1.0):
         check(_lib.lib().cmtip_nufft_execute(self._h, ptr(c), ptr(f), int(flags), ptr(support), float(scale),
@@ -47,6 +60,7 @@ class NufftPlan:
     def set_bins(self, bin_size=0, chunk=0, wg_per_cu=0):
         """Bin edge, points per chunk and workgroups per CU (0 = default), used from the
         next setpts on."""
+        self._pts_key = None  # the next setpts bins again with these settings
         check(_lib.lib().cmtip_nufft_set_bins(self._h, int(bin_size), int(chunk), int(wg_per_cu)),
               "cmtip_nufft_set_bins")
 
@@ -96,6 +110,7 @@ class NufftPlan:
         return r.value, u.value
 
     def slice_real(self, quats, s, P, recip_extent, out, out_bf16=None, moments=None, order=None):
+        self._pts_key = None  # the slicing routes reuse the plan's bin buffers
         n_orient = quats.shape[0]
         check(_lib.lib().cmtip_slice_real(self._h, ptr(quats), n_orient, ptr(s), ptr(order), int(P),
                                           float(recip_extent), ptr(out), ptr(out_bf16), ptr(moments), stream()),

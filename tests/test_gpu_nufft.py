@@ -170,3 +170,39 @@ def test_lds_read_probe_values():
     want = (4 * idx + 6).astype(np.float64).sum(1)
     got = out.cpu().numpy().reshape(wg, 512)
     np.testing.assert_allclose(got, np.broadcast_to(want, (wg, 512)), rtol=1e-6)
+
+
+def test_setpts_skips_only_unchanged_points():
+    # a repeated setpts of the same tensors is skipped; an in-place change of the
+    # coordinates, new tensors or new bin settings register the points again
+    from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_REAL
+    from cmtip_amd.nufft import NufftPlan
+    d = dev()
+    M, N = 21, 50_000
+    g = torch.Generator(device=d).manual_seed(21)
+    x, y, z = (torch.empty(N, device=d).uniform_(-3.0, 3.0, generator=g) for _ in range(3))
+    f = torch.randn((M, M, M), device=d, generator=g)
+    flags = CMTIP_GRID_REAL | CMTIP_PTS_REAL
+    say("setpts skip logic")
+
+    def fresh(xx, yy, zz):
+        p = NufftPlan(2, M, -1)
+        p.setpts(xx.clone(), yy.clone(), zz.clone())
+        o = torch.empty(N, device=d)
+        p.execute(o, f, flags)
+        return o
+
+    p = NufftPlan(2, M, -1)
+    out = torch.empty(N, device=d)
+    p.setpts(x, y, z)
+    p.setpts(x, y, z)  # skipped
+    p.execute(out, f, flags)
+    assert torch.equal(out, fresh(x, y, z))
+    x.mul_(0.5)  # in place: the version counter moves
+    p.setpts(x, y, z)
+    p.execute(out, f, flags)
+    assert torch.equal(out, fresh(x, y, z))
+    p.set_bins(6, 0, 0)
+    p.setpts(x, y, z)  # bins again with the new settings
+    p.execute(out, f, flags)
+    assert rel_l2(out, fresh(x, y, z)) < 1e-6
