@@ -59,10 +59,11 @@ def parse():
     p.add_argument("--nbatches", type=int, default=1)
     p.add_argument("--no-extras", action="store_true", help="time the step only")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic", "solve", "c4", "c5"],
+    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic", "solve", "autograd", "c4", "c5"],
                    help="slice: one slicing pass of the workload (for counter runs); nufft: setpts / execute "
                         "timing of both transforms; traffic: a 1 GiB copy (counter calibration) then one slicing "
-                        "pass; c4 / c5: matching throughput at the larger workloads (see large_main)")
+                        "pass; autograd: forward + backward of the differentiable slice (2048 orientations); c4 / "
+                        "c5: matching throughput at the larger workloads (see large_main)")
     return p.parse_args()
 
 
@@ -343,6 +344,30 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         del a, b
     ac = torch.from_numpy(synth.gaussian_ac(M, atoms, seed=0)).to(dev)
     s, ext = synth.detector(side)
+    if args.part == "autograd":
+        # forward (type 2) + backward (type 1 spread and three ik-weighted type 2 passes)
+        # of the L2 loss over 2048 orientations x P pixels, one warm-up and args.steps timed
+        from cmtip_amd import autograd as AG
+        nb = 2048
+        q = torch.from_numpy(synth.random_quaternions(nb, seed=0)).to(dev).requires_grad_(True)
+        acg = ac.clone().requires_grad_(True)
+        target = alignment.Slicer(ac, s, ext).slice(torch.from_numpy(synth.random_quaternions(nb, seed=1)).to(dev))
+
+        def fwdbwd():
+            q.grad = None
+            acg.grad = None
+            AG.l2_loss(AG.slice_quaternions(acg, q, s, ext), target).backward()
+        fwdbwd()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fwdbwd()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        res = {"part": "autograd", "M": M, "orientations": nb, "points": nb * side * side, "fwd_bwd_ms": ms}
+        log(f"[bench] autograd {res}")
+        print(json.dumps(res), flush=True)
+        return
     if args.part in ("slice", "traffic"):
         ref = synth.random_quaternions(n_ref, seed=0)
         sl = alignment.Slicer(ac, s, ext)
