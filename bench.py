@@ -276,7 +276,7 @@ def main():
                                          (1, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type1_real_M81"),
                                          (1, 2 * M, CMTIP_GRID_REAL | CMTIP_PTS_ONES, "type1_ones_M162")):
             p = nufft.get_plan(typ, modes, -1 if typ == 2 else 1)
-            ts = tm(lambda: p.setpts(x, y, z))
+            ts = tm(lambda: p.setpts(x, y, z, force=True))
             if typ == 2:
                 o = torch.empty(Np, device=dev, dtype=torch.float32 if flags & CMTIP_PTS_REAL else torch.complex64)
                 te = tm(lambda: p.execute(o, ac_d, flags))
@@ -419,7 +419,7 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
                                      (1, M, CMTIP_GRID_REAL | CMTIP_PTS_REAL, "type1_real"),
                                      (1, 2 * M, CMTIP_GRID_REAL | CMTIP_PTS_ONES, "type1_ones_2M")):
         p = nufft.get_plan(typ, modes, -1 if typ == 2 else 1)
-        ts = tm(lambda: p.setpts(x, y, z), 3)
+        ts = tm(lambda: p.setpts(x, y, z, force=True), 3)
         if typ == 2:
             out = torch.empty(N, device=dev, dtype=torch.float32 if flags & CMTIP_PTS_REAL else torch.complex64)
             te = tm(lambda: p.execute(out, ac, flags))

@@ -32,12 +32,14 @@ class NufftPlan:
         check(L.cmtip_nufft_info(h, ctypes.byref(w), ctypes.byref(n), ctypes.byref(beta)), "cmtip_nufft_info")
         self.w, self.n_fine, self.beta = w.value, n.value, beta.value
 
-    def setpts(self, x, y, z):
+    def setpts(self, x, y, z, force=False):
         """x, y, z: float32 device tensors of equal length (kept alive by the plan).
 
-        Registering the points this plan already holds is skipped: same storage (the plan
-        keeps the previous tensors alive, so an equal address is the same memory), same
-        length and the same version counter (no in-place write since)."""
+        Registering the points this plan already holds is skipped unless force: same
+        storage (the plan keeps the previous tensors alive, so an equal address is the
+        same memory), same length and the same version counter (no torch in-place write
+        since).  Writes that bypass torch's version counter (a kernel writing through a
+        raw pointer) are not seen: register such points with force=True."""
         n = x.numel()
         assert y.numel() == n and z.numel() == n
         for t in (x, y, z):
@@ -46,7 +48,7 @@ class NufftPlan:
             key = tuple((t.data_ptr(), t.numel(), t._version) for t in (x, y, z))
         except RuntimeError:  # inference tensors carry no version counter
             key = None
-        if key is not None and key == self._pts_key:
+        if not force and key is not None and key == self._pts_key:
             return
         self._pts = (x, y, z)
         self._pts_key = None
