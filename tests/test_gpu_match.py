@@ -282,6 +282,50 @@ def test_prepare_images_norms_one_pass(P):
     assert float(((n1.double() - want).abs() / want).max()) < 1e-6
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_match_gamma_exact(masked):
+    # std(images) / std(model) from (sum, sum of squares[, count]) in FP64 with the
+    # reference's order of operations, rounded to float32, written to every slot
+    import struct
+    from cmtip_amd._lib import check, lib, ptr, stream
+    d = dev()
+    rng = np.random.default_rng(13 + masked)
+    img = rng.random(3) * np.array([1e6, 1e9, 1.0])
+    img_count = 0.0 if masked else 4.2e7
+    if masked:
+        img[2] = 3.1e7
+    mdl = rng.random(2) * np.array([2e5, 3e7])
+    model_count = 1.6384e8
+    img_d = torch.tensor(img, dtype=torch.float64, device=d)
+    mdl_d = torch.tensor(mdl, dtype=torch.float64, device=d)
+    out = torch.zeros(37, dtype=torch.float64, device=d)
+    say(f"match_gamma masked={masked}")
+    check(lib().cmtip_match_gamma(ptr(img_d), img_count, ptr(mdl_d), model_count, 37, ptr(out), stream()))
+    ci = max(img[2], 1.0) if masked else img_count
+    mi = img[0] / ci
+    si = np.sqrt(max(img[1] / ci - mi * mi, 0.0))
+    mm = mdl[0] / model_count
+    sm = np.sqrt(max(mdl[1] / model_count - mm * mm, 0.0))
+    want = struct.unpack("f", struct.pack("f", si / sm))[0]
+    assert torch.equal(out, torch.full_like(out, want))
+
+
+def test_weights2_cache():
+    # squared pixel weights: cached per host detector array and order (one device
+    # tensor reused), recomputed for device inputs, equal to generate_weights squared
+    from cmtip_amd.alignment import align as AL
+    s, _ = synth.detector(32)
+    say("squared weights cache")
+    for order in (0, -2):
+        a = AL._weights2(s, order)
+        assert AL._weights2(s.copy(), order) is a
+        w = AL.generate_weights(torch.as_tensor(s, device=dev()), order=order).reshape(-1)
+        assert torch.equal(a, w * w)
+        t = torch.as_tensor(s, device=dev())
+        assert AL._weights2(t, order) is not AL._weights2(t, order)
+    assert AL._weights2(s, 0) is not AL._weights2(s, -2)
+
+
 def test_cross_bf16_vs_fp32_product():
     from cmtip_amd._lib import check, lib, ptr, stream
     d = dev()
