@@ -19,6 +19,7 @@ CMTIP_GRID_REAL = 1
 CMTIP_PTS_REAL = 2
 CMTIP_PTS_ONES = 4
 CMTIP_PTS_IMAG = 8
+CMTIP_EXEC_ADJOINT = 16
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

@@ -6,15 +6,16 @@ Forward (compute_slices, cmtip/alignment/align.py:39-65):
 Backward, for an upstream gradient g (real, one value per point):
     dL/df_k     = sum_j g_j cos(k.x_j) = Re(type 1 of g)_k          (the adjoint spread)
     dL/dx_{j,d} = g_j Im(sum_k k_d f_k exp(-i k.x_j))               (an ik-weighted type 2, d = 0, 1, 2)
-Both run on the same HIP NUFFT plans as the forward; the coordinate gradient uses the
-type-2 imaginary-part output (real interpolation of the fine grid's imaginary part).  ``rotated_coords`` builds the
+Both run on the forward's type-2 plan and its point binning: the spread as the plan's
+adjoint (CMTIP_EXEC_ADJOINT), the coordinate gradient as type-2 imaginary-part outputs
+(real interpolation of the fine grid's imaginary part).  ``rotated_coords`` builds the
 rotated pixel coordinates from quaternions with torch operations, so gradients reach
 the quaternions for orientation refinement.
 """
 import numpy as np
 import torch
 
-from ._lib import CMTIP_GRID_REAL, CMTIP_PTS_IMAG, CMTIP_PTS_REAL
+from ._lib import CMTIP_EXEC_ADJOINT, CMTIP_GRID_REAL, CMTIP_PTS_IMAG, CMTIP_PTS_REAL
 from .nufft.plan import DEFAULT_EPS, get_plan
 
 
@@ -46,14 +47,13 @@ class NufftSlice(torch.autograd.Function):
         M = ac.shape[0]
         g = grad.contiguous().float().reshape(-1)
         grad_ac = grad_H = grad_K = grad_L = None
+        p2 = get_plan(2, M, -1, eps)
+        p2.setpts(x, y, z)  # the forward's points: already registered unless the plan was used since
         if ctx.needs_input_grad[0]:
-            p1 = get_plan(1, M, +1, eps)
-            p1.setpts(x, y, z)
+            # the adjoint of the forward plan: a type-1 spread over the same bins
             grad_ac = torch.empty_like(ac)
-            p1.execute(g, grad_ac, CMTIP_GRID_REAL | CMTIP_PTS_REAL, None, 1.0)
+            p2.execute(g, grad_ac, CMTIP_GRID_REAL | CMTIP_PTS_REAL | CMTIP_EXEC_ADJOINT, None, 1.0)
         if any(ctx.needs_input_grad[1:4]):
-            p2 = get_plan(2, M, -1, eps)
-            p2.setpts(x, y, z)
             k = _modes(M, ac.device)
             outs = []
             for d in range(3):

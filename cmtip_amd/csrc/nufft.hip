@@ -2023,9 +2023,21 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
                                    float scale, void* stream) {
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
     hipStream_t st = as_stream(stream);
+    // the adjoint runs the other direction with the opposite sign on the same bins; the
+    // sign is flipped on the plan for the duration of the call
+    const bool adjoint = (flags & CMTIP_EXEC_ADJOINT) != 0;
+    const int type = adjoint ? 3 - p->type : p->type;
+    struct SignScope {
+        cmtip_nufft_plan_s* p;
+        bool flip;
+        ~SignScope() {
+            if (flip) p->iflag = -p->iflag;
+        }
+    } sign_scope{p, adjoint};
+    if (adjoint) p->iflag = -p->iflag;
     const bool grid_real = flags & CMTIP_GRID_REAL;
-    const bool pts_ones = (flags & CMTIP_PTS_ONES) && p->type == 1;
-    const bool pts_imag = (flags & CMTIP_PTS_IMAG) && p->type == 2;
+    const bool pts_ones = (flags & CMTIP_PTS_ONES) && type == 1;
+    const bool pts_imag = (flags & CMTIP_PTS_IMAG) && type == 2;
     const bool pts_real = (flags & CMTIP_PTS_REAL) || pts_ones || pts_imag;
     if (pts_ones) c = nullptr;
     if (!f || (p->npts > 0 && !c && !pts_ones)) {
@@ -2036,7 +2048,7 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
         set_error("execute: setpts not called");
         return CMTIP_ERR_STATE;
     }
-    if (p->type == 2) {
+    if (type == 2) {
         p->slice_ready = 0;
         // the interpolation weights are real: Re / Im of c interpolate Re / Im of the fine
         // grid, so every form is one or two real passes (complex c: real part into the
@@ -2052,6 +2064,10 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
         return CMTIP_OK;
     }
     // type 1
+    if (adjoint && p->npts > 0 && subgrid_lds(p->bs, p->w, true, !pts_real) + kSpreadStaticLds > kMaxLds) {
+        set_error("execute: bin size %d of this type-2 plan is too large for the adjoint spread", p->bs);
+        return CMTIP_ERR_ARG;
+    }
     p->slice_ready = 0;
     const int64_t n3 = (int64_t)p->n * p->n * p->n;
     CMTIP_TRY(fill_zero(p->fine.p, pts_real ? n3 : 2 * n3, st));

@@ -75,6 +75,13 @@ int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, cons
                                all-ones kernel input of autocorrelation.py:143,146 */
 #define CMTIP_PTS_IMAG 8    /* type 2 only: c (float) receives Im(T2(f)), e.g. the
                                coordinate gradient of a real slice */
+#define CMTIP_EXEC_ADJOINT 16 /* run the adjoint of the plan's transform on its registered
+                               points: a type-2 plan spreads (type 1, sign flipped), a
+                               type-1 plan interpolates (type 2, sign flipped); c and f
+                               keep their meaning (c: values at the points, f: the
+                               uniform grid) and the flags above apply to the direction
+                               executed.  The bin size must suit that direction (the
+                               defaults do). */
 
 /* Type 2: c = T2(support (.) f); the support (float32 M^3, may be NULL) multiplies f
  * on load (forward.py:34-35); with CMTIP_PTS_REAL only Re(c) is written.

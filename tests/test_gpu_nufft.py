@@ -206,3 +206,48 @@ def test_setpts_skips_only_unchanged_points():
     p.setpts(x, y, z)  # bins again with the new settings
     p.execute(out, f, flags)
     assert rel_l2(out, fresh(x, y, z)) < 1e-6
+
+
+@pytest.mark.parametrize("real", [True, False])
+def test_adjoint_flag_matches_the_other_plan_type(real):
+    # a plan's adjoint on its own bins gives what the plan of the other type gives on
+    # the same points (same bins, same sign): the interpolation bit for bit, the spread
+    # up to the order of the float atomics that flush its chunks into the fine grid
+    from cmtip_amd._lib import CMTIP_EXEC_ADJOINT, CMTIP_GRID_REAL, CMTIP_PTS_REAL, CmtipError
+    from cmtip_amd.nufft import NufftPlan
+    d = dev()
+    M, N = 33, 200_000
+    g = torch.Generator(device=d).manual_seed(23)
+    x, y, z = (torch.empty(N, device=d).uniform_(-3.1, 3.1, generator=g) for _ in range(3))
+    if real:
+        flags = CMTIP_GRID_REAL | CMTIP_PTS_REAL
+        c = torch.randn(N, device=d, generator=g)
+        f = torch.randn((M, M, M), device=d, generator=g)
+        fo, co = torch.empty((M, M, M), device=d), torch.empty(N, device=d)
+    else:
+        flags = 0
+        c = torch.randn(N, dtype=torch.complex64, device=d, generator=g)
+        f = torch.randn((M, M, M), dtype=torch.complex64, device=d, generator=g)
+        fo, co = (torch.empty((M, M, M), dtype=torch.complex64, device=d),
+                  torch.empty(N, dtype=torch.complex64, device=d))
+    say(f"adjoint flag real={real}")
+    p2, p1 = NufftPlan(2, M, -1), NufftPlan(1, M, +1)
+    p2.setpts(x, y, z)
+    p1.setpts(x, y, z)
+    want_f = torch.empty_like(fo)
+    p1.execute(c, want_f, flags, None, 1.0 / M ** 3)
+    p2.execute(c, fo, flags | CMTIP_EXEC_ADJOINT, None, 1.0 / M ** 3)
+    assert rel_l2(fo, want_f) < 1e-6
+    want_c = torch.empty_like(co)
+    p2.execute(want_c, f, flags)
+    p1.execute(co, f, flags | CMTIP_EXEC_ADJOINT)
+    assert torch.equal(co, want_c)
+    # the plan keeps its own direction and sign afterwards
+    again = torch.empty_like(co)
+    p2.execute(again, f, flags)
+    assert torch.equal(again, want_c)
+    # a type-2 bin size beyond the spreading subgrid is refused for the adjoint
+    p2.set_bins(26, 0, 0)
+    p2.setpts(x, y, z)
+    with pytest.raises(CmtipError):
+        p2.execute(c, fo, flags | CMTIP_EXEC_ADJOINT, None, 1.0)
