@@ -290,6 +290,17 @@ def main():
         t1 = res_nufft["type1_real_M81"]["execute_ms"] * 1e-3
         # 20 B per point + 4 M^3 per transform (real form), execute only
         res_nufft["type2_plus_type1_real_hbm_frac"] = ((20 * Np + 4 * M ** 3) * 2 / (t2 + t1) / 1e9) / HBM_PEAK_GBS
+        # the limits these kernels meet: LDS reads (interpolation: 49 rows x 28 B per point
+        # at w = 7) and 64-bit LDS atomics (spreading: w^3 per point)
+        w = nufft.get_plan(2, M, -1).w
+        if w == 7:
+            read_peak, atomic_peak = lds_read_ceiling(dev), lds_atomic_ceiling(dev)
+            rd = 49 * 28 * Np / t2 / 1e12
+            at = w ** 3 * Np / t1 / 1e9
+            res_nufft["lds"] = {"type2_real_read_TBps": rd, "read_ceiling_TBps": read_peak,
+                                "type2_read_frac": rd / read_peak, "type1_real_atomic_Gops": at,
+                                "atomic_ceiling_Gops": atomic_peak, "type1_atomic_frac": at / atomic_peak,
+                                "note": "ceilings from cmtip_lds_read_probe / cmtip_lds_atomic_probe on this GPU"}
         result["nufft"] = res_nufft
         log(f"[bench] nufft {result['nufft']}")
         # CG iteration time of the autocorrelation solve (20 iterations)
@@ -431,29 +442,49 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
     print(json.dumps({"part": "nufft", "M": M, **res}), flush=True)
 
 
+def _probe_best_ms(fn, dev, wg, iters):
+    import torch
+
+    from cmtip_amd._lib import check, ptr, stream
+
+    out = torch.empty(wg * 512, device=dev)
+    check(fn(wg, iters, ptr(out), stream()), "LDS probe")
+    best = None
+    for _ in range(3):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        check(fn(wg, iters, ptr(out), stream()), "LDS probe")
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        best = ms if best is None else min(best, ms)
+    return best
+
+
 def lds_read_ceiling(dev):
     """TB/s of conflict-free ds_read_b128 over the whole GPU (cmtip_amd's probe kernel:
     2 workgroups of 512 threads per CU, 64 KB of LDS each), best of 3 timed launches."""
     import torch
 
     from cmtip_amd import _lib
-    from cmtip_amd._lib import check, ptr, stream
 
-    L = _lib.lib()
     wg = 2 * torch.cuda.get_device_properties(dev).multi_processor_count
     iters = 20000
-    out = torch.empty(wg * 512, device=dev)
-    check(L.cmtip_lds_read_probe(wg, iters, ptr(out), stream()), "cmtip_lds_read_probe")
-    best = None
-    for _ in range(3):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        check(L.cmtip_lds_read_probe(wg, iters, ptr(out), stream()), "cmtip_lds_read_probe")
-        b.record()
-        torch.cuda.synchronize()
-        ms = a.elapsed_time(b)
-        best = ms if best is None else min(best, ms)
-    return wg * 512 * 16.0 * iters / (best * 1e-3) / 1e12
+    ms = _probe_best_ms(_lib.lib().cmtip_lds_read_probe, dev, wg, iters)
+    return wg * 512 * 16.0 * iters / (ms * 1e-3) / 1e12
+
+
+def lds_atomic_ceiling(dev):
+    """G ops/s of 64-bit LDS atomic adds over the whole GPU (the same launch shape, each
+    lane on its own counter), best of 3 timed launches."""
+    import torch
+
+    from cmtip_amd import _lib
+
+    wg = 2 * torch.cuda.get_device_properties(dev).multi_processor_count
+    iters = 20000
+    ms = _probe_best_ms(_lib.lib().cmtip_lds_atomic_probe, dev, wg, iters)
+    return wg * 512.0 * iters / (ms * 1e-3) / 1e9
 
 
 def large_main(args, dev):

@@ -68,6 +68,7 @@ SIGNATURES = {
     "cmtip_center_rows_sq": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_match_rescore_masked": [_vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _i, _vp, _vp, _vp],
     "cmtip_lds_read_probe": [_i, _i, _vp, _vp],
+    "cmtip_lds_atomic_probe": [_i, _i, _vp, _vp],
     "cmtip_acsolver_create": [_i, _i, ctypes.POINTER(_vp)],
     "cmtip_acsolver_destroy": [_vp],
     "cmtip_ac_support": [_vp, _i, _vp, _vp],

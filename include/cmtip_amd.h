@@ -344,6 +344,10 @@ int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, 
  * write one float per thread to out (workgroups x 512 floats): the LDS read ceiling
  * that the interpolation kernels are measured against, timed by the caller. */
 int cmtip_lds_read_probe(int workgroups, int iters, float* out, void* stream);
+/* No counterpart in the reference.  The same launch shape with ds_add_u64 (each lane its
+ * own 8-byte counter, a wave on consecutive counters): the 64-bit LDS atomic rate the
+ * spreading kernel's accumulation is measured against. */
+int cmtip_lds_atomic_probe(int workgroups, int iters, float* out, void* stream);
 
 #ifdef __cplusplus
 }

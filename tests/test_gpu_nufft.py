@@ -251,3 +251,19 @@ def test_adjoint_flag_matches_the_other_plan_type(real):
     p2.setpts(x, y, z)
     with pytest.raises(CmtipError):
         p2.execute(c, fo, flags | CMTIP_EXEC_ADJOINT, None, 1.0)
+
+
+def test_lds_atomic_probe_values():
+    # counter c receives it + 1 from every (thread t, iteration it) with
+    # (t + 64 it) mod 8192 = c; thread t reports counter t
+    from cmtip_amd._lib import check, lib, ptr, stream
+    wg, iters = 2, 300
+    out = torch.empty(wg * 512, device=dev())
+    say("lds atomic probe")
+    check(lib().cmtip_lds_atomic_probe(wg, iters, ptr(out), stream()))
+    acc = np.zeros(8192)
+    t = np.arange(512)
+    for it in range(iters):
+        np.add.at(acc, (t + 64 * it) & 8191, it + 1)
+    got = out.cpu().numpy().reshape(wg, 512)
+    np.testing.assert_allclose(got, np.broadcast_to(acc[:512], (wg, 512)), rtol=1e-6)
