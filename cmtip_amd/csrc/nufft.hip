@@ -157,7 +157,10 @@ constexpr int kSpreadThreads = CMTIP_SPREAD_THREADS;  // spreading workgroups
 #define CMTIP_SPREAD_DEAL 1
 #endif
 constexpr bool kSpreadDeal = CMTIP_SPREAD_DEAL;  // deal points to lanes by LDS bank class
-constexpr int kSpreadClasses = 32;  // LDS bank pairs of 64-bit accumulators
+// a 64-bit accumulator sits on one of the 16 pairs of the 32 four-byte LDS banks;
+// dealing by those 16 classes measured 8.4-8.5 ms at M = 81 (81.92 M points, real
+// strengths) against 9.0 (32 classes), 10.0 (8) and 11.0 (64)
+constexpr int kSpreadClasses = 16;
 // static LDS of the spreading kernel (point dealing), beside its dynamic subgrid
 constexpr size_t kSpreadStaticLds = 3 * kChunk + 4 * 3 * kSpreadClasses + 64;
 constexpr size_t kMaxLds = 160 * 1024;     // LDS per workgroup on gfx950
@@ -638,9 +641,9 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         }
         __syncthreads();
         // deal the chunk's points to lanes by the LDS bank pair of their first accumulator
-        // (class = its 64-bit cell index mod 32): the two lanes of a wave that share a
-        // class sit 32 lanes apart, so every atomic of the 343-add sequence, which adds
-        // the same offset for all lanes, hits distinct bank pairs within a 32-lane pass
+        // (class = its 64-bit cell index mod 16): lanes that share a class sit 16 lanes
+        // apart, so every atomic of the 343-add sequence, which adds the same offset for
+        // all lanes, hits 16 distinct bank pairs within each group of 16 lanes
         const int npts = kSpreadDeal ? ch.z - ch.y : 0;  // <= kChunk
         for (int i = threadIdx.x; i < npts; i += kSpreadThreads) {
             float t;
