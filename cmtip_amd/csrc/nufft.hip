@@ -641,7 +641,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         }
         __syncthreads();
         // deal the chunk's points to lanes by the LDS bank pair of their first accumulator
-        // (class = its 64-bit cell index mod 16): lanes that share a class sit 16 lanes
+        // (class = its cell index mod 16): lanes that share a class sit 16 lanes
         // apart, so every atomic of the 343-add sequence, which adds the same offset for
         // all lanes, hits 16 distinct bank pairs within each group of 16 lanes
         const int npts = kSpreadDeal ? ch.z - ch.y : 0;  // <= kChunk
@@ -651,7 +651,10 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             const int ux = first_node_wrapped(xs[k], hi, lo, n, W, &t) - ox;
             const int uy = first_node_wrapped(ys[k], hi, lo, n, W, &t) - oy;
             const int uz = first_node_wrapped(zs[k], hi, lo, n, W, &t) - oz;
-            const int cl = (NC * (ux * SSP + uy * SP + uz)) & (kSpreadClasses - 1);
+            // the cell index, not the accumulator index: a complex cell's two accumulators
+            // would give only even classes and leave half the lanes without points
+            // (complex type 1 over 41 M points: 20.1 ms that way, 11.9 ms this way)
+            const int cl = (ux * SSP + uy * SP + uz) & (kSpreadClasses - 1);
             cls_of[i] = (uint8_t)cl;
             atomicAdd(cls_count + cl, 1);
         }
