@@ -65,7 +65,7 @@ struct cmtip_nufft_plan_s {
     DevBuf offset;    // nbins + 1 int32
     DevBuf nchunk;    // nbins + 1 int32: chunk counts, then their exclusive scan
     DevBuf perm;      // N int32: sorted position -> original index
-    DevBuf xs, ys, zs;  // N float: coordinates in sorted order
+    DevBuf sorted;    // N float4 (x, y, z, original index as int bits) in sorted order
     DevBuf chunks;    // max_chunks x int4 (bin, start, end, 0)
     DevBuf ctrl;      // int32: [0] number of chunks, [1] work counter
     int64_t max_chunks = 0;
@@ -347,25 +347,24 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count,
     }
 }
 
-// COPY: also write the coordinates in sorted order (registered points); the slicing
-// route regenerates them from perm instead.
+// COPY (registered points): write one 16-byte record (x, y, z, original index) per
+// sorted position, one scattered store per point; else (the slicing route, which
+// regenerates the coordinates) write the original index to perm.
 template <bool COPY>
 __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                    const float* __restrict__ z, int64_t N, const int* __restrict__ bin_of,
                                    const int* __restrict__ offset, int* __restrict__ cursor, int* __restrict__ perm,
-                                   float* __restrict__ xs, float* __restrict__ ys, float* __restrict__ zs) {
+                                   float4* __restrict__ sorted) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = j < N;
     const int b = valid ? bin_of[j] : 0;
     const int slot = wave_bin_add(cursor, b, valid);
     if (!valid) return;
     const int pos = offset[b] + slot;
-    perm[pos] = (int)j;
-    if (COPY) {
-        xs[pos] = x[j];
-        ys[pos] = y[j];
-        zs[pos] = z[j];
-    }
+    if (COPY)
+        sorted[pos] = make_float4(x[j], y[j], z[j], __int_as_float((int)j));
+    else
+        perm[pos] = (int)j;
 }
 
 // Workgroup-privatised binning (when the bin counters fit LDS): a workgroup takes a
@@ -405,8 +404,7 @@ template <bool COPY>
 __global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int64_t N,
     int64_t per_block, int nbins, const int* __restrict__ bin_of, const int* __restrict__ offset,
-    int* __restrict__ cursor, int* __restrict__ perm, float* __restrict__ xs, float* __restrict__ ys,
-    float* __restrict__ zs) {
+    int* __restrict__ cursor, int* __restrict__ perm, float4* __restrict__ sorted) {
     extern __shared__ int hist[];
     for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
     __syncthreads();
@@ -421,12 +419,10 @@ __global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
     __syncthreads();
     for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
         const int pos = atomicAdd(hist + bin_of[j], 1);
-        perm[pos] = (int)j;
-        if (COPY) {
-            xs[pos] = x[j];
-            ys[pos] = y[j];
-            zs[pos] = z[j];
-        }
+        if (COPY)
+            sorted[pos] = make_float4(x[j], y[j], z[j], __int_as_float((int)j));
+        else
+            perm[pos] = (int)j;
     }
 }
 
@@ -609,8 +605,8 @@ __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsign
 template <int W, bool CPLX>
 __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     typename Cell<CPLX>::T* __restrict__ grid, int n, const EsKernel esk, float hi, float lo, int bs, int nbx,
-    const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
-    const int* __restrict__ perm, const typename Cell<CPLX>::T* __restrict__ c, const unsigned* __restrict__ cmax_bits,
+    const float4* __restrict__ sorted, const typename Cell<CPLX>::T* __restrict__ c,
+    const unsigned* __restrict__ cmax_bits,
     const int4* __restrict__ chunks, int* __restrict__ ctrl) {
     constexpr int NC = CPLX ? 2 : 1;  // integer accumulators per cell
     extern __shared__ unsigned long long acc[];
@@ -647,10 +643,10 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         const int npts = kSpreadDeal ? ch.z - ch.y : 0;  // <= kChunk
         for (int i = threadIdx.x; i < npts; i += kSpreadThreads) {
             float t;
-            const int k = ch.y + i;
-            const int ux = first_node_wrapped(xs[k], hi, lo, n, W, &t) - ox;
-            const int uy = first_node_wrapped(ys[k], hi, lo, n, W, &t) - oy;
-            const int uz = first_node_wrapped(zs[k], hi, lo, n, W, &t) - oz;
+            const float4 q = sorted[ch.y + i];
+            const int ux = first_node_wrapped(q.x, hi, lo, n, W, &t) - ox;
+            const int uy = first_node_wrapped(q.y, hi, lo, n, W, &t) - oy;
+            const int uz = first_node_wrapped(q.z, hi, lo, n, W, &t) - oz;
             // the cell index, not the accumulator index: a complex cell's two accumulators
             // would give only even classes and leave half the lanes without points
             // (complex type 1 over 41 M points: 20.1 ms that way, 11.9 ms this way)
@@ -687,9 +683,11 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                 if (k >= ch.z) continue;
             }
             float tx, ty, tz;
-            const int lxw = first_node_wrapped(xs[k], hi, lo, n, W, &tx);
-            const int lyw = first_node_wrapped(ys[k], hi, lo, n, W, &ty);
-            const int lzw = first_node_wrapped(zs[k], hi, lo, n, W, &tz);
+            const float4 q = sorted[k];
+            const int src = __float_as_int(q.w);
+            const int lxw = first_node_wrapped(q.x, hi, lo, n, W, &tx);
+            const int lyw = first_node_wrapped(q.y, hi, lo, n, W, &ty);
+            const int lzw = first_node_wrapped(q.z, hi, lo, n, W, &tz);
             int lx, ly, lz;
             float wx[W], wy[W], wz[W];
             local_weights<W>(tx, lxw, ox, esk, &lx, wx);
@@ -699,11 +697,11 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             if ((unsigned)lx >= (unsigned)bs || (unsigned)ly >= (unsigned)bs || (unsigned)lz >= (unsigned)bs) continue;
             float vr, vi = 0.f;
             if (CPLX) {
-                const float2 cv = reinterpret_cast<const float2*>(c)[perm[k]];
+                const float2 cv = reinterpret_cast<const float2*>(c)[src];
                 vr = ldexpf(cv.x, sc);
                 vi = ldexpf(cv.y, sc);
             } else {
-                vr = ldexpf(c ? reinterpret_cast<const float*>(c)[perm[k]] : 1.0f, sc);
+                vr = ldexpf(c ? reinterpret_cast<const float*>(c)[src] : 1.0f, sc);
             }
             unsigned long long* base = acc + NC * (lx * SSP + ly * SP + lz);
 #pragma unroll
@@ -745,15 +743,13 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
 // the sorted coordinate copies of setpts; SortedRotated (below): regenerated from
 // perm[k] for the slicing route.
 struct SortedCopies {
-    const float* xs;
-    const float* ys;
-    const float* zs;
-    const int* perm;
+    const float4* sorted;
     __device__ __forceinline__ int64_t at(int k, float* px, float* py, float* pz) const {
-        *px = xs[k];
-        *py = ys[k];
-        *pz = zs[k];
-        return perm[k];
+        const float4 q = sorted[k];
+        *px = q.x;
+        *py = q.y;
+        *pz = q.z;
+        return __float_as_int(q.w);
     }
 };
 
@@ -1512,9 +1508,7 @@ struct InterpLaunch {
         if (rot)
             launch(p, SortedRotated{*rot, p->perm.as<const int>()}, out, ostride, st);
         else
-            launch(p, SortedCopies{p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
-                                   p->perm.as<const int>()},
-                   out, ostride, st);
+            launch(p, SortedCopies{p->sorted.as<const float4>()}, out, ostride, st);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -1545,14 +1539,12 @@ struct SpreadLaunch {
         if (cplx)
             hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadThreads), lds, st,
                                p->fine.as<float2>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
-                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
-                               p->perm.as<const int>(), reinterpret_cast<const float2*>(c), cmax, ch,
+                               p->sorted.as<const float4>(), reinterpret_cast<const float2*>(c), cmax, ch,
                                p->ctrl.as<int>());
         else
             hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
                                p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
-                               p->xs.as<const float>(), p->ys.as<const float>(), p->zs.as<const float>(),
-                               p->perm.as<const int>(), reinterpret_cast<const float*>(c), cmax, ch,
+                               p->sorted.as<const float4>(), reinterpret_cast<const float*>(c), cmax, ch,
                                p->ctrl.as<int>());
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
@@ -1572,12 +1564,10 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     CMTIP_TRY(p->count.ensure(sizeof(int) * (size_t)p->nbins));
     CMTIP_TRY(p->offset.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
     CMTIP_TRY(p->nchunk.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
-    CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)N));
-    if (!rot) {
-        CMTIP_TRY(p->xs.ensure(sizeof(float) * (size_t)N));
-        CMTIP_TRY(p->ys.ensure(sizeof(float) * (size_t)N));
-        CMTIP_TRY(p->zs.ensure(sizeof(float) * (size_t)N));
-    }
+    if (rot)
+        CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)N));
+    else
+        CMTIP_TRY(p->sorted.ensure(sizeof(float4) * (size_t)N));
     CMTIP_TRY(p->chunks.ensure(sizeof(int4) * (size_t)p->max_chunks));
     CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
     const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
@@ -1610,12 +1600,11 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
             hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, N, per_block, nb,
                                p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                               p->perm.as<int>(), (float*)nullptr, (float*)nullptr, (float*)nullptr);
+                               p->perm.as<int>(), (float4*)nullptr);
         else
             hipLaunchKernelGGL(bin_scatter_local_kernel<true>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st, p->x,
                                p->y, p->z, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
-                               p->count.as<int>(), p->perm.as<int>(), p->xs.as<float>(), p->ys.as<float>(),
-                               p->zs.as<float>());
+                               p->count.as<int>(), (int*)nullptr, p->sorted.as<float4>());
         hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                            p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
         CMTIP_LAUNCH_CHECK();
@@ -1632,12 +1621,11 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     if (rot)
         hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(gp), dim3(kBlock), 0, st, (const float*)nullptr,
                            (const float*)nullptr, (const float*)nullptr, N, p->bin_of.as<const int>(),
-                           p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(), (float*)nullptr,
-                           (float*)nullptr, (float*)nullptr);
+                           p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr);
     else
         hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
                            p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                           p->perm.as<int>(), p->xs.as<float>(), p->ys.as<float>(), p->zs.as<float>());
+                           (int*)nullptr, p->sorted.as<float4>());
     hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                        p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
     CMTIP_LAUNCH_CHECK();
@@ -1686,8 +1674,7 @@ struct TileLaunch {
             hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), hlds, st,
                                (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, ntiles, per_block,
                                (int)(nbins + 1), p->bin_of.as<const int>(), p->offset.as<const int>(),
-                               p->count.as<int>(), p->perm.as<int>(), (float*)nullptr, (float*)nullptr,
-                               (float*)nullptr);
+                               p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr);
         else
             hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
                                p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
@@ -2175,7 +2162,7 @@ extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     p->tmpB.release();
     p->rinv.release();
     p->part.release();
-    for (DevBuf* b : {&p->bin_of, &p->count, &p->offset, &p->nchunk, &p->perm, &p->xs, &p->ys, &p->zs, &p->chunks,
+    for (DevBuf* b : {&p->bin_of, &p->count, &p->offset, &p->nchunk, &p->perm, &p->sorted, &p->chunks,
                       &p->ctrl, &p->probe})
         b->release();
     delete p;
