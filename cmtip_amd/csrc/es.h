@@ -37,10 +37,12 @@ inline double es_phi_host(double z, int w, double beta) {
 // f_a(y) = phi(a - tau); the mirror tap satisfies f_{w-1-a}(y) = f_a(-y).  Each f_a
 // (a < w/2) is the degree-(w + 1) polynomial interpolating it at the Chebyshev points of
 // [-1/2, 1/2]; it is stored as its even part E(y^2) and odd part y O(y^2), so the pair
-// is E + y O and E - y O.  Odd w adds the even part of the middle tap.  Interpolation
-// error against phi (max over y, double precision): w = 7, 5e-8 at the two outer taps
-// (the square-root edge of phi) and 2e-9 at the inner ones; at most ~eps / 20 for
-// every w <= 8.
+// is E + y O and E - y O.  Odd w adds the even part of the middle tap.  Error against
+// phi with the float32 coefficients (max over taps and y; eps = 10^(1 - w), the
+// tolerance a width serves): w = 2 .. 8: 1.4e-2, 3.8e-4, 8.5e-5, 5.4e-6, 6.2e-7, 5.9e-8,
+// 2.7e-8, i.e. below eps / 7 for w <= 7 and eps / 3.7 at w = 8 (float32 rounding).
+// At w = 7 the outer taps (the square-root edge of phi) carry it; the inner ones are
+// below 3.1e-8.
 constexpr int kEsPolyMaxW = 8;
 constexpr int kEsPolyCoefs = 40;  // w = 8: 4 pairs x 10
 
