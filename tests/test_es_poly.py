@@ -110,6 +110,13 @@ def test_es_poly_mirror_sign(es):
     t = taps_from_poly(c, w, np.array([0.5]))[:, 0]
     assert t[0] < 1e-6
     assert abs(t[-1] - phi(2.5, w, beta)) < 1e-7 and t[-1] > 1e-3
+    # the same comparison as test_es_poly_taps_match_phi fails when the reconstruction
+    # swaps the mirror signs (taps read as E - yO and E + yO)
+    y = np.linspace(-0.5, 0.5, 4001)[1:]
+    tau = y + 0.5 * (w - 1)
+    exact = np.stack([phi(a - tau, w, beta) for a in range(w)])
+    swapped = taps_from_poly(c, w, -y)
+    assert np.abs(swapped - exact).max() > 1e-2
 
 
 def fine_size_brute(M, w):
