@@ -13,6 +13,7 @@ set, otherwise a seeded uniform set (cmtip_amd.synth.random_quaternions) is draw
 import collections
 import ctypes
 import hashlib
+import itertools
 
 import numpy as np
 import torch
@@ -155,6 +156,9 @@ def pixel_order(s):
     return torch.argsort(code, stable=True).to(torch.int32).contiguous()
 
 
+_SLICER_IDS = itertools.count(1)
+
+
 class Slicer:
     """Slices of one autocorrelation at many orientations (compute_slices, align.py:39-65).
 
@@ -162,7 +166,9 @@ class Slicer:
     Re(T2(ac)) at the rotated pixels of a batch of quaternions.  Pixels are processed
     in ``pixel_order`` (the output layout does not change).  The library picks the
     fused-tile or the binned route per detector geometry (``plan.set_slice_route``
-    fixes one); both give the same values.
+    fixes one); both give the same values.  Slicers of the same M share one plan: a
+    Slicer whose volume is no longer the one in the plan's grid loads it again before
+    slicing.
     """
 
     def __init__(self, ac, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS, tile_pixels=True):
@@ -172,9 +178,12 @@ class Slicer:
         self.s, self.order = _geometry(pixel_position_reciprocal, tile_pixels)
         self.P = self.s.shape[1]
         self.ext = _ext32(reciprocal_extent)
-        self.plan.slice_prepare(self.ac)
+        self._token = next(_SLICER_IDS)
+        self.plan.slice_prepare(self.ac, owner=self._token)
 
     def slice(self, quats, out=None, out_bf16=None, moments=None):
+        if self.plan.slice_owner != self._token:
+            self.plan.slice_prepare(self.ac, owner=self._token)
         q = _arrays.to_dev(quats, torch.float32).reshape(-1, 4)
         if out is None:
             out = torch.empty((q.shape[0], self.P), dtype=torch.float32, device=q.device)

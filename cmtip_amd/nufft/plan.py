@@ -28,6 +28,7 @@ class NufftPlan:
         self.eps = float(eps)
         self._pts = None
         self._pts_key = None
+        self.slice_owner = None
         w, n, beta = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(L.cmtip_nufft_info(h, ctypes.byref(w), ctypes.byref(n), ctypes.byref(beta)), "cmtip_nufft_info")
         self.w, self.n_fine, self.beta = w.value, n.value, beta.value
@@ -98,8 +99,12 @@ class NufftPlan:
         table = [tuple(cands[i]) + (times[2 * i], times[2 * i + 1]) for i in range(n)]
         return tuple(cands[best.value]), table
 
-    def slice_prepare(self, ac, support=None):
+    def slice_prepare(self, ac, support=None, owner=None):
+        """Loads ac into the plan's slicing grid; ``owner`` (an integer token) is recorded as
+        the holder of that grid, so users sharing the plan can tell whose volume it is."""
+        self.slice_owner = None
         check(_lib.lib().cmtip_slice_prepare(self._h, ptr(ac), ptr(support), stream()), "cmtip_slice_prepare")
+        self.slice_owner = owner
 
     def set_slice_route(self, route):
         """0 automatic, 1 fused tiles, 2 binned points, 3 binned tiles."""

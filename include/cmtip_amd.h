@@ -333,8 +333,10 @@ int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* out, double a
 /* scipy.sparse.linalg.cg (scipy/sparse/linalg/_isolve/iterative.py:305-422) on W with
  * rtol, atol = 0, the given maxiter, x0 = x (overwritten by the solution).  Returns
  * the number of completed iterations (the reference's callback count,
- * autocorrelation.py:228-246) in *iters and scipy's info in *info.  Synchronises the
- * stream once per iteration to read the FP64 residual norm. */
+ * autocorrelation.py:228-246) in *iters and scipy's info in *info.  Iterations run in
+ * blocks of 5 (one captured hipGraph replayed on the solver's stream; the iterations
+ * after convergence inside a block are switched off on the device); the host
+ * synchronises once per block to read the FP64 convergence state. */
 int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
                       double phi, int maxiter, double rtol, int* iters, int* info, void* stream);
 

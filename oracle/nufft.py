@@ -40,6 +40,8 @@ def lib():
         L.oracle_spread.argtypes = [ci, ci, cd, i64, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_nudft2.argtypes = [ci, dp, dp, i64, dp, dp, dp, dp, dp]
         L.oracle_nudft1.argtypes = [ci, i64, dp, dp, dp, dp, dp, dp, dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_nudft1_modes.argtypes = [i64, dp, dp, dp, dp, dp, ci, ip, ci, ip, ci, ip, dp, dp]
         L.oracle_num_threads.restype = ci
         _LIB = L
     return _LIB
@@ -115,6 +117,24 @@ def nudft1(c, x, y, z, M):
     x, y, z = _f64(x).ravel(), _f64(y).ravel(), _f64(z).ravel()
     fr, fi = np.empty((M, M, M)), np.empty((M, M, M))
     lib().oracle_nudft1(M, x.size, _p(x), _p(y), _p(z), _p(cr), _p(ci), _p(fr), _p(fi))
+    return fr + 1j * fi
+
+
+def nudft1_modes(c, x, y, z, ka, kb, kc):
+    """Exact type-1 sums (C, FP64) at the tensor product of mode numbers ka x kb x kc
+    (k = array index - floor(M/2)); returns complex128 (len(ka), len(kb), len(kc)).
+    Linear in the points, so usable at tens of millions of points."""
+    c = np.asarray(c).ravel()
+    cr = _f64(c.real)
+    ci = _f64(c.imag) if np.iscomplexobj(c) else None
+    x, y, z = _f64(x).ravel(), _f64(y).ravel(), _f64(z).ravel()
+    ks = [np.ascontiguousarray(k, dtype=np.int32) for k in (ka, kb, kc)]
+    assert all(1 <= k.size <= 256 for k in ks)
+    fr, fi = np.empty((ks[0].size, ks[1].size, ks[2].size)), np.empty((ks[0].size, ks[1].size, ks[2].size))
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib().oracle_nudft1_modes(x.size, _p(x), _p(y), _p(z), _p(cr), _p(ci) if ci is not None else None,
+                              ks[0].size, ks[0].ctypes.data_as(ip), ks[1].size, ks[1].ctypes.data_as(ip),
+                              ks[2].size, ks[2].ctypes.data_as(ip), _p(fr), _p(fi))
     return fr + 1j * fi
 
 

@@ -222,6 +222,48 @@ void oracle_nudft1(int M, int64_t N, const double *x, const double *y, const dou
     free(eyr); free(eyi); free(ezr); free(ezi);
 }
 
+/* Exact type-1 sums at a tensor product of selected modes: out[(a*nb + b)*nc + c] =
+ * sum_j c_j exp(+i (ka[a] x_j + kb[b] y_j + kc[c] z_j)) for integer mode numbers ka, kb,
+ * kc (k = i - floor(M/2) for array index i).  Each point costs na + nb + nc sincos
+ * evaluations and na nb nc complex products, which keeps exact checks of a few hundred
+ * modes affordable at tens of millions of points (the full-grid sum above is O(N M^3)). */
+void oracle_nudft1_modes(int64_t N, const double *x, const double *y, const double *z,
+                         const double *c_re, const double *c_im, int na, const int *ka, int nb,
+                         const int *kb, int nc, const int *kc, double *out_re, double *out_im) {
+    const int nt = na * nb * nc;
+    for (int t = 0; t < nt; ++t) out_re[t] = out_im[t] = 0.0;
+#pragma omp parallel
+    {
+        double *ar = calloc((size_t)nt, sizeof(double)), *ai = calloc((size_t)nt, sizeof(double));
+        double ex_r[MAXW * 8], ex_i[MAXW * 8], ey_r[MAXW * 8], ey_i[MAXW * 8], ez_r[MAXW * 8], ez_i[MAXW * 8];
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < N; ++j) {
+            for (int a = 0; a < na; ++a) { ex_r[a] = cos(ka[a] * x[j]); ex_i[a] = sin(ka[a] * x[j]); }
+            for (int b = 0; b < nb; ++b) { ey_r[b] = cos(kb[b] * y[j]); ey_i[b] = sin(kb[b] * y[j]); }
+            for (int c = 0; c < nc; ++c) { ez_r[c] = cos(kc[c] * z[j]); ez_i[c] = sin(kc[c] * z[j]); }
+            const double vr = c_re[j], vi = c_im ? c_im[j] : 0.0;
+            for (int a = 0; a < na; ++a) {
+                const double pr = vr * ex_r[a] - vi * ex_i[a], pi = vr * ex_i[a] + vi * ex_r[a];
+                for (int b = 0; b < nb; ++b) {
+                    const double qr = pr * ey_r[b] - pi * ey_i[b], qi = pr * ey_i[b] + pi * ey_r[b];
+                    double *dr = ar + ((int64_t)a * nb + b) * nc, *di = ai + ((int64_t)a * nb + b) * nc;
+                    for (int c = 0; c < nc; ++c) {
+                        dr[c] += qr * ez_r[c] - qi * ez_i[c];
+                        di[c] += qr * ez_i[c] + qi * ez_r[c];
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        for (int t = 0; t < nt; ++t) {
+            out_re[t] += ar[t];
+            out_im[t] += ai[t];
+        }
+        free(ar);
+        free(ai);
+    }
+}
+
 int oracle_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

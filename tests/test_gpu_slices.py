@@ -208,3 +208,20 @@ def test_tile_route_small_grid():
     o3 = _slice_route(sl, q, 3)[0]
     want = OA.compute_slices(q.cpu().numpy(), s, ext, ac, eps=1e-12)
     assert rel_l2(o3.cpu().numpy().ravel(), want) < 1e-5
+
+
+def test_two_slicers_of_one_size_keep_their_volumes():
+    # Slicers of the same M share one cached plan; interleaved calls must each slice
+    # their own autocorrelation (align.py:39-65 slices the ac it is given)
+    M, side = 21, 32
+    s, ext = synth.detector(side)
+    acs = [synth.gaussian_ac(M, 40, seed=31), synth.gaussian_ac(M, 40, seed=32)]
+    q = synth.random_quaternions(8, seed=33)
+    say("two slicers sharing a plan")
+    a = alignment.Slicer(acs[0], s, ext)
+    b = alignment.Slicer(acs[1], s, ext)
+    got_a, got_b = a.slice(q).cpu().numpy(), b.slice(q).cpu().numpy()
+    got_a2 = a.slice(q).cpu().numpy()
+    for ac, got in ((acs[0], got_a), (acs[1], got_b), (acs[0], got_a2)):
+        want = OA.compute_slices(q, s, ext, ac, exact=True).reshape(8, -1)
+        assert rel_l2(got, want) < 1e-5
