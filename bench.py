@@ -5,13 +5,17 @@ autocorrelation (1000 atoms) on an 81^3 grid, 5000 images of 128 x 128 pixels
 (16384 q-vectors, one flat panel) and 10000 reference orientations.  A step is one
 orientation-matching pass: prepare the fine grid from the autocorrelation, slice all
 10000 references (type-2 NUFFT at 163.84 M rotated pixels), per-batch std scaling,
-the weighted distance cross term on MFMA and the per-image argmin.  The headline
-value is images aligned per second (5000 images per step).
+the weighted distance cross term on MFMA and the per-image argmin, in FP32 (the
+workload's stated precision).  The headline value is images aligned per second
+(5000 images per step) over --steps timed steps.
 
-Also measured in the same run (printed in the JSON line): the FP32-plain form of
-the step, type-2 and type-1 NUFFT throughput (Mpts/s), the CG iteration time of the
-autocorrelation solve, a roofline entry for the dominant kernel and a CPU baseline
-(the oracle on a bounded sample, on this host's cores).
+Also measured in the same run (printed in the JSON line): the BF16 form of the step
+(BF16 cross term, FP32 top-8 rescoring) over the same number of steps and its ratio to
+the FP32 form, one full reconstruction generation (matching + autocorrelation solve
+with 20 CG iterations), type-2 and type-1 NUFFT throughput (Mpts/s), the CG iteration
+time and the FFT passes' bandwidth, roofline entries for the dominant kernel and the
+slicing kernel, and a CPU baseline (the oracle on a bounded sample, on this host's
+cores).
 """
 import argparse
 import ctypes
@@ -55,12 +59,16 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    p.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16"],
+                   help="precision of the headline step (the workload's own: FP32 distances and argmin); the "
+                        "other form is timed beside it")
     p.add_argument("--nbatches", type=int, default=1)
     p.add_argument("--no-extras", action="store_true", help="time the step only")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--part", default="all", choices=["all", "slice", "nufft", "traffic", "solve", "autograd", "c4", "c5"],
-                   help="slice: one slicing pass of the workload (for counter runs); nufft: setpts / execute "
+    p.add_argument("--part", default="all",
+                   choices=["all", "slice", "match", "nufft", "traffic", "solve", "autograd", "c4", "c5"],
+                   help="slice: one slicing pass of the workload (for counter runs); match: --steps matching "
+                        "steps of the workload at --precision after one untimed step; nufft: setpts / execute "
                         "timing of both transforms; traffic: a 1 GiB copy (counter calibration) then one slicing "
                         "pass; autograd: forward + backward of the differentiable slice (2048 orientations); c4 / "
                         "c5: matching throughput at the larger workloads (see large_main)")
@@ -183,21 +191,30 @@ def main():
         kernel_ms["route"] = sl.plan.slice_route()[1]
         return el, q
 
-    log(f"[bench] warmup {args.warmup}, timing {args.steps} steps ({args.precision})")
-    el, q = timed(args.precision, args.steps, args.warmup)
-    ms_step = el / args.steps * 1e3
-    value = n_img * args.steps / el
-    slice_ms, gemm_ms = slice_t.mean_ms(), gemm_t.mean_ms()
-    topk_ms = topk_t.mean_ms() if topk_t.pairs else 0.0
-    log(f"[bench] {value:.1f} images/s, {ms_step:.2f} ms/step (slice {slice_ms:.2f} ms, cross term {gemm_ms:.2f} ms,"
-        f" top-k {topk_ms:.2f} ms)")
+    # the configuration's own precision (FP32 distances and argmin) is the headline; the
+    # BF16 form runs the same step over the same number of steps beside it
+    other = "bf16" if args.precision == "fp32" else "fp32"
+    forms = {}
+    for prec in (args.precision, other):
+        log(f"[bench] {prec}: warmup {args.warmup}, timing {args.steps} steps")
+        el, q = timed(prec, args.steps, args.warmup)
+        forms[prec] = {"el": el, "q": q, "ms_per_step": el / args.steps * 1e3, "images_per_s": n_img * args.steps / el,
+                       "slice_ms": slice_t.mean_ms(), "cross_ms": gemm_t.mean_ms(),
+                       "topk_ms": topk_t.mean_ms() if topk_t.pairs else 0.0,
+                       "slice_kernel_ms": kernel_ms["slice_kernel"], "route": kernel_ms["route"]}
+        f = forms[prec]
+        log(f"[bench] {prec}: {f['images_per_s']:.1f} images/s, {f['ms_per_step']:.2f} ms/step (slice "
+            f"{f['slice_ms']:.2f} ms, cross term {f['cross_ms']:.2f} ms, top-k {f['topk_ms']:.2f} ms)")
+        if args.no_extras:
+            break
+    head = forms[args.precision]
+    value, ms_step = head["images_per_s"], head["ms_per_step"]
 
-    # roofline of the dominant kernel
     N = n_ref * P
     n_fine = sl.plan.n_fine
-    route = kernel_ms["route"]
-    k_ms = kernel_ms["slice_kernel"]
-    if k_ms >= gemm_ms:
+
+    def slice_roofline(f):
+        route, k_ms = f["route"], f["slice_kernel_ms"]
         kname = {1: "slice_real_kernel", 2: "binned points route", 3: "slice_tiles_kernel"}[route]
         n_tiles = n_ref * ((P + 63) // 64)
         bytes_launch = 4 * N + 24 * P + 16 * n_ref + 4 * n_fine ** 3 + (4 * n_tiles if route == 3 else 0)
@@ -207,7 +224,7 @@ def main():
                 if args.config == "c2" and args.nbatches == 1 else None,
                 "per_unit": "4 B per rotated pixel + 24 B per pixel + 16 B per orientation + fine grid"
                             + (" + 4 B per tile" if route == 3 else ""),
-                "avg_ms": k_ms, "slicing_call_ms": slice_ms}
+                "bytes_per_launch": bytes_launch, "avg_ms": k_ms, "slicing_call_ms": f["slice_ms"]}
         if route == 3 and sl.plan.w == 7:
             # the kernel's real limit: LDS reads, 49 rows of 7 taps per pixel read as
             # b128 + b64 + b32 (28 B), against this GPU's conflict-free b128 read rate
@@ -217,40 +234,79 @@ def main():
                            "frac": lds_bytes / (k_ms * 1e-3) / 1e12 / lds_peak, "bytes_per_launch": lds_bytes,
                            "per_unit": "49 x 28 B of LDS reads per rotated pixel",
                            "peak_source": "cmtip_lds_read_probe: conflict-free ds_read_b128 on this GPU"}
-    else:
-        flops = 2.0 * n_img * n_ref * P
-        peak = BF16_PEAK_TFS if args.precision == "bf16" else FP32_MFMA_PEAK_TFS
-        achieved = flops / (gemm_ms * 1e-3) / 1e12
-        roof = {"kernel": "hipBLASLt bf16 GEMM" if args.precision == "bf16" else "match_argmin_f32_kernel",
-                "slicing_kernel_ms": k_ms,
-                "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": None, "per_unit": "2 flop per (image, reference, pixel)", "avg_ms": gemm_ms}
+        return roof
+
+    def cross_roofline(prec, f):
+        flops = 2.0 * n_img * (n_ref // args.nbatches) * P
+        peak = BF16_PEAK_TFS if prec == "bf16" else FP32_MFMA_PEAK_TFS
+        achieved = flops / (f["cross_ms"] * 1e-3) / 1e12
+        kname = "match_argmin_f32_kernel" if prec == "fp32" else "hipBLASLt bf16 GEMM (cmtip_match_cross_bf16)"
+        return {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": recorded_traffic(kname)
+                if args.config == "c2" and args.nbatches == 1 else None,
+                "per_unit": "2 flop per (image, reference, pixel)", "flop_per_launch": flops,
+                "avg_ms": f["cross_ms"]}
+
+    # roofline of the step's dominant kernel (the longer of the cross term and the slicing)
+    roof = (cross_roofline(args.precision, head) if head["cross_ms"] >= head["slice_kernel_ms"]
+            else slice_roofline(head))
 
     result = {
         "metric": "images aligned/s + type-1/2 NUFFT Mpts/s at M=81; BF16/autograd vs plain FP32",
         "value": value, "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 slices; bf16 MFMA cross term + fp32 top-8 rescoring" if args.precision == "bf16"
-        else "fp32 (fp32 MFMA cross term)",
+        "dtype": "fp32" if args.precision == "fp32" else "bf16 cross term (fp32 accumulation) + fp32 top-8 rescoring",
         "data": "synthetic (seeded Gaussian-atom autocorrelation, flat-panel detector, Poisson images)",
-        "config": {"workload": f"orientation matching at M={M}: {n_img} images x {n_ref} references x {P} pixels",
+        "config": {"workload": f"orientation matching at M={M}: {n_img} images x {n_ref} references x {P} pixels, "
+                               f"{args.precision} distances + argmin",
                    "M": M, "n_images": n_img, "n_references": n_ref, "pixels": P, "nbatches": args.nbatches,
                    "precision": args.precision, "nufft_eps": nufft.DEFAULT_EPS},
-        "breakdown_ms": {"slice": slice_ms, "cross_term": gemm_ms, "topk": topk_ms},
+        "breakdown_ms": {"slice": head["slice_ms"], "cross_term": head["cross_ms"], "topk": head["topk_ms"]},
         "roofline": roof,
+        "roofline_slice": slice_roofline(head),
     }
 
     if not args.no_extras:
-        other = "fp32" if args.precision == "bf16" else "bf16"
-        log(f"[bench] {other} form, 2 steps")
-        el2, q2 = timed(other, 2, 1)
-        result[f"{other}_images_per_s"] = n_img * 2 / el2
-        # the BF16 form against the plain FP32 form of the same step (same slicing)
-        result["bf16_over_fp32_form"] = (value / result["fp32_images_per_s"] if other == "fp32"
-                                         else result["bf16_images_per_s"] / value)
-        agree = float((q2 == q).all(dim=1).float().mean())
-        result["assignment_agreement_fp32_vs_bf16"] = agree
+        f = forms[other]
+        result[other] = {"images_per_s": f["images_per_s"], "ms_per_step": f["ms_per_step"], "steps": args.steps,
+                         "warmup": args.warmup,
+                         "breakdown_ms": {"slice": f["slice_ms"], "cross_term": f["cross_ms"], "topk": f["topk_ms"]},
+                         "roofline_cross_term": cross_roofline(other, f)}
+        if other == "fp32":
+            result["roofline_cross_term_fp32"] = result[other]["roofline_cross_term"]
+        bf, fp = forms["bf16"], forms["fp32"]
+        # the BF16 form against the plain FP32 form of the same step (same slicing), and
+        # of the cross term alone; the dense MFMA peaks bound the latter at 2500 / 157.3
+        result["bf16_over_fp32_form"] = bf["images_per_s"] / fp["images_per_s"]
+        result["bf16_over_fp32_cross_term"] = fp["cross_ms"] / max(bf["cross_ms"] + bf["topk_ms"], 1e-9)
+        result["bf16_over_fp32_peak_bound"] = BF16_PEAK_TFS / FP32_MFMA_PEAK_TFS
+        result["assignment_agreement_fp32_vs_bf16"] = float((bf["q"] == fp["q"]).all(dim=1).float().mean())
         sl.plan.slice_real = plan_slice
+        # one generation of the reconstruction loop (reconstruct.py:72-87): FP32 matching of
+        # the images against the references, then the autocorrelation solve from the assigned
+        # orientations (kernel spread at 2M modes, rhs spread at M, support from the current
+        # estimate, 20 CG iterations, Tikhonov weight 1e-4)
+        log("[bench] full iteration: match + solve_ac")
+
+        def generation():
+            qa = step(args.precision)
+            acs = AC.solve_ac(1, s_d, ext, imgs.reshape(n_img, 1, P), M, orientations=qa, ac_estimate=ac_d.clone(),
+                              maxiter=20, rtol=0.0, rlambda=1e-4)
+            return acs
+        s_d = torch.from_numpy(s).to(dev)
+        generation()
+        torch.cuda.synchronize()
+        reps = 3
+        t = time.perf_counter()
+        for _ in range(reps):
+            generation()
+        torch.cuda.synchronize()
+        t_gen = (time.perf_counter() - t) / reps
+        result["full_iteration"] = {"ms": t_gen * 1e3, "match_ms": ms_step, "solve_ms": t_gen * 1e3 - ms_step,
+                                    "images_per_s": n_img / t_gen, "reps": reps, "cg_iterations": 20,
+                                    "note": f"{args.precision} matching step + solve_ac (both type-1 spreads over "
+                                            f"{n_img * P} points, support, 20 CG iterations); warm plans"}
+        log(f"[bench] full iteration {result['full_iteration']}")
         # type-2 with coordinates in HBM (the image q-vectors, 81.92 M points) and type-1
         log("[bench] NUFFT throughput")
         x, y, z = AC.gen_nonuniform_positions(torch.from_numpy(q_img).to(dev), torch.from_numpy(s).to(dev),
@@ -315,6 +371,25 @@ def main():
             solver.cg(d, xx, 1.0, rl, 1e3, 20, 0.0)
         tcg = tm(cg20, reps=2)
         result["solve"] = {"cg_iterations": 20, "cg_ms_per_iteration": tcg / 20 * 1e3}
+        # the FFT axis passes: one application of the Toeplitz operator (core_problem_convolution,
+        # five axis_pass_kernel launches over the L^3 embedding) timed with HIP events
+        u = torch.rand(M ** 3, device=dev)
+        solver.conv(u)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 20
+        ev[0].record()
+        for _ in range(reps):
+            solver.conv(u)
+        ev[1].record()
+        torch.cuda.synchronize()
+        t_conv = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+        cb = conv_pass_bytes(M, solver.L)
+        result["fft"] = {"kernel": "axis_pass_kernel", "operator": "core_problem_convolution", "M": M, "L": solver.L,
+                         "passes": 5, "ms": t_conv * 1e3, "bytes_per_application": cb,
+                         "achieved_GBps": cb / t_conv / 1e9, "frac_hbm": cb / t_conv / 1e9 / HBM_PEAK_GBS,
+                         "per_unit": "each pass reads and writes its lines once (8 B per complex element, 4 B per "
+                                     "real one) plus the spectrum (4 L^3) and the support (4 M^3, twice)"}
+        log(f"[bench] fft {result['fft']}")
         # differentiable slicing: forward + backward for 2048 orientations x P pixels
         from cmtip_amd import autograd as AG
         nb = min(2048, n_ref)
@@ -377,6 +452,32 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         ms = (time.perf_counter() - t) / args.steps * 1e3
         res = {"part": "autograd", "M": M, "orientations": nb, "points": nb * side * side, "fwd_bwd_ms": ms}
         log(f"[bench] autograd {res}")
+        print(json.dumps(res), flush=True)
+        return
+    if args.part == "match":
+        from cmtip_amd.alignment import align as AL
+        P = side * side
+        s_d = s
+        ref = torch.from_numpy(synth.random_quaternions(n_ref, seed=0)).to(dev)
+        q_img = synth.random_quaternions(n_img, seed=1)
+        clean = alignment.Slicer(ac, s, ext).slice(q_img).clamp_(min=0)
+        torch.manual_seed(2)
+        imgs = torch.poisson(clean * (10.0 / float(clean.mean())))
+        del clean
+
+        def one():
+            m = AL._Matcher(s_d, imgs, 0, args.precision, 8)
+            return m.run(AL.Slicer(ac, s_d, ext), ref, args.nbatches)
+        one()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            one()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        res = {"part": "match", "precision": args.precision, "M": M, "images": n_img, "references": n_ref,
+               "pixels": P, "ms_per_step": ms, "images_per_s": n_img / ms * 1e3}
+        log(f"[bench] match {res}")
         print(json.dumps(res), flush=True)
         return
     if args.part in ("slice", "traffic"):
@@ -442,6 +543,19 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
     print(json.dumps({"part": "nufft", "M": M, **res}), flush=True)
 
 
+def conv_pass_bytes(M, L):
+    """Bytes the five axis passes of the Toeplitz operator move at sizes (M, L): axis 2
+    pads M -> L (real u and the support in, complex out), axis 1 likewise, axis 0
+    forward x spectrum x inverse keeps M, then the inverse passes crop back to M^3."""
+    m3, m2l, ml2, l3 = M ** 3, M * M * L, M * L * L, L ** 3
+    p1 = 4 * m3 + 4 * m3 + 8 * m2l
+    p2 = 8 * m2l + 8 * ml2
+    p3 = 8 * ml2 + 4 * l3 + 8 * ml2
+    p4 = 8 * ml2 + 8 * m2l
+    p5 = 8 * m2l + 4 * m3 + 4 * m3
+    return p1 + p2 + p3 + p4 + p5
+
+
 def _probe_best_ms(fn, dev, wg, iters):
     import torch
 
@@ -489,7 +603,8 @@ def lds_atomic_ceiling(dev):
 
 def large_main(args, dev):
     """Whole match_orientations calls at the larger workloads, BF16 cross term with top-8
-    FP32 rescoring; one warm-up call, then timed calls.  c4: M = 81 (1000 atoms), 20000
+    FP32 rescoring; one warm-up call, then timed calls; then the plain FP32 form and the
+    BF16 form on the same (at most 20000) images, for their ratio.  c4: M = 81 (1000 atoms), 20000
     images x 50000 references x 128 x 128 pixels, Poisson mean 5, log1p-scaled.  c5:
     M = 151 (5000 atoms), 100000 images x 100000 references with a random circular
     beamstop mask per image, at 128 x 128 and at 96 x 96 pixels, Poisson mean 10."""
@@ -501,7 +616,7 @@ def large_main(args, dev):
         cases, steps = [(81, 1000, 128, 20000, 50000, False)], max(1, args.steps)
     else:
         cases, steps = [(151, 5000, 128, 100000, 100000, True), (151, 5000, 96, 100000, 100000, True)], 1
-    out = {"part": args.part, "precision": "bf16", "steps": steps, "cases": []}
+    out = {"part": args.part, "precision": "bf16 (fp32 beside it)", "steps": steps, "cases": []}
     for M, atoms, side, n_img, n_ref, masked in cases:
         P = side * side
         log(f"[bench] {args.part}: M={M} side={side} images={n_img} references={n_ref} masked={masked}")
@@ -526,22 +641,41 @@ def large_main(args, dev):
         del clean
         torch.cuda.synchronize()
 
-        def call():
-            return alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref, precision="bf16",
-                                                masks=masks)
-        call()
-        torch.cuda.synchronize()
-        log(f"[bench] {args.part}: warm-up call done")
-        t = time.perf_counter()
-        for _ in range(steps):
-            call()
-        torch.cuda.synchronize()
-        el = (time.perf_counter() - t) / steps
+        def call(precision, n):
+            return alignment.match_orientations(0, s, ext, imgs[:n], ac, n_ref, ref_orientations=ref,
+                                                precision=precision, masks=None if masks is None else masks[:n],
+                                                return_index=True)[1]
+
+        def timed_call(precision, n, reps):
+            call(precision, n)
+            torch.cuda.synchronize()
+            log(f"[bench] {args.part}: {precision} warm-up call done ({n} images)")
+            t = time.perf_counter()
+            for _ in range(reps):
+                idx = call(precision, n)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t) / reps, idx
+
+        el, _ = timed_call("bf16", n_img, steps)
         K = 3 * P if masked else P
         case = {"M": M, "pixels": P, "images": n_img, "references": n_ref, "masked": masked,
                 "ms_per_call": el * 1e3, "images_per_s": n_img / el,
                 "cross_term_tflop": 2.0 * n_img * n_ref * K / 1e12,
                 "whole_call_tflop_per_s": 2.0 * n_img * n_ref * K / el / 1e12}
+        # the plain FP32 form (FP32 MFMA cross term over P columns, 2P with masks) on the
+        # first n_cmp images, and the BF16 form on the same images: the ratio where the
+        # cross term is nearly all of the work
+        n_cmp = min(n_img, 20000)
+        el32, idx32 = timed_call("fp32", n_cmp, 1)
+        el16, idx16 = timed_call("bf16", n_cmp, 1) if n_cmp != n_img else (el, call("bf16", n_cmp))
+        K32 = 2 * P if masked else P
+        case["fp32"] = {"images": n_cmp, "ms_per_call": el32 * 1e3, "images_per_s": n_cmp / el32,
+                        "cross_term_tflop": 2.0 * n_cmp * n_ref * K32 / 1e12,
+                        "whole_call_tflop_per_s": 2.0 * n_cmp * n_ref * K32 / el32 / 1e12,
+                        "bf16_ms_per_call_same_images": el16 * 1e3,
+                        "bf16_over_fp32": el32 / el16,
+                        "assignment_agreement": float((torch.as_tensor(idx32) == torch.as_tensor(idx16)).float()
+                                                      .mean())}
         log(f"[bench] {args.part}: {case}")
         out["cases"].append(case)
         del imgs, masks
