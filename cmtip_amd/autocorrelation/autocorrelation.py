@@ -270,3 +270,36 @@ def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, 
     if return_info:
         return res, {"iterations": iters, "info": info}
     return res
+
+
+def solve_ac_mult(generation, pixel_position_reciprocal, reciprocal_extent, slices_, M, n_conformations,
+                  orientations=None, conformations=None, ac_estimate=None, use_gpu=False, use_recip_sym=True, **kw):
+    """One autocorrelation per conformation (n_conformations, M, M, M), float64.
+
+    The single-GPU share of solve_ac_mpi_mult (autocorrelation_mpi.py:193-243): images
+    with no conformation assigned are given a random one (np.random.randint, :218-219),
+    then conformation nc is solved from the images and orientations assigned to it
+    (``slices_[conformations == nc]``, :234-243) with ``ac_estimate[nc]`` as its
+    estimate.  Keyword arguments are passed on to solve_ac (maxiter, rlambda, ...).
+    Orientations must be given when skopi's random draw (:222) is wanted otherwise;
+    without them solve_ac draws seeded uniform ones per conformation."""
+    n_img = slices_.shape[0]
+    if conformations is None:
+        conformations = np.random.randint(low=0, high=n_conformations, size=n_img)
+    conf = np.asarray(conformations.cpu() if isinstance(conformations, torch.Tensor) else conformations)
+    if ac_estimate is None:
+        ac_estimate = n_conformations * [None]
+    as_torch = _arrays.is_torch(slices_)
+    out = []
+    for nc in range(n_conformations):
+        sel = np.flatnonzero(conf == nc)
+        idx = torch.as_tensor(sel, device=slices_.device) if isinstance(slices_, torch.Tensor) else sel
+        ori = None
+        if orientations is not None:
+            ori = orientations[torch.as_tensor(sel, device=orientations.device)] if isinstance(
+                orientations, torch.Tensor) else np.asarray(orientations)[sel]
+        est = ac_estimate[nc]
+        res = solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_[idx], M, orientations=ori,
+                       ac_estimate=est, use_gpu=use_gpu, use_recip_sym=use_recip_sym, **kw)
+        out.append(torch.as_tensor(res, dtype=torch.float64) if as_torch else np.asarray(res, dtype=np.float64))
+    return torch.stack(out) if as_torch else np.stack(out)

@@ -159,3 +159,39 @@ def test_cg_edge_cases():
     x = torch.zeros_like(d)
     it, info = W.solver.cg(d, x, 1.0, rl, 1e3, 3, 1e-12)
     assert (it, info) == (3, 3)  # loop exhausted: info = maxiter
+
+
+def test_solve_ac_mult_per_conformation():
+    # solve_ac_mpi_mult (autocorrelation_mpi.py:193-243) on one GPU: conformation nc is
+    # solved from slices_[conformations == nc] with its own orientations and estimate
+    M, n_side = 11, 16
+    s, ext = synth.detector(n_side)
+    acs = [synth.gaussian_ac(M, 20, seed=41), synth.gaussian_ac(M, 20, seed=42)]
+    q = synth.random_quaternions(90, seed=43)
+    conf = np.random.default_rng(44).integers(0, 2, 90)
+    x, y, z = geom.rotated_coords(q, s, ext)
+    P = n_side * n_side
+    data = np.stack([O.nufft2(acs[c].astype(np.float64), x[i * P:(i + 1) * P], y[i * P:(i + 1) * P],
+                              z[i * P:(i + 1) * P], eps=1e-12).real for i, c in enumerate(conf)])
+    data = data.reshape(90, 1, P).astype(np.float32)
+    est = [acs[0].astype(np.float64), None]
+    say("solve_ac_mult M=11, two conformations")
+    got = A.solve_ac_mult(0, s, ext, data, M, 2, orientations=q, conformations=conf,
+                          ac_estimate=[None if e is None else e.copy() for e in est])
+    assert got.shape == (2, M, M, M) and got.dtype == np.float64
+    for nc in range(2):
+        sel = conf == nc
+        alone = A.solve_ac(0, s, ext, data[sel], M, orientations=q[sel],
+                           ac_estimate=None if est[nc] is None else est[nc].copy())
+        # the same solve as solve_ac on the subset; the spreads' float-atomic flushes
+        # make repeated solves differ in the last bits, which W's conditioning amplifies
+        assert rel_l2(got[nc], alone) < 1e-3
+        want, its, sup, d, F = OAC.solve(q[sel], s, ext, data[sel], M, ac_estimate=est[nc])
+        anti = OAC.gen_F_antisupport(M)
+        rl = M ** 3 / data[sel].size / 1000
+
+        def W(u):
+            return (OAC.core_problem_convolution(u, M, F, 2 * M, sup) + rl * u
+                    + 1e3 * OAC.fourier_reg(u, sup, anti, M))
+        assert np.linalg.norm(W(got[nc].ravel()) - d) / np.linalg.norm(d) < 5e-5
+        assert rel_l2(got[nc], want) < 1e-3
