@@ -52,6 +52,7 @@ SIGNATURES = {
     "cmtip_moments": [_vp, _i64, _vp, _vp],
     "cmtip_match_prepare_images": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_image_stats": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
+    "cmtip_image_stats_noalloc": [_vp, _i64, _i64, _vp, _vp, _vp],
     "cmtip_match_gamma": [_vp, ctypes.c_double, _vp, ctypes.c_double, _i64, _vp, _vp],
     "cmtip_center_rows": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cmtip_match_prepare_images_centered": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],

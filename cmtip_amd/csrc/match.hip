@@ -920,6 +920,68 @@ extern "C" int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double
     return CMTIP_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free image statistics
+//
+// cmtip_image_stats with its column-sum workspace in a file-scope device array instead of
+// a caller buffer, for callers that cannot allocate device memory.  In host code the
+// array's name is only a stand-in for the symbol: its device address is looked up once
+// per device with hipGetSymbolAddress and only that address is passed to kernels.
+// Every launch overwrites the workspace entirely (column_sums_kernel writes each of its
+// 2P entries), so nothing needs clearing between calls.
+
+namespace {
+constexpr int64_t kStatsScratchPixels = 262144;  // 2 x 2^18 doubles = 4 MiB per device
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double image_stats_scratch[2 * kStatsScratchPixels];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's copy of image_stats_scratch, looked up on first use
+// with the device current.
+int stats_scratch(double** out) {
+    int dev = -1;
+    CMTIP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("image_stats_noalloc: device ordinal %d out of range", dev);
+        return CMTIP_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CMTIP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(image_stats_scratch)));
+        if (!p) {
+            set_error("image_stats_noalloc: no device address for the scratch array");
+            return CMTIP_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CMTIP_OK;
+}
+}  // namespace
+
+extern "C" int cmtip_image_stats_noalloc(const float* X, int64_t rows, int64_t P, float* c, double* mom,
+                                         void* stream) {
+    if (rows < 0 || P < 1 || (rows > 0 && !X)) {
+        set_error("image_stats_noalloc: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (P > kStatsScratchPixels) {
+        set_error("image_stats_noalloc: P = %lld pixels exceeds the workspace limit of %lld", (long long)P,
+                  (long long)kStatsScratchPixels);
+        return CMTIP_ERR_ARG;
+    }
+    double* colsum = nullptr;
+    CMTIP_TRY(stats_scratch(&colsum));
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(column_sums_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64 * kStatWaves), 0, st, X, rows, P,
+                       colsum);
+    hipLaunchKernelGGL(finish_stats_kernel, dim3(1), dim3(kFinishThreads), 0, st, (const double*)colsum, rows, P, c, mom);
+    CMTIP_LAUNCH_CHECK();
+    return CMTIP_OK;
+}
+
 extern "C" int cmtip_match_gamma(const double* img_mom, double img_count, const double* model_mom,
                                  double model_count, int64_t n, double* gamma, void* stream) {
     if (!img_mom || !model_mom || !(model_count > 0.0) || n < 1 || !gamma) {

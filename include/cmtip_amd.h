@@ -202,6 +202,15 @@ int cmtip_slice_timer(cmtip_nufft_plan plan, int start, float* total_ms, int* co
  * FP64 with a fixed summation order. */
 int cmtip_image_stats(const float* X, int64_t rows, int64_t P, double* colsum, float* c, double* mom,
                       void* stream);
+/* cmtip_image_stats without the colsum workspace argument: the 2P doubles live in a
+ * file-scope device array of the library (4 MiB per device, held until the process
+ * ends), so the call allocates nothing and needs no caller workspace.  No counterpart
+ * in the reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  P at most
+ * 262144 pixels (CMTIP_ERR_ARG beyond).  All calls on one GPU share the array, so calls
+ * that overlap on one GPU (other streams, other threads) race; calls ordered on one
+ * stream do not.  Same kernels and results as cmtip_image_stats. */
+int cmtip_image_stats_noalloc(const float* X, int64_t rows, int64_t P, float* c, double* mom, void* stream);
 /* Data/model standard-deviation ratio of one reference batch (align.py:122-124: both
  * population stds, the ratio rounded to float32 as numpy keeps it), written to the n
  * doubles of gamma.  img_mom = (sum, sum of squares[, count]) of the images with
