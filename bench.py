@@ -408,6 +408,8 @@ def main():
         result["autograd"] = {"orientations": nb, "points": nb * P, "fwd_bwd_ms_l2": t_l2 * 1e3,
                               "fwd_bwd_ms_poisson": t_pn * 1e3, "Mpts_s_l2": nb * P / t_l2 / 1e6}
         log(f"[bench] autograd {result['autograd']}")
+        result["phasing"] = bench_phasing(ac, args.no_cpu_baseline)
+        log(f"[bench] phasing {result['phasing']}")
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ac, s, ext, ref, imgs.cpu().numpy(), n_img, n_ref, P)
@@ -712,6 +714,64 @@ def cpu_baseline(ac, s, ext, ref, imgs, n_img, n_ref, P):
                       f"and sklearn euclidean_distances + argmin of those {n_s} x {m_s} images, extrapolated "
                       f"linearly to {n_ref} references x {n_img} images",
             "threads_openmp": O.num_threads(), "solve_parts": parts}
+
+
+def phase_step_bytes(M):
+    """HBM bytes of one ER / HIO step's five axis passes on an M^3 volume: real density
+    in (4 B) and complex out (8 B); two complex read + write passes; the amplitude pass
+    (complex in and out plus 4 B of amplitudes); the last pass reads the complex volume,
+    the density and the support and writes the density."""
+    m3 = M ** 3
+    return m3 * ((4 + 8) + 16 + (8 + 4 + 8) + 16 + (8 + 4 + 4 + 4))
+
+
+def bench_phasing(ac, no_cpu):
+    """phase() (phasing/phase.py:165-219) with the reference's schedule: 10 x (50 ER +
+    25 HIO + 50 ER + shrink-wrap) + 50 ER, then recentring and the phased
+    autocorrelation, on the benchmark's autocorrelation; one untimed call, then one
+    timed.  CPU: the oracle (numpy complex128 FFTs) over 25 steps, extrapolated."""
+    import torch
+
+    from cmtip_amd import phasing
+
+    M = ac.shape[0]
+    np.random.seed(0)
+    phasing.phase(0, ac, n_iterations=1, nER=2, nHIO=2)
+    torch.cuda.synchronize()
+    np.random.seed(0)
+    t = time.perf_counter()
+    acp, sup, rho = phasing.phase(0, ac)
+    el = time.perf_counter() - t
+    steps = 10 * (50 + 25 + 50) + 50
+    # the ER / HIO steps alone, timed with HIP events
+    ph = phasing.get_phaser(M)
+    r = torch.from_numpy(rho).float().cuda()
+    sp = torch.from_numpy(sup.astype(np.float32)).cuda()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    ph.iterate(r, sp, 1, 100, 0.0, np.inf)
+    ev[1].record()
+    torch.cuda.synchronize()
+    step_ms = ev[0].elapsed_time(ev[1]) / 100
+    b = phase_step_bytes(M)
+    out = {"M": M, "schedule": "10 x (50 ER + 25 HIO + 50 ER + shrink-wrap) + 50 ER", "steps": steps,
+           "phase_ms": el * 1e3, "step_ms": step_ms, "step_bytes": b, "step_GBps": b / (step_ms * 1e-3) / 1e9,
+           "step_frac_hbm": b / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "ac_phased_cc_vs_input": float(np.corrcoef(acp.ravel(), np.asarray(ac, np.float64).ravel())[0, 1])}
+    if not no_cpu:
+        from oracle import phase as OP
+        ac_filt_, amp, mask = OP.amplitudes_of(np.asarray(ac, np.float64))
+        sup0 = OP.create_support_(ac_filt_, M, int((M - 1) / 4))
+        rho0 = sup0 * np.random.default_rng(0).random(sup0.shape)
+        n = 25
+        t = time.perf_counter()
+        for _ in range(n):
+            OP.ER(rho0, amp, mask, sup0, np.inf)
+        per = (time.perf_counter() - t) / n
+        out["cpu_baseline"] = {"step_ms": per * 1e3, "phase_ms_extrapolated": per * steps * 1e3, "cores": 1,
+                               "kind": "port", "sample": f"oracle ER steps (numpy complex128 FFTs) x {n}, "
+                                                         f"extrapolated to {steps} steps"}
+    return out
 
 
 def cpu_solve_parts(M, s, ext, n_o):

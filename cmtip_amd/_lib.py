@@ -53,6 +53,15 @@ SIGNATURES = {
     "cmtip_match_prepare_images": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_image_stats": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_image_stats_noalloc": [_vp, _i64, _i64, _vp, _vp, _vp],
+    "cmtip_phaser_create": [_i, ctypes.POINTER(_vp)],
+    "cmtip_phaser_destroy": [_vp],
+    "cmtip_phase_setup": [_vp, _vp, _vp, _vp, _i, _vp],
+    "cmtip_phase_get_amplitudes": [_vp, _vp, _vp],
+    "cmtip_phase_set_amplitudes": [_vp, _vp, _vp],
+    "cmtip_phase_iterate": [_vp, _vp, _vp, _i, _i, _d, _d, _vp],
+    "cmtip_phase_shrink_wrap": [_vp, _vp, _d, _d, _vp, _vp],
+    "cmtip_phase_recenter": [_vp, _vp, _vp, _pi, _vp],
+    "cmtip_phase_ac": [_vp, _vp, _vp, _vp],
     "cmtip_match_gamma": [_vp, ctypes.c_double, _vp, ctypes.c_double, _i64, _vp, _vp],
     "cmtip_center_rows": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cmtip_match_prepare_images_centered": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],

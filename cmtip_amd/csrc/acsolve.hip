@@ -83,8 +83,8 @@ __global__ void ifftshift3_kernel(const float* __restrict__ in, float* __restric
 // ---------------------------------------------------------------- support filter
 // scipy.ndimage.gaussian_filter(ac, 0.5): radius int(4 * 0.5 + 0.5) = 2, weights
 // exp(-x^2 / (2 sigma^2)) normalised, mode 'reflect' (edge sample repeated), one axis
-// at a time; symmetric weights are applied as w0 x[i] + sum_j wj (x[i-j] + x[i+j]),
-// the order of scipy's correlate1d.  round_f32 rounds after each axis (float32 input).
+// at a time; symmetric weights are applied as w0 x[i] + sum_j wj (x[i-j] + x[i+j]) with
+// the farthest pair first, the order of scipy's correlate1d.  round_f32 rounds after each axis (float32 input).
 __device__ __forceinline__ int reflect(int i, int M) {
     // half-sample symmetric reflection, valid for |overhang| <= M
     if (i < 0) i = -i - 1;
@@ -104,8 +104,8 @@ __global__ void gauss_axis_kernel(const double* __restrict__ in, double* __restr
     const int64_t base = t - (int64_t)i * stride;
     auto at = [&](int j) -> double { return in[base + (int64_t)reflect(j, M) * stride]; };
     double v = at(i) * w0;
-    v += (at(i - 1) + at(i + 1)) * w1;
     v += (at(i - 2) + at(i + 2)) * w2;
+    v += (at(i - 1) + at(i + 1)) * w1;
     if (round_f32) v = (double)(float)v;
     out[t] = v;
 }

@@ -48,7 +48,7 @@ struct AxisDesc {
     const float* in_mask = nullptr;  // per-element input weight (float, input offsets)
 
     void* out = nullptr;
-    int out_mode = 0;  // 0: float2, 1: real part as float, 2: imaginary part as float
+    int out_mode = 0;  // 0: float2, 1: real part as float, 2: imaginary part as float, 3: |value| as float
     int64_t out_sa = 0, out_su = 0, out_sv = 0;
     int oa = 0, ob = 0, og1 = 0, og2 = 0;
     const float* out_w = nullptr;     // per-LDS-position output weight (length n)
@@ -56,6 +56,15 @@ struct AxisDesc {
     const float* out_mask = nullptr;  // per-element output weight (float, output offsets)
     const float* add_src = nullptr;   // real mode only: out += add_scale * add_src[off]
     float add_scale = 0.0f;
+    // Real mode only: a phase-retrieval update of the output array in place, with r the
+    // real value after weights: 1 (error reduction) out = (S > 0 and r > 0) ? r : 0;
+    // 2 (hybrid input-output) out = (S > 0 and r > 0) ? r : out - beta r; then where
+    // r > rho_max, ER sets out = rho_max and HIO adds 2 beta r - rho_max.  S is
+    // upd_support[off] (float 0 / 1).
+    int upd_mode = 0;
+    const float* upd_support = nullptr;
+    float upd_beta = 0.0f;
+    float upd_rho_max = 0.0f;
 
     int nu = 1, nv = 1;
     int sign = -1;  // -1: exp(-i...), +1: exp(+i...); both unnormalised
@@ -64,6 +73,11 @@ struct AxisDesc {
     // inverse (-sign) transform of the same line: value *= mid[u*s_u+v*s_v+p*s_a].
     const float2* mid_c = nullptr;
     const float* mid_r = nullptr;
+    // Other mid-pass maps (instead of the product): mid_amp replaces the magnitude,
+    // value -> a value / |value| with a = mid_amp[off] (a zero value becomes (a, 0)),
+    // and keeps the value where a < 0; mid_abs2 maps value -> (|value|^2, 0).
+    const float* mid_amp = nullptr;
+    int mid_abs2 = 0;
     int64_t mid_sa = 0, mid_su = 0, mid_sv = 0;
 
     // Optional device flag: when *skip != 0 the pass does nothing (lets a captured

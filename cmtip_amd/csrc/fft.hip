@@ -240,10 +240,19 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
             const uint32_t v = gl - u * (uint32_t)d.nv;
             const int64_t off = (int64_t)u * d.mid_su + (int64_t)v * d.mid_sv + (int64_t)p * d.mid_sa;
             float2 x = res[line * a.ldl + p];
-            if (d.mid_c)
+            if (d.mid_amp) {
+                const float am = d.mid_amp[off];
+                if (am >= 0.f) {
+                    const float r = sqrtf(x.x * x.x + x.y * x.y);
+                    x = r > 0.f ? cscale(x, am / r) : make_float2(am, 0.f);
+                }
+            } else if (d.mid_abs2) {
+                x = make_float2(x.x * x.x + x.y * x.y, 0.f);
+            } else if (d.mid_c) {
                 x = cmul(x, d.mid_c[off]);
-            else
+            } else {
                 x = cscale(x, d.mid_r[off]);
+            }
             res[line * a.ldl + p] = x;
         }
         __syncthreads();
@@ -272,10 +281,25 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
         float2 x = cscale(res[line * a.ldl + p], wgt);
         if (OUT_MODE == 0) {
             reinterpret_cast<float2*>(d.out)[off] = x;
+        } else if (OUT_MODE == 3) {
+            reinterpret_cast<float*>(d.out)[off] = sqrtf(x.x * x.x + x.y * x.y);
         } else {
             float y = OUT_MODE == 1 ? x.x : x.y;
             if (d.add_src) y += d.add_scale * d.add_src[off];
-            reinterpret_cast<float*>(d.out)[off] = y;
+            float* o = reinterpret_cast<float*>(d.out);
+            if (OUT_MODE == 1 && d.upd_mode) {
+                const bool keep = d.upd_support[off] > 0.f && y > 0.f;
+                float r;
+                if (d.upd_mode == 1) {
+                    r = keep ? y : 0.f;
+                    if (y > d.upd_rho_max) r = d.upd_rho_max;
+                } else {
+                    r = keep ? y : o[off] - d.upd_beta * y;
+                    if (y > d.upd_rho_max) r += 2.f * d.upd_beta * y - d.upd_rho_max;
+                }
+                y = r;
+            }
+            o[off] = y;
         }
     }
 }
@@ -340,7 +364,10 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     a.nst = plan.nst;
     const int64_t nlines = (int64_t)d.nu * d.nv;
     if (nlines <= 0) return CMTIP_OK;
-    if (d.ia + d.ib > plan.n || d.oa + d.ob > plan.n || (d.mid_c && d.mid_r)) {
+    if (d.ia + d.ib > plan.n || d.oa + d.ob > plan.n ||
+        ((d.mid_c != nullptr) + (d.mid_r != nullptr) + (d.mid_amp != nullptr) + (d.mid_abs2 != 0)) > 1 ||
+        (d.upd_mode && (d.out_mode != 1 || !d.upd_support)) || d.out_mode < 0 || d.out_mode > 3 ||
+        (d.in_real && d.out_mode == 3)) {
         set_error("fft_axis_pass: bad segments (n=%d ia=%d ib=%d oa=%d ob=%d)", plan.n, d.ia, d.ib,
                   d.oa, d.ob);
         return CMTIP_ERR_ARG;
@@ -374,7 +401,7 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     a.tw = plan.tw.as<const float2>();
     const size_t lds = (size_t)2 * lb * ldl * sizeof(float2);
     const unsigned grid = (unsigned)((nlines + lb - 1) / lb);
-    const bool mid = d.mid_c || d.mid_r;
+    const bool mid = d.mid_c || d.mid_r || d.mid_amp || d.mid_abs2;
 #define CMTIP_AXIS_LAUNCH(IR, OM, MD)                                                                       \
     do {                                                                                                    \
         static bool attr_set_ = false;                                                                      \
@@ -398,8 +425,10 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
             if (mid) CMTIP_AXIS_LAUNCH(false, 0, true); else CMTIP_AXIS_LAUNCH(false, 0, false);
         } else if (d.out_mode == 1) {
             if (mid) CMTIP_AXIS_LAUNCH(false, 1, true); else CMTIP_AXIS_LAUNCH(false, 1, false);
-        } else {
+        } else if (d.out_mode == 2) {
             if (mid) CMTIP_AXIS_LAUNCH(false, 2, true); else CMTIP_AXIS_LAUNCH(false, 2, false);
+        } else {
+            if (mid) CMTIP_AXIS_LAUNCH(false, 3, true); else CMTIP_AXIS_LAUNCH(false, 3, false);
         }
     }
 #undef CMTIP_AXIS_LAUNCH

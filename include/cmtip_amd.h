@@ -349,6 +349,47 @@ int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* out, double a
 int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
                       double phi, int maxiter, double rtol, int* iters, int* info, void* stream);
 
+/* ---------------------------------------------------------------- phasing
+ * Phase retrieval of the oversampled autocorrelation (cmtip/phasing/phase.py): error
+ * reduction, hybrid input-output, shrink-wrap and recentring on the device, every step
+ * in FP32 with complex64 FFTs (hand-written axis passes) and the Gaussian filters in
+ * FP64.  Volumes are float32 M^3; those whose names end in an underscore are in FFT
+ * position order (DC at index 0, the reference's convention, phase.py:4-9), the others
+ * centred.  Supports are float32 0 / 1.  A phaser owns the amplitudes and an M^3
+ * complex scratch volume; it is not thread-safe. */
+typedef struct cmtip_phaser_s* cmtip_phaser;
+
+int cmtip_phaser_create(int M, cmtip_phaser* out);
+int cmtip_phaser_destroy(cmtip_phaser p);
+
+/* From the centred autocorrelation ac: ac_filt = gaussian_filter(max(ac, 0),
+ * mode='constant', sigma=1, truncate=2) (FP64), the phaser's amplitudes_ =
+ * sqrt(|fftn(ifftshift(ac_filt))|) with the DC entry masked (kept by the projection)
+ * (phase.py:186-195).  ac_filt_ (may be NULL) receives ifftshift(ac_filt); support_
+ * (may be NULL) receives create_support_(ac_filt_, M, Mquat) (phase.py:47-64). */
+int cmtip_phase_setup(cmtip_phaser p, const float* ac, float* ac_filt_, float* support_, int Mquat, void* stream);
+/* Copy the amplitudes out / in (a negative entry marks a value the projection keeps,
+ * the DC entry after setup). */
+int cmtip_phase_get_amplitudes(cmtip_phaser p, float* amplitudes_, void* stream);
+int cmtip_phase_set_amplitudes(cmtip_phaser p, const float* amplitudes_, void* stream);
+/* n steps of error reduction (mode 1, ER phase.py:83-99) or hybrid input-output (mode 2,
+ * HIO phase.py:102-119, feedback beta) on rho_ in place, each with the amplitude
+ * projection of step_phase (phase.py:122-144); rho_max = +inf disables the threshold as
+ * the reference's driver does.  Mode 0 replaces rho_ by the projected density rho_mod_
+ * of step_phase itself (support_ unused, may be NULL). */
+int cmtip_phase_iterate(cmtip_phaser p, float* rho_, const float* support_, int mode, int n, double beta,
+                        double rho_max, void* stream);
+/* shrink_wrap (phase.py:147-162): support_ = gaussian_filter(|rho_|, mode='wrap', sigma,
+ * truncate=2) > max|rho_| * cutoff. */
+int cmtip_phase_shrink_wrap(cmtip_phaser p, const float* rho_, double cutoff, double sigma, float* support_,
+                            void* stream);
+/* recenter (phase.py:11-44): rolls rho_ and support_ in place so the centre of mass of
+ * |rho_| moves to the origin; shift (3 ints, may be NULL) receives the vector rolled by.
+ * Synchronises the stream once (the shift is computed on the host). */
+int cmtip_phase_recenter(cmtip_phaser p, float* rho_, float* support_, int* shift, void* stream);
+/* ac_phased = fftshift(|ifftn(|fftn(rho_)|^2)|) (phase.py:214-217), centred. */
+int cmtip_phase_ac(cmtip_phaser p, const float* rho_, float* ac_phased, void* stream);
+
 /* ---------------------------------------------------------------- measurement
  * No counterpart in the reference.  workgroups x 512 threads each read a 64 KB LDS
  * buffer iters times with conflict-free ds_read_b128 (16 B per lane per read) and
