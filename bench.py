@@ -410,6 +410,8 @@ def main():
         log(f"[bench] autograd {result['autograd']}")
         result["phasing"] = bench_phasing(ac, args.no_cpu_baseline)
         log(f"[bench] phasing {result['phasing']}")
+        result["data_layer"] = bench_data_layer(dev, args.no_cpu_baseline)
+        log(f"[bench] data layer {result['data_layer']}")
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ac, s, ext, ref, imgs.cpu().numpy(), n_img, n_ref, P)
@@ -714,6 +716,55 @@ def cpu_baseline(ac, s, ext, ref, imgs, n_img, n_ref, P):
                       f"and sklearn euclidean_distances + argmin of those {n_s} x {m_s} images, extrapolated "
                       f"linearly to {n_ref} references x {n_img} images",
             "threads_openmp": O.num_threads(), "solve_parts": parts}
+
+
+def bench_data_layer(dev, no_cpu):
+    """The data layer on the workload's image stack: bin_data of 5000 images of 256 x 256
+    pixels by 2 (prep_data.py:197-226) and clip_data of the binned stack at 12 A
+    (:155-169), HIP events over 5 calls each; CPU: the numpy oracle on 200 images."""
+    import torch
+
+    from cmtip_amd import prep_data as PD
+    from cmtip_amd import synth
+
+    n, side = 5000, 256
+    g = torch.Generator(device=dev).manual_seed(4)
+    raw = torch.poisson(torch.rand((n, 1, side * side), device=dev, generator=g) * 10, generator=g)
+    s, ext = synth.detector(side // 2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    binned = PD.bin_data(raw, 2, det_shape=(1, side, side))
+    clipped = PD.clip_data(binned, s, 12.0)
+    torch.cuda.synchronize()
+    reps = 5
+    ev[0].record()
+    for _ in range(reps):
+        PD.bin_data(raw, 2, det_shape=(1, side, side))
+    ev[1].record()
+    for _ in range(reps):
+        PD.clip_data(binned, s, 12.0)
+    ev[2].record()
+    torch.cuda.synchronize()
+    t_bin = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    t_clip = ev[1].elapsed_time(ev[2]) / reps * 1e-3
+    b_bin = 4 * raw.numel() + 4 * binned.numel()
+    b_clip = 4 * clipped.numel() * 2 + 3 * 4 * s.shape[-1]
+    out = {"bin": {"images": n, "in_pixels": side * side, "factor": 2, "ms": t_bin * 1e3, "bytes": b_bin,
+                   "GBps": b_bin / t_bin / 1e9, "frac_hbm": b_bin / t_bin / 1e9 / HBM_PEAK_GBS},
+           "clip": {"images": n, "kept_pixels": clipped.shape[-1], "of": binned.shape[-1], "ms": t_clip * 1e3,
+                    "bytes": b_clip, "GBps": b_clip / t_clip / 1e9, "frac_hbm": b_clip / t_clip / 1e9 / HBM_PEAK_GBS,
+                    "note": "time includes the pixel selection (one synchronisation for the kept count)"}}
+    if not no_cpu:
+        from oracle import prep as OPR
+        sub = raw[:200].cpu().numpy()
+        t = time.perf_counter()
+        bs = OPR.bin_data(sub, 2, det_shape=(1, side, side))
+        tb = time.perf_counter() - t
+        t = time.perf_counter()
+        OPR.clip_data(bs, s, 12.0)
+        tc = time.perf_counter() - t
+        out["cpu_baseline"] = {"bin_ms_extrapolated": tb * n / 200 * 1e3, "clip_ms_extrapolated": tc * n / 200 * 1e3,
+                               "cores": 1, "kind": "port", "sample": "numpy oracle on 200 of the 5000 images"}
+    return out
 
 
 def phase_step_bytes(M):

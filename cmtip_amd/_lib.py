@@ -62,6 +62,11 @@ SIGNATURES = {
     "cmtip_phase_shrink_wrap": [_vp, _vp, _d, _d, _vp, _vp],
     "cmtip_phase_recenter": [_vp, _vp, _vp, _pi, _vp],
     "cmtip_phase_ac": [_vp, _vp, _vp, _vp],
+    "cmtip_bin_images": [_vp, _i64, _i, _i, _i, _vp, _vp],
+    "cmtip_resolution_select": [_vp, _i64, _i64, _d, _vp, _vp, _vp, _vp],
+    "cmtip_gather_columns": [_vp, _i, _i64, _i64, _vp, _i64, _vp, _vp],
+    "cmtip_assemble_index": [_vp, _i64, _i64, _i64, _vp, _vp],
+    "cmtip_bin_index_map": [_vp, _i64, _i, _i, _i, _vp, _vp],
     "cmtip_match_gamma": [_vp, ctypes.c_double, _vp, ctypes.c_double, _i64, _vp, _vp],
     "cmtip_center_rows": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "cmtip_match_prepare_images_centered": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp],

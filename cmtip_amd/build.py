@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcmtip_amd.so")
-SOURCES = ["common.cpp", "fft.hip", "nufft.hip", "match.hip", "acsolve.hip", "phase.hip", "probe.hip"]
+SOURCES = ["common.cpp", "fft.hip", "nufft.hip", "match.hip", "acsolve.hip", "phase.hip", "prep.hip", "probe.hip"]
 ARCH = os.environ.get("CMTIP_OFFLOAD_ARCH", "gfx950")
 
 

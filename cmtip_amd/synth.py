@@ -75,3 +75,26 @@ def poisson_images(slices, mean_photons, seed=2):
     sl = np.clip(np.asarray(slices, dtype=np.float64), 0, None)
     scale = mean_photons / max(sl.mean(), 1e-30)
     return rng.poisson(sl * scale).astype(np.float32)
+
+
+def panel_detector(px=64, py=64, grid=(2, 2), gap=4, det_width=0.08, det_dist=0.2, wavelength=BEAM_WAVELENGTH_M):
+    """A flat detector of grid[0] x grid[1] panels of px x py pixels with gaps of ``gap``
+    pixels between panels, in the layout load_h5 leaves (prep_data.py:54-93):
+    (pixel_index_map (n_panels, px, py, 2) int64, pixel_position_reciprocal
+    (3, 1, n_panels * px * py) float32, det_shape (n_panels, px, py), reciprocal_extent)."""
+    n_panels = grid[0] * grid[1]
+    X = grid[0] * px + (grid[0] - 1) * gap
+    Y = grid[1] * py + (grid[1] - 1) * gap
+    pix = det_width / max(X, Y)
+    imap = np.zeros((n_panels, px, py, 2), dtype=np.int64)
+    for a in range(grid[0]):
+        for b in range(grid[1]):
+            l = a * grid[1] + b
+            imap[l, :, :, 0] = a * (px + gap) + np.arange(px)[:, None]
+            imap[l, :, :, 1] = b * (py + gap) + np.arange(py)[None, :]
+    cx = (imap[..., 0] - (X - 1) / 2.0) * pix
+    cy = (imap[..., 1] - (Y - 1) / 2.0) * pix
+    P = np.stack([cx.ravel(), cy.ravel(), np.full(cx.size, det_dist)])
+    u = P / np.linalg.norm(P, axis=0, keepdims=True)
+    s = ((u - np.array([[0.0], [0.0], [1.0]])) / wavelength).astype(np.float32).reshape(3, 1, -1)
+    return imap, s, (n_panels, px, py), np.float32(np.linalg.norm(s, axis=0).max())

@@ -390,6 +390,38 @@ int cmtip_phase_recenter(cmtip_phaser p, float* rho_, float* support_, int* shif
 /* ac_phased = fftshift(|ifftn(|fftn(rho_)|^2)|) (phase.py:214-217), centred. */
 int cmtip_phase_ac(cmtip_phaser p, const float* rho_, float* ac_phased, void* stream);
 
+/* ---------------------------------------------------------------- data layer
+ * Which pixels enter the path and at what binning (cmtip/prep_data.py:96-262).
+ * Copy / integer work bound by HBM. */
+
+/* bin_data (prep_data.py:197-226): out[s, i, j] = mean of in[s] over the b x b block at
+ * (b i, b j), for n_stack stacked (nx, ny) panels (float32; nx, ny divisible by b,
+ * 1 <= b <= 128).  numpy's float32 order of operations: per block row the b values are
+ * summed in numpy's pairwise order and divided by b, the b row means are added in
+ * order and divided by b (bit-identical to reshape(...).mean(-1).mean(3)). */
+int cmtip_bin_images(const float* in, int64_t n_stack, int nx, int ny, int bin_factor, float* out, void* stream);
+/* The pixel selection of clip_data (prep_data.py:166-167): flags[p] (uint8, P) = 1 where
+ * (1 / sqrt((x^2 + y^2) + z^2)) * 1e10 > res_limit, all in float32 (res_limit rounded to
+ * float32, as numpy compares a float32 array with a Python float), for pixel p of
+ * s = (x[P], y[P], z[P]) with components stride floats apart; idx receives the
+ * selected pixel indices in increasing order and *count (device int64) their number. */
+int cmtip_resolution_select(const float* s, int64_t stride, int64_t P, double res_limit, uint8_t* flags,
+                            int64_t* idx, int64_t* count, void* stream);
+/* out[r, k] = in[r, idx[k]] (0 where idx[k] < 0) for rows of P_in elements in and P_out
+ * out, elements of 4 or 8 bytes: clip_data's arr[:, :, indices] (prep_data.py:168) and
+ * trim_dataset's masked gathers (:142-148). */
+int cmtip_gather_columns(const void* in, int elem_bytes, int64_t rows, int64_t P_in, const int64_t* idx,
+                         int64_t P_out, void* out, void* stream);
+/* assemble_image_stack_batch (prep_data.py:245-262) as an index: src[x * Y + y] = the
+ * flat pixel index (panel-major) that the index map (n_pixels x 2 int64) sends to (x, y),
+ * the last one when several do (numpy's fancy assignment), or -1. */
+int cmtip_assemble_index(const int64_t* index_map, int64_t n_pixels, int64_t X, int64_t Y, int64_t* src,
+                         void* stream);
+/* bin_pixel_index_map (prep_data.py:229-242): out[l, i, j, c] = min over a, e in {0, 1} of
+ * in[l, b i + a, b j + e, c], floor-divided by b (int64, (n_panels, px, py, 2) in). */
+int cmtip_bin_index_map(const int64_t* in, int64_t n_panels, int px, int py, int bin_factor, int64_t* out,
+                        void* stream);
+
 /* ---------------------------------------------------------------- measurement
  * No counterpart in the reference.  workgroups x 512 threads each read a 64 KB LDS
  * buffer iters times with conflict-free ds_read_b128 (16 B per lane per read) and
