@@ -49,6 +49,26 @@ __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsig
     return a < b ? a : b;
 }
 
+// XCD-aware tile order.  Workgroups are dealt to the 8 XCDs in turn (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  Workgroup b is given position
+// v = (b % 8) * (n8 / 8) + b / 8 of a grouped tile order (n8: the tile count rounded
+// down to a multiple of 8; the remainder keeps v = b), so the workgroups resident on one
+// XCD at a time hold consecutive positions; the grouped order walks kGroupM tile rows
+// before the next tile column, so those positions form a compact kGroupM-row block of
+// tiles whose A and B panels the XCD's L2 shares.
+constexpr int kGroupM = 8;
+__device__ __forceinline__ void tile_of(int b, int tiles_m, int tiles, int* tm, int* tn) {
+    const int n8 = tiles & ~7;
+    const int v = b < n8 ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
+    const int per_group = kGroupM * (tiles / tiles_m);  // tiles_n columns of kGroupM rows
+    const int g = v / per_group;
+    const int first = g * kGroupM;
+    const int gm = min(tiles_m - first, kGroupM);
+    const int r = v - g * per_group;
+    *tm = first + r % gm;
+    *tn = r / gm;
+}
+
 // ---------------------------------------------------------------- small reductions
 
 __global__ void weighted_norms_kernel(const float* __restrict__ X, int64_t P, const float* __restrict__ w2,
@@ -450,10 +470,15 @@ __global__ void center_rows_sq_kernel(const float* __restrict__ X, int64_t P, co
 // computed as 4 x 4 tiles of v_mfma_f32_16x16x4f32.  Operand layouts of that
 // instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and D[i][j] in
 // lane j + 16 (i / 4), register i % 4.  K is staged through double-buffered LDS in
-// slabs of 32.
+// slabs of 32.  The order of the K terms inside a slab is free (the same for A and
+// B): lane group lk (lanes 16 lk .. 16 lk + 15) takes slab columns 8 lk .. 8 lk + 7 in
+// its eight MFMA steps, so a lane reads its operands as two 16-byte LDS reads per tile
+// row per slab instead of eight 4-byte reads.  The LDS row pitch of 36 floats keeps
+// those reads 16-byte aligned, and 8 rows of one lane group start on 8 distinct
+// 4-bank groups.
 
 constexpr int BK32 = 32;
-constexpr int LD32 = BK32 + 1;  // padded row stride (floats)
+constexpr int LD32 = BK32 + 4;  // padded row stride (floats), a multiple of 4
 
 __device__ __forceinline__ void load_slab_f32(const float* __restrict__ X, int64_t rows, int64_t P,
                                               int64_t row0, int64_t k0, float4 (&reg)[4]) {
@@ -478,30 +503,43 @@ __device__ __forceinline__ void load_slab_f32(const float* __restrict__ X, int64
     }
 }
 
-__device__ __forceinline__ void store_slab_f32(float* __restrict__ S, const float4 (&reg)[4]) {
+// Fast form for full slabs of 16-byte aligned rows (P % 4 == 0, aligned base): rows past
+// the end are clamped to the last row (their products land in rows / columns the
+// epilogue skips), so the loads carry no branches.
+__device__ __forceinline__ void load_slab_f32_full(const float* __restrict__ X, int64_t rows, int64_t P,
+                                                   int64_t row0, int64_t k0, float4 (&reg)[4]) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        float* d = S + (t / 8 + 32 * i) * LD32 + (t % 8) * 4;
-        d[0] = reg[i].x;
-        d[1] = reg[i].y;
-        d[2] = reg[i].z;
-        d[3] = reg[i].w;
+        int64_t r = row0 + t / 8 + 32 * i;
+        r = r < rows ? r : rows - 1;
+        reg[i] = *reinterpret_cast<const float4*>(X + r * P + k0 + (t % 8) * 4);
     }
+}
+
+__device__ __forceinline__ void store_slab_f32(float* __restrict__ S, const float4 (&reg)[4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<float4*>(S + (t / 8 + 32 * i) * LD32 + (t % 8) * 4) = reg[i];
+}
+
+__device__ __forceinline__ float f4_at(const float4& v, int c) {
+    return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
 }
 
 // WRITE_DIST: instead of the argmin, write the full matrix D[r, i] = sqrt(max(d2, 0))
 // (float32, row-major n_ref x n_img) as sklearn's euclidean_distances returns it.
-template <bool WRITE_DIST>
+template <bool WRITE_DIST, bool ALIGNED>
 __global__ void __launch_bounds__(kThreads, 2)
     match_argmin_f32_kernel(const float* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
                             const float* __restrict__ Bw, int64_t n_img, int64_t P,
                             const float* __restrict__ an, const float* __restrict__ bn,
                             const double* __restrict__ gamma, unsigned long long* __restrict__ best,
                             float* __restrict__ dist, int tiles_m) {
-    __shared__ float As[2][BM * LD32];
-    __shared__ float Bs[2][BN * LD32];
-    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    __shared__ __align__(16) float As[2][BM * LD32];
+    __shared__ __align__(16) float Bs[2][BN * LD32];
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -514,32 +552,42 @@ __global__ void __launch_bounds__(kThreads, 2)
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     float4 ra[4], rb[4];
-    load_slab_f32(A, n_ref, P, m0, 0, ra);
-    load_slab_f32(Bw, n_img, P, n0, 0, rb);
     const int nk = (int)((P + BK32 - 1) / BK32);
+    // slabs [0, nfull) are whole 32-column slabs; with ALIGNED they load without branches
+    const int nfull = ALIGNED ? (int)(P / BK32) : 0;
+    auto load = [&](int kt) {
+        if (kt < nfull) {
+            load_slab_f32_full(A, n_ref, P, m0, (int64_t)kt * BK32, ra);
+            load_slab_f32_full(Bw, n_img, P, n0, (int64_t)kt * BK32, rb);
+        } else {
+            load_slab_f32(A, n_ref, P, m0, (int64_t)kt * BK32, ra);
+            load_slab_f32(Bw, n_img, P, n0, (int64_t)kt * BK32, rb);
+        }
+    };
+    load(0);
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         store_slab_f32(As[buf], ra);
         store_slab_f32(Bs[buf], rb);
         __syncthreads();
-        if (kt + 1 < nk) {
-            load_slab_f32(A, n_ref, P, m0, (int64_t)(kt + 1) * BK32, ra);
-            load_slab_f32(Bw, n_img, P, n0, (int64_t)(kt + 1) * BK32, rb);
-        }
-        const float* as = As[buf] + (wm * 64 + li) * LD32 + lk;
-        const float* bs = Bs[buf] + (wn * 64 + li) * LD32 + lk;
+        if (kt + 1 < nk) load(kt + 1);
+        const float* as = As[buf] + (wm * 64 + li) * LD32 + lk * 8;
+        const float* bs = Bs[buf] + (wn * 64 + li) * LD32 + lk * 8;
 #pragma unroll
-        for (int kk = 0; kk < BK32 / 4; ++kk) {
-            float af[4], bf[4];
+        for (int h = 0; h < 2; ++h) {
+            float4 av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = as[i * 16 * LD32 + kk * 4];
+            for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const float4*>(as + i * 16 * LD32 + 4 * h);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = bs[j * 16 * LD32 + kk * 4];
+            for (int j = 0; j < 4; ++j) bv[j] = *reinterpret_cast<const float4*>(bs + j * 16 * LD32 + 4 * h);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_at(av[i], c), f4_at(bv[j], c),
+                                                                         acc[i][j], 0, 0, 0);
         }
     }
 
@@ -624,6 +672,19 @@ __device__ __forceinline__ void load_slab_bf16(const uint16_t* __restrict__ X, i
     }
 }
 
+// Fast form for full slabs of 16-byte aligned rows (P % 8 == 0, aligned base), rows past
+// the end clamped to the last row (the epilogue skips their products)
+__device__ __forceinline__ void load_slab_bf16_full(const uint16_t* __restrict__ X, int64_t rows, int64_t P,
+                                                    int64_t row0, int64_t k0, uint4 (&reg)[4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int64_t r = row0 + t / 8 + 32 * i;
+        r = r < rows ? r : rows - 1;
+        reg[i] = *reinterpret_cast<const uint4*>(X + r * P + k0 + (t % 8) * 8);
+    }
+}
+
 __device__ __forceinline__ void store_slab_bf16(uint16_t* __restrict__ S, const uint4 (&reg)[4]) {
     const int t = threadIdx.x;
 #pragma unroll
@@ -654,6 +715,7 @@ __device__ __forceinline__ void global_insert(unsigned long long* slots, int k, 
 constexpr size_t kBf16Lds = 2 * (size_t)(BM + BN) * LD16 * sizeof(uint16_t);
 static_assert(kBf16Lds >= (size_t)BM * BN * sizeof(float), "epilogue reuses the operand LDS");
 
+template <bool ALIGNED>
 __global__ void __launch_bounds__(kThreads, 2)
     match_topk_bf16_kernel(const uint16_t* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
                            const uint16_t* __restrict__ Bw, int64_t n_img, int64_t P,
@@ -663,7 +725,8 @@ __global__ void __launch_bounds__(kThreads, 2)
     __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kBf16Lds];
     uint16_t* As0 = reinterpret_cast<uint16_t*>(lds_raw);
     uint16_t* Bs0 = As0 + 2 * BM * LD16;
-    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    int tm, tn;
+    tile_of(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -676,9 +739,18 @@ __global__ void __launch_bounds__(kThreads, 2)
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     uint4 ra[4], rb[4];
-    load_slab_bf16(A, n_ref, P, m0, 0, ra);
-    load_slab_bf16(Bw, n_img, P, n0, 0, rb);
     const int nk = (int)((P + BK16 - 1) / BK16);
+    const int nfull = ALIGNED ? (int)(P / BK16) : 0;
+    auto load = [&](int kt) {
+        if (kt < nfull) {
+            load_slab_bf16_full(A, n_ref, P, m0, (int64_t)kt * BK16, ra);
+            load_slab_bf16_full(Bw, n_img, P, n0, (int64_t)kt * BK16, rb);
+        } else {
+            load_slab_bf16(A, n_ref, P, m0, (int64_t)kt * BK16, ra);
+            load_slab_bf16(Bw, n_img, P, n0, (int64_t)kt * BK16, rb);
+        }
+    };
+    load(0);
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         uint16_t* As = As0 + buf * BM * LD16;
@@ -686,10 +758,7 @@ __global__ void __launch_bounds__(kThreads, 2)
         store_slab_bf16(As, ra);
         store_slab_bf16(Bs, rb);
         __syncthreads();
-        if (kt + 1 < nk) {
-            load_slab_bf16(A, n_ref, P, m0, (int64_t)(kt + 1) * BK16, ra);
-            load_slab_bf16(Bw, n_img, P, n0, (int64_t)(kt + 1) * BK16, rb);
-        }
+        if (kt + 1 < nk) load(kt + 1);
         const uint16_t* as = As + (wm * 64 + li) * LD16 + lk * 8;
         const uint16_t* bs = Bs + (wn * 64 + li) * LD16 + lk * 8;
 #pragma unroll
@@ -1092,9 +1161,14 @@ extern "C" int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref
     int64_t tiles_m, tiles;
     CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, best, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(match_argmin_f32_kernel<false>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
-                       A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best, (float*)nullptr,
-                       (int)tiles_m);
+    if (P % 4 == 0 && aligned16(A) && aligned16(Bw))
+        hipLaunchKernelGGL((match_argmin_f32_kernel<false, true>), dim3((unsigned)tiles), dim3(kThreads), 0,
+                           as_stream(stream), A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best,
+                           (float*)nullptr, (int)tiles_m);
+    else
+        hipLaunchKernelGGL((match_argmin_f32_kernel<false, false>), dim3((unsigned)tiles), dim3(kThreads), 0,
+                           as_stream(stream), A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best,
+                           (float*)nullptr, (int)tiles_m);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -1104,9 +1178,14 @@ extern "C" int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int
     int64_t tiles_m, tiles;
     CMTIP_TRY(check_match_args(n_a, 0, n_b, P, A, B, an, bn, D, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(match_argmin_f32_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
-                       A, n_a, 0u, B, n_b, P, an, bn, (const double*)nullptr, (unsigned long long*)nullptr, D,
-                       (int)tiles_m);
+    if (P % 4 == 0 && aligned16(A) && aligned16(B))
+        hipLaunchKernelGGL((match_argmin_f32_kernel<true, true>), dim3((unsigned)tiles), dim3(kThreads), 0,
+                           as_stream(stream), A, n_a, 0u, B, n_b, P, an, bn, (const double*)nullptr,
+                           (unsigned long long*)nullptr, D, (int)tiles_m);
+    else
+        hipLaunchKernelGGL((match_argmin_f32_kernel<true, false>), dim3((unsigned)tiles), dim3(kThreads), 0,
+                           as_stream(stream), A, n_a, 0u, B, n_b, P, an, bn, (const double*)nullptr,
+                           (unsigned long long*)nullptr, D, (int)tiles_m);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -1366,8 +1445,13 @@ extern "C" int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t r
     int64_t tiles_m, tiles;
     CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, cand, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(match_topk_bf16_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream), A,
-                       n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, (int)tiles_m);
+    if (P % 8 == 0 && aligned16(A) && aligned16(Bw))
+        hipLaunchKernelGGL(match_topk_bf16_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
+                           A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, (int)tiles_m);
+    else
+        hipLaunchKernelGGL(match_topk_bf16_kernel<false>, dim3((unsigned)tiles), dim3(kThreads), 0,
+                           as_stream(stream), A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand,
+                           (int)tiles_m);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
