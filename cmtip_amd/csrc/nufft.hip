@@ -34,7 +34,7 @@ struct cmtip_nufft_plan_s {
     DevBuf tmpA;     // M * n * n float2
     DevBuf tmpB;     // M * M * n float2
     DevBuf rinv;     // 9 doubles per orientation
-    DevBuf part;     // kMomentSlots x 2 doubles: striped partial moments
+    DevBuf part;     // slice-moment partials: 2 doubles per tile (or per wave), then the reduction stage
     int slice_ready = 0;
     int64_t npts = 0;
     const float* x = nullptr;
@@ -74,7 +74,7 @@ struct cmtip_nufft_plan_s {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMomentSlots = 1024;  // striped accumulators for the slice moments
+constexpr int kMomentBlocks = 256;  // first-stage blocks of the fixed-order moment reduction
 
 // ---------------------------------------------------------------- device helpers
 
@@ -821,20 +821,44 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
     }
 }
 
-// moments[0..1] += sum of the kMomentSlots striped partials; one wave
-__global__ void finish_moments_kernel(const double* __restrict__ part, double* __restrict__ moments) {
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < kMomentSlots; i += 64) {
+// Slice moments in a fixed order: every tile (or wave) leaves its (sum, sum of squares)
+// in its own slot, stage 1 sums contiguous slot ranges per block with a fixed tree,
+// stage 2 adds the block sums in order, so the moments do not depend on scheduling.
+__global__ void __launch_bounds__(kBlock) moments_stage1_kernel(const double* __restrict__ part, int64_t nslots,
+                                                                double* __restrict__ stage) {
+    __shared__ double red[2][kBlock];
+    const int64_t per = (nslots + kMomentBlocks - 1) / kMomentBlocks;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(nslots, b0 + per);
+    double a = 0.0, q = 0.0;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
         a += part[2 * i];
-        b += part[2 * i + 1];
+        q += part[2 * i + 1];
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_down(a, off, 64);
-        b += __shfl_down(b, off, 64);
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = q;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
+        stage[2 * blockIdx.x] = red[0][0];
+        stage[2 * blockIdx.x + 1] = red[1][0];
+    }
+}
+
+__global__ void moments_stage2_kernel(const double* __restrict__ stage, double* __restrict__ moments) {
+    if (threadIdx.x == 0) {
+        double a = 0.0, q = 0.0;
+        for (int i = 0; i < kMomentBlocks; ++i) {
+            a += stage[2 * i];
+            q += stage[2 * i + 1];
+        }
         moments[0] += a;
-        moments[1] += b;
+        moments[1] += q;
     }
 }
 
@@ -980,9 +1004,9 @@ __global__ void __launch_bounds__(kBlock) slice_finish_kernel(const float* __res
         s2v += __shfl_down(s2v, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        double* slot = part + 2 * ((blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) % kMomentSlots);
-        unsafeAtomicAdd(slot, s1v);
-        unsafeAtomicAdd(slot + 1, s2v);
+        double* slot = part + 2 * ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+        slot[0] = s1v;
+        slot[1] = s2v;
     }
 }
 
@@ -1207,15 +1231,15 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
         if (out_bf16) out_bf16[j] = f32_to_bf16(val);
     }
     if (moments) {
-        double s1v = (double)val, s2v = (double)val * (double)val;
+        const double v = active ? (double)val : 0.0;
+        double s1v = v, s2v = v * v;
         for (int off = 32; off > 0; off >>= 1) {
             s1v += __shfl_down(s1v, off, 64);
             s2v += __shfl_down(s2v, off, 64);
         }
         if (lane == 0 && tile_ok) {
-            double* slot = moments + 2 * (blockIdx.x % kMomentSlots);
-            unsafeAtomicAdd(slot, s1v);
-            unsafeAtomicAdd(slot + 1, s2v);
+            moments[2 * tile] = s1v;
+            moments[2 * tile + 1] = s2v;
         }
     }
 }
@@ -1344,7 +1368,6 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int total = ctrl[0];
     const int nbins = nbx * nbx * nbx;
-    double m1 = 0.0, m2 = 0.0;
     for (;;) {
         if (threadIdx.x == 0) chunk_id = atomicAdd(ctrl + 1, 1);
         __syncthreads();
@@ -1423,22 +1446,23 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
                 const int64_t j = o * rot.P + p;
                 out[j] = acc;
                 if (out_bf16) out_bf16[j] = f32_to_bf16(acc);
-                m1 += (double)acc;
-                m2 += (double)acc * (double)acc;
+            }
+            if (part) {
+                // this tile's moments in its own slot (a fixed-order reduction follows)
+                const double v = active ? (double)acc : 0.0;
+                double m1 = v, m2 = v * v;
+                for (int off = 32; off > 0; off >>= 1) {
+                    m1 += __shfl_down(m1, off, 64);
+                    m2 += __shfl_down(m2, off, 64);
+                }
+                if (lane == 0) {
+                    const int t = tperm[k];
+                    part[2 * (int64_t)t] = m1;
+                    part[2 * (int64_t)t + 1] = m2;
+                }
             }
         }
         __syncthreads();
-    }
-    if (part) {
-        for (int off = 32; off > 0; off >>= 1) {
-            m1 += __shfl_down(m1, off, 64);
-            m2 += __shfl_down(m2, off, 64);
-        }
-        if (lane == 0) {
-            double* slot = part + 2 * ((blockIdx.x * (kTileBlock / 64) + wave) % kMomentSlots);
-            unsafeAtomicAdd(slot, m1);
-            unsafeAtomicAdd(slot + 1, m2);
-        }
     }
 }
 
@@ -2379,13 +2403,15 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
         route = auto_route(p, P, n_orient);
     }
     p->route_used = route;
+    // moment partials: one slot per tile of 64 pixels (fused and tile routes) or per wave
+    // of the finishing pass (binned points); every slot is written by the launch
     double* part = nullptr;
+    const int64_t total = n_orient * P;
+    const int64_t finish_grid = std::min<int64_t>(4096, (total + kBlock - 1) / kBlock);
+    const int64_t nslots = route == 2 ? finish_grid * (kBlock / 64) : ((P + 63) / 64) * n_orient;
     if (moments) {
-        CMTIP_TRY(p->part.ensure(sizeof(double) * 2 * kMomentSlots));
+        CMTIP_TRY(p->part.ensure(sizeof(double) * 2 * (size_t)(nslots + kMomentBlocks)));
         part = p->part.as<double>();
-        hipLaunchKernelGGL(fill_f64_kernel, dim3(2 * kMomentSlots / kBlock), dim3(kBlock), 0, st, part,
-                           (int64_t)2 * kMomentSlots, 0.0);
-        CMTIP_LAUNCH_CHECK();
     }
     const bool timed = p->timing && p->timed < 64;
     if (route == 1) {
@@ -2400,15 +2426,16 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
         CMTIP_TRY(slice_binned(p, s, order, P, n_orient, recip_extent, out, st));
         if (timed) CMTIP_TRY(timer_mark(p, 1, st));
         if (out_bf16 || part) {
-            const int64_t total = n_orient * P;
-            hipLaunchKernelGGL(slice_finish_kernel,
-                               dim3((unsigned)std::min<int64_t>(4096, (total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                               st, (const float*)out, total, out_bf16, part);
+            hipLaunchKernelGGL(slice_finish_kernel, dim3((unsigned)finish_grid), dim3(kBlock), 0, st,
+                               (const float*)out, total, out_bf16, part);
             CMTIP_LAUNCH_CHECK();
         }
     }
     if (moments) {
-        hipLaunchKernelGGL(finish_moments_kernel, dim3(1), dim3(64), 0, st, (const double*)part, moments);
+        double* stage = part + 2 * nslots;
+        hipLaunchKernelGGL(moments_stage1_kernel, dim3(kMomentBlocks), dim3(kBlock), 0, st, (const double*)part,
+                           nslots, stage);
+        hipLaunchKernelGGL(moments_stage2_kernel, dim3(1), dim3(64), 0, st, (const double*)stage, moments);
         CMTIP_LAUNCH_CHECK();
     }
     return CMTIP_OK;

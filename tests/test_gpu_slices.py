@@ -225,3 +225,23 @@ def test_two_slicers_of_one_size_keep_their_volumes():
     for ac, got in ((acs[0], got_a), (acs[1], got_b), (acs[0], got_a2)):
         want = OA.compute_slices(q, s, ext, ac, exact=True).reshape(8, -1)
         assert rel_l2(got, want) < 1e-5
+
+
+@pytest.mark.parametrize("route", [1, 2, 3])
+def test_slice_moments_deterministic(route):
+    # the per-batch std ratio (align.py:122-124) comes from these moments: every route
+    # sums them in a fixed order, so repeated calls give the same bits
+    M, n_side, n_q = 81, 128, 300
+    s, ext = synth.detector(n_side)
+    ac = synth.gaussian_ac(M, 300, seed=5)
+    q = synth.random_quaternions(n_q, seed=6)
+    sl = alignment.Slicer(ac, s, ext)
+    say(f"moments twice, route {route}")
+    out1, _, mom1, used = _slice_route(sl, q, route)
+    out2, _, mom2, _ = _slice_route(sl, q, route)
+    assert used == route
+    assert torch.equal(out1, out2)
+    assert torch.equal(mom1, mom2)
+    o64 = out1.double()
+    assert abs(float(mom1[0]) - float(o64.sum())) <= 1e-12 * float(o64.abs().sum())
+    assert abs(float(mom1[1]) - float((o64 * o64).sum())) <= 1e-12 * float((o64 * o64).sum())
