@@ -594,12 +594,34 @@ __device__ __forceinline__ int spread_scale_exp(float cmax) {
     return 62 - 13 - e;  // kChunk = 2^12 points, one bit of margin
 }
 
+// out_bits[0] = bits of max |c| (non-negative floats order like their bits); out_bits[1]
+// is set to 1 when some c < 0 (the caller clears both)
 __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsigned* __restrict__ out_bits) {
     float m = 0.0f;
+    bool neg = false;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) m = fmaxf(m, fabsf(c[i]));
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const float v = c[i];
+        m = fmaxf(m, fabsf(v));
+        neg = neg || v < 0.0f;
+    }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(m));  // order of non-negative floats
+    const bool any_neg = __ballot(neg) != 0;
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out_bits, __float_as_uint(m));
+        if (any_neg) atomicOr(out_bits + 1, 1u);
+    }
+}
+
+// Fixed-point value of a non-negative float x < 2^63, rounded to nearest: the two 32-bit
+// halves from hi = floor(x / 2^32) and lo = x - hi 2^32, both exact (x below 2^32 is lo
+// itself; above, x is a multiple of 2^9 and its low half fits 24 bits).  Six
+// instructions where the general signed conversion takes eleven.
+__device__ __forceinline__ unsigned long long fixed_nonneg(float x) {
+    x = rintf(x);
+    const float hi = floorf(x * 2.3283064365386963e-10f);
+    const float lo = fmaf(hi, -4294967296.0f, x);
+    return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
 }
 
 template <int W, bool CPLX>
@@ -621,6 +643,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     __shared__ uint8_t cls_of[kChunk];
     const int total = ctrl[0];
     const int sc = spread_scale_exp(c ? __uint_as_float(*cmax_bits) : 1.0f);
+    const bool nonneg = !CPLX && (c == nullptr || cmax_bits[1] == 0u);
     const double down = ldexp(1.0, -sc);
     for (;;) {
         if (threadIdx.x == 0) chunk_id = atomicAdd(ctrl + 1, 1);
@@ -704,20 +727,34 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                 vr = ldexpf(c ? reinterpret_cast<const float*>(c)[src] : 1.0f, sc);
             }
             unsigned long long* base = acc + NC * (lx * SSP + ly * SP + lz);
+            if (nonneg) {
+                // real non-negative strengths (image counts, unit strengths): every
+                // contribution is >= 0, so the cheaper unsigned conversion applies
 #pragma unroll
-            for (int a = 0; a < W; ++a)
+                for (int a = 0; a < W; ++a)
 #pragma unroll
-                for (int bb = 0; bb < W; ++bb) {
-                    const float wab = wx[a] * wy[bb];
-                    unsigned long long* row = base + NC * (a * SSP + bb * SP);
-                    const float pr = wab * vr;
-                    const float pi = wab * vi;
+                    for (int bb = 0; bb < W; ++bb) {
+                        unsigned long long* row = base + (a * SSP + bb * SP);
+                        const float pr = (wx[a] * wy[bb]) * vr;
 #pragma unroll
-                    for (int cc = 0; cc < W; ++cc) {
-                        atomicAdd(row + NC * cc, (unsigned long long)__float2ll_rn(wz[cc] * pr));
-                        if (CPLX) atomicAdd(row + NC * cc + 1, (unsigned long long)__float2ll_rn(wz[cc] * pi));
+                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, fixed_nonneg(wz[cc] * pr));
                     }
-                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < W; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < W; ++bb) {
+                        const float wab = wx[a] * wy[bb];
+                        unsigned long long* row = base + NC * (a * SSP + bb * SP);
+                        const float pr = wab * vr;
+                        const float pi = wab * vi;
+#pragma unroll
+                        for (int cc = 0; cc < W; ++cc) {
+                            atomicAdd(row + NC * cc, (unsigned long long)__float2ll_rn(wz[cc] * pr));
+                            if (CPLX) atomicAdd(row + NC * cc + 1, (unsigned long long)__float2ll_rn(wz[cc] * pi));
+                        }
+                    }
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < S3; i += kSpreadThreads) {
@@ -1546,7 +1583,7 @@ struct SpreadLaunch {
         unsigned* cmax = reinterpret_cast<unsigned*>(p->ctrl.as<int>() + 2);
         if (c) {
             hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int*>(cmax),
-                               (int64_t)1, 0);
+                               (int64_t)2, 0);
             const int64_t cnt = (cplx ? 2 : 1) * p->npts;
             hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)std::min<int64_t>(4096, (cnt + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, st, reinterpret_cast<const float*>(c), cnt, cmax);
