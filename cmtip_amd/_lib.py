@@ -37,6 +37,7 @@ SIGNATURES = {
     "cmtip_nufft_makeplan": [_i, _i, _i, _d, ctypes.POINTER(_vp)],
     "cmtip_nufft_info": [_vp, _pi, _pi, _pd],
     "cmtip_nufft_setpts": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "cmtip_nufft_points_max_abs": [_vp, ctypes.POINTER(ctypes.c_float), _vp],
     "cmtip_nufft_execute": [_vp, _vp, _vp, _i, _vp, _f, _vp],
     "cmtip_nufft_destroy": [_vp],
     "cmtip_nufft_set_bins": [_vp, _i, _i, _i],

@@ -60,6 +60,8 @@ struct cmtip_nufft_plan_s {
     // kernel timer (cmtip_slice_timer): event pairs around the main slicing kernel
     int timing = 0, timed = 0;
     hipEvent_t tev[2 * 64] = {};
+    DevBuf amax;      // 1 uint32: bits of max |coordinate| of the registered points (setpts)
+    int amax_valid = 0;  // 1: amax describes the registered point arrays; 2: none registered (0)
     DevBuf bin_of;    // N int32
     DevBuf count;     // nbins int32 (also the scatter cursors)
     DevBuf offset;    // nbins + 1 int32
@@ -265,21 +267,34 @@ struct PtsArrays {
     }
 };
 
+// |x| for the coordinate range check, NaN counted as out of range (+inf)
+__device__ __forceinline__ float range_abs(float v) { return v == v ? fabsf(v) : __int_as_float(0x7f800000); }
+
+// max |coordinate| of a wave's points into *amax (non-negative float bits order like
+// the values)
+__device__ __forceinline__ void wave_amax(float m, unsigned* amax) {
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(m));
+}
+
 template <class Src>
 __global__ void bin_count_kernel(Src pts, int64_t N, int n, int W, int bs, int nbx, float hi, float lo,
-                                 int* __restrict__ bin_of, int* __restrict__ count) {
+                                 int* __restrict__ bin_of, int* __restrict__ count, unsigned* __restrict__ amax) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = j < N;
     int b = 0;
+    float m = 0.0f;
     if (valid) {
         float t, x, y, z;
         pts.at(j, &x, &y, &z);
+        if (amax) m = fmaxf(range_abs(x), fmaxf(range_abs(y), range_abs(z)));
         const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
         const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
         const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
         b = (bx * nbx + by) * nbx + bz;
         bin_of[j] = b;
     }
+    if (amax) wave_amax(m, amax);
     wave_bin_add(count, b, valid);
 }
 
@@ -380,14 +395,17 @@ template <class Src>
 __global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int64_t N, int64_t per_block, int n,
                                                                    int W, int bs, int nbx, int nbins, float hi,
                                                                    float lo, int* __restrict__ bin_of,
-                                                                   int* __restrict__ count) {
+                                                                   int* __restrict__ count,
+                                                                   unsigned* __restrict__ amax) {
     extern __shared__ int hist[];
     for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
     __syncthreads();
     const int64_t j0 = (int64_t)blockIdx.x * per_block, j1 = min(N, j0 + per_block);
+    float m = 0.0f;
     for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
         float t, x, y, z;
         pts.at(j, &x, &y, &z);
+        if (amax) m = fmaxf(m, fmaxf(range_abs(x), fmaxf(range_abs(y), range_abs(z))));
         const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
         const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
         const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
@@ -395,6 +413,7 @@ __global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int
         bin_of[j] = b;
         atomicAdd(hist + b, 1);
     }
+    if (amax) wave_amax(m, amax);
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += kBinBlock)
         if (hist[i]) atomicAdd(count + i, hist[i]);
@@ -1634,6 +1653,15 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
     const unsigned gp = blocks_for(N);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), p->nbins, 0);
+    // registered point arrays: their max |coordinate| for the caller's range check
+    unsigned* amax = nullptr;
+    p->amax_valid = 0;
+    if (!rot) {
+        CMTIP_TRY(p->amax.ensure(sizeof(unsigned)));
+        amax = p->amax.as<unsigned>();
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int*>(amax), (int64_t)1, 0);
+        p->amax_valid = 1;
+    }
     if (p->nbins <= kLocalBinsMax) {
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((N + 8191) / 8192, 4 * cu_count()));
         const int64_t per_block = (N + blocks - 1) / blocks;
@@ -1650,11 +1678,11 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
         if (rot)
             hipLaunchKernelGGL(bin_count_local_kernel<PtsRotated>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                *rot, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi, p->tc_lo,
-                               p->bin_of.as<int>(), p->count.as<int>());
+                               p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr);
         else
             hipLaunchKernelGGL(bin_count_local_kernel<PtsArrays>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                PtsArrays{p->x, p->y, p->z}, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi,
-                               p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+                               p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), amax);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                            p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
         if (rot)
@@ -1673,10 +1701,11 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     }
     if (rot)
         hipLaunchKernelGGL(bin_count_kernel<PtsRotated>, dim3(gp), dim3(kBlock), 0, st, *rot, N, p->n, p->w, p->bs,
-                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr);
     else
         hipLaunchKernelGGL(bin_count_kernel<PtsArrays>, dim3(gp), dim3(kBlock), 0, st, PtsArrays{p->x, p->y, p->z}, N,
-                           p->n, p->w, p->bs, p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>());
+                           p->n, p->w, p->bs, p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(),
+                           amax);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                        p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
     if (rot)
@@ -2069,8 +2098,38 @@ extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float*
     p->x = x;
     p->y = y;
     p->z = z;
-    if (npts == 0) return CMTIP_OK;
+    if (npts == 0) {
+        p->amax_valid = 2;  // no points: the maximum is 0
+        return CMTIP_OK;
+    }
     return bin_points(p, as_stream(stream));
+}
+
+extern "C" int cmtip_nufft_points_max_abs(cmtip_nufft_plan p, float* max_abs, void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (!max_abs) {
+        set_error("points_max_abs: null output");
+        return CMTIP_ERR_ARG;
+    }
+    if (p->amax_valid == 2) {
+        *max_abs = 0.0f;
+        return CMTIP_OK;
+    }
+    if (!p->amax_valid) {
+        set_error("points_max_abs: no point arrays registered since the last setpts");
+        return CMTIP_ERR_STATE;
+    }
+    hipStream_t st = as_stream(stream);
+    unsigned bits = 0;
+    CMTIP_HIP(hipMemcpyAsync(&bits, p->amax.p, sizeof(bits), hipMemcpyDeviceToHost, st));
+    CMTIP_HIP(hipStreamSynchronize(st));
+    union {
+        unsigned u;
+        float f;
+    } cv;
+    cv.u = bits;
+    *max_abs = cv.f;
+    return CMTIP_OK;
 }
 
 extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int flags, const float* support,
@@ -2224,7 +2283,7 @@ extern "C" int cmtip_nufft_destroy(cmtip_nufft_plan p) {
     p->rinv.release();
     p->part.release();
     for (DevBuf* b : {&p->bin_of, &p->count, &p->offset, &p->nchunk, &p->perm, &p->sorted, &p->chunks,
-                      &p->ctrl, &p->probe})
+                      &p->ctrl, &p->probe, &p->amax})
         b->release();
     delete p;
     return CMTIP_OK;

@@ -11,7 +11,6 @@ from .plan import DEFAULT_EPS, get_plan, tuned_setpts
 
 
 def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False, autotune=False):
-    assert _arrays.max_abs(H_, K_, L_) < 3 * np.pi  # adjoint.py:29, 66
     as_torch = _arrays.is_torch(nuvect, H_)
     x = _arrays.to_dev(H_, torch.float32).reshape(-1)
     y = _arrays.to_dev(K_, torch.float32).reshape(-1)
@@ -41,6 +40,8 @@ def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, one
         tuned_setpts(plan, x, y, z, c, f, flags, sup, scale)
     else:
         plan.setpts(x, y, z)
+    # the range check of adjoint.py:29, 66: the maximum is recorded while setpts bins
+    assert plan.points_max_abs() < 3 * np.pi
     plan.execute(c, f, flags, sup, scale)
     return _arrays.out(f, as_torch)
 

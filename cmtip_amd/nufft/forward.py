@@ -12,8 +12,14 @@ from .plan import DEFAULT_EPS, get_plan
 
 
 def _forward(ugrid, H_, K_, L_, support, use_recip_sym, eps, real_out=False):
-    # points must lie in finufft's permissible domain (forward.py:27, 64)
-    assert _arrays.max_abs(H_, K_, L_) < 3 * np.pi
+    x = _arrays.to_dev(H_, torch.float32).reshape(-1)
+    y = _arrays.to_dev(K_, torch.float32).reshape(-1)
+    z = _arrays.to_dev(L_, torch.float32).reshape(-1)
+    plan = get_plan(2, ugrid.shape[0], -1, eps)
+    plan.setpts(x, y, z)
+    # points must lie in finufft's permissible domain (forward.py:27, 64); the maximum is
+    # recorded while setpts bins the points
+    assert plan.points_max_abs() < 3 * np.pi
     # the reference applies the support in place on the caller's grid (forward.py:34-35, 66-67)
     if support is not None:
         if isinstance(ugrid, torch.Tensor):
@@ -33,17 +39,12 @@ def _forward(ugrid, H_, K_, L_, support, use_recip_sym, eps, real_out=False):
         f = _arrays.to_dev(ugrid, torch.complex64)
     M = f.shape[0]
     assert f.dim() == 3 and f.shape[1] == M and f.shape[2] == M, "ugrid must be (M, M, M)"
-    x = _arrays.to_dev(H_, torch.float32).reshape(-1)
-    y = _arrays.to_dev(K_, torch.float32).reshape(-1)
-    z = _arrays.to_dev(L_, torch.float32).reshape(-1)
     flags = CMTIP_GRID_REAL if real_grid else 0
     if real_out:
         c = torch.empty(x.numel(), dtype=torch.float32, device=f.device)
         flags |= CMTIP_PTS_REAL
     else:
         c = torch.empty(x.numel(), dtype=torch.complex64, device=f.device)
-    plan = get_plan(2, M, -1, eps)
-    plan.setpts(x, y, z)
     plan.execute(c, f, flags)
     return _arrays.out(c.reshape(shape), as_torch)
 

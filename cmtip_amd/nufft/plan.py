@@ -56,6 +56,12 @@ class NufftPlan:
         check(_lib.lib().cmtip_nufft_setpts(self._h, n, ptr(x), ptr(y), ptr(z), stream()), "cmtip_nufft_setpts")
         self._pts_key = key
 
+    def points_max_abs(self):
+        """Largest |coordinate| of the registered points (NaN as inf), recorded by setpts."""
+        v = ctypes.c_float()
+        check(_lib.lib().cmtip_nufft_points_max_abs(self._h, ctypes.byref(v), stream()), "cmtip_nufft_points_max_abs")
+        return v.value
+
     def execute(self, c, f, flags, support=None, scale=1.0):
         check(_lib.lib().cmtip_nufft_execute(self._h, ptr(c), ptr(f), int(flags), ptr(support), float(scale),
                                              stream()), "cmtip_nufft_execute")

@@ -68,6 +68,12 @@ int cmtip_nufft_info(cmtip_nufft_plan plan, int* w, int* n_fine, double* beta);
 int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, const float* y,
                        const float* z, void* stream);
 
+/* Largest |coordinate| of the point arrays registered by the last setpts (NaN counts as
+ * +inf), recorded while the points were binned: the range check of forward.py:27,64 and
+ * adjoint.py:29,66 without another pass over the points.  Synchronises the stream.
+ * CMTIP_ERR_STATE when the plan's points were replaced by a slicing route. */
+int cmtip_nufft_points_max_abs(cmtip_nufft_plan plan, float* max_abs, void* stream);
+
 /* execute flags */
 #define CMTIP_GRID_REAL 1   /* the uniform grid f is float32 (real); else complex64 */
 #define CMTIP_PTS_REAL 2    /* the point values c are float32 (real); else complex64 */

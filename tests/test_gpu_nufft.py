@@ -130,11 +130,24 @@ def test_empty_points():
 
 
 def test_out_of_range_asserts():
+    # forward.py:27,64 and adjoint.py:29,66 assert max |coordinate| < 3 pi; the maximum is
+    # recorded while setpts bins the points (NaN counts as out of range)
     M = 8
     f = np.ones((M, M, M), np.float32)
-    x = np.array([3 * np.pi + 0.1], np.float32)
-    with pytest.raises(AssertionError):
-        nufft.forward_gpu(f, x, x, x)
+    rng = np.random.default_rng(3)
+    ok = rng.uniform(-3.0, 3.0, 5000).astype(np.float32)
+    for bad in (3 * np.pi + 0.1, -3 * np.pi - 1e-3, np.nan):
+        x = ok.copy()
+        x[4321] = bad
+        say(f"range check with {bad}")
+        with pytest.raises(AssertionError):
+            nufft.forward_gpu(f, ok, ok, x)
+        with pytest.raises(AssertionError):
+            nufft.adjoint_gpu(np.ones(5000, np.float32), x, ok, ok, M)
+    # in range: the recorded maximum is the exact max |x|
+    from cmtip_amd.nufft.plan import get_plan
+    nufft.forward_gpu(f, ok, ok, ok)
+    assert get_plan(2, M, -1).points_max_abs() == np.abs(ok).max()
 
 
 @pytest.mark.parametrize("cplx", [False, True])
