@@ -63,6 +63,8 @@ def parse():
                    help="precision of the headline step (the workload's own: FP32 distances and argmin); the "
                         "other form is timed beside it")
     p.add_argument("--nbatches", type=int, default=1)
+    p.add_argument("--cross", default="lib", choices=["lib", "fused"],
+                   help="BF16 cross term of --part match: the library GEMM + top-k pass, or the fused kernel")
     p.add_argument("--no-extras", action="store_true", help="time the step only")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--part", default="all",
@@ -470,7 +472,7 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
         del clean
 
         def one():
-            m = AL._Matcher(s_d, imgs, 0, args.precision, 8)
+            m = AL._Matcher(s_d, imgs, 0, args.precision, 8, cross=args.cross)
             return m.run(AL.Slicer(ac, s_d, ext), ref, args.nbatches)
         one()
         torch.cuda.synchronize()
