@@ -734,12 +734,14 @@ __device__ __forceinline__ void store_slab_bf16(uint16_t* __restrict__ S, const 
         *reinterpret_cast<uint4*>(S + (t / 8 + 32 * i) * LD16 + (t % 8) * 8) = reg[i];
 }
 
-constexpr int kTopK = 8;
+constexpr int kTopK = 8;       // candidates per image of the fused kernel
+constexpr int kTopKMax = 16;   // of the library-GEMM top-k pass and the rescoring
 
-__device__ __forceinline__ void topk_insert(unsigned long long (&lst)[kTopK], unsigned long long v) {
-    if (v >= lst[kTopK - 1]) return;
+template <int L>
+__device__ __forceinline__ void topk_insert(unsigned long long (&lst)[L], unsigned long long v) {
+    if (v >= lst[L - 1]) return;
 #pragma unroll
-    for (int s = 0; s < kTopK; ++s) {
+    for (int s = 0; s < L; ++s) {
         const unsigned long long lo = umin64(lst[s], v);
         v = lst[s] < v ? v : lst[s];
         lst[s] = lo;
@@ -866,6 +868,7 @@ __global__ void __launch_bounds__(kThreads, 2)
 // rescore_kernel is the scalar form for other shapes: each wave takes candidates
 // s = wave, wave + 4, ...
 
+template <int L>
 __global__ void __launch_bounds__(kThreads) rescore4_kernel(const float* __restrict__ A, int64_t n_ref_total,
                                                             const float* __restrict__ B, int64_t P,
                                                             const float* __restrict__ w2,
@@ -873,14 +876,14 @@ __global__ void __launch_bounds__(kThreads) rescore4_kernel(const float* __restr
                                                             const unsigned long long* __restrict__ cand,
                                                             unsigned long long* __restrict__ best,
                                                             const uint8_t* __restrict__ mask) {
-    __shared__ float red[kThreads / 64][kTopK];
+    __shared__ float red[kThreads / 64][L];
     const int64_t i = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const float4* a4[kTopK];
-    float g[kTopK], acc[kTopK];
-    uint32_t r_of[kTopK];
+    const float4* a4[L];
+    float g[L], acc[L];
+    uint32_t r_of[L];
 #pragma unroll
-    for (int s = 0; s < kTopK; ++s) {
+    for (int s = 0; s < L; ++s) {
         const unsigned long long key = s < k ? cand[i * k + s] : kEmpty;
         const uint32_t r = (uint32_t)(key & 0xffffffffu);
         const bool ok = key != kEmpty && (int64_t)r < n_ref_total;
@@ -903,12 +906,12 @@ __global__ void __launch_bounds__(kThreads) rescore4_kernel(const float* __restr
             wv.z = mv.z ? wv.z : 0.f;
             wv.w = mv.w ? wv.w : 0.f;
         }
-        float4 av[kTopK];
+        float4 av[L];
 #pragma unroll
-        for (int s = 0; s < kTopK; ++s)
+        for (int s = 0; s < L; ++s)
             if (r_of[s] != 0xffffffffu) av[s] = a4[s][q];  // same for every thread of the block
 #pragma unroll
-        for (int s = 0; s < kTopK; ++s) {
+        for (int s = 0; s < L; ++s) {
             if (r_of[s] == 0xffffffffu) continue;
             float d = fmaf(g[s], av[s].x, -bv.x);
             float t = fmaf(wv.x * d, d, acc[s]);
@@ -921,7 +924,7 @@ __global__ void __launch_bounds__(kThreads) rescore4_kernel(const float* __restr
         }
     }
 #pragma unroll
-    for (int s = 0; s < kTopK; ++s) {
+    for (int s = 0; s < L; ++s) {
         float v = acc[s];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if (lane == 0) red[wave][s] = v;
@@ -1245,7 +1248,7 @@ constexpr int kCrossBatch = 4;  // loads in flight per lane before the inserts
 // VEC: n_ref a multiple of 4 and 16-byte aligned Dt and an (float4 loads).  The lists
 // hold the lane's smallest keys whatever the order the references arrive in, so both
 // forms give the same top-k.
-template <bool VEC>
+template <bool VEC, int L>
 __global__ void __launch_bounds__(64 * kCrossWaves) topk_cross_kernel(const float* __restrict__ Dt, int64_t n_ref,
                                                                       uint32_t ref_offset, int64_t n_img,
                                                                       const float* __restrict__ an,
@@ -1258,9 +1261,9 @@ __global__ void __launch_bounds__(64 * kCrossWaves) topk_cross_kernel(const floa
     const float g = gamma ? (float)gamma[0] : 1.0f;
     const float g2 = g * g, m2g = -2.0f * g;
     const float bni = bn[i];
-    unsigned long long lst[kTopK];
+    unsigned long long lst[L];
 #pragma unroll
-    for (int s = 0; s < kTopK; ++s) lst[s] = kEmpty;
+    for (int s = 0; s < L; ++s) lst[s] = kEmpty;
     unsigned long long* slots = cand + i * k;
     if (lane < k) topk_insert(lst, slots[lane]);
     const float* row = Dt + i * n_ref;
@@ -1311,8 +1314,8 @@ __global__ void __launch_bounds__(64 * kCrossWaves) topk_cross_kernel(const floa
         for (int off = 32; off > 0; off >>= 1) m = umin64(m, shfl_xor_u64(m, off));
         if (lst[0] == m && m != kEmpty) {
 #pragma unroll
-            for (int q = 0; q < kTopK - 1; ++q) lst[q] = lst[q + 1];
-            lst[kTopK - 1] = kEmpty;
+            for (int q = 0; q < L - 1; ++q) lst[q] = lst[q + 1];
+            lst[L - 1] = kEmpty;
         }
         if (lane == 0) slots[s] = m;
     }
@@ -1451,8 +1454,8 @@ extern "C" int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const ui
 extern "C" int cmtip_match_topk_cross(const float* Dt, int64_t n_ref, int64_t ref_offset, int64_t n_img,
                                       const float* an, const float* bn, const double* gamma, int k,
                                       unsigned long long* cand, void* stream) {
-    if (k < 1 || k > kTopK) {
-        set_error("match_topk_cross: k must be in [1, %d]", kTopK);
+    if (k < 1 || k > kTopKMax) {
+        set_error("match_topk_cross: k must be in [1, %d]", kTopKMax);
         return CMTIP_ERR_ARG;
     }
     if (n_ref < 0 || n_img < 0 || ref_offset < 0 || ref_offset + n_ref > 0xffffffffLL ||
@@ -1466,12 +1469,17 @@ extern "C" int cmtip_match_topk_cross(const float* Dt, int64_t n_ref, int64_t re
         set_error("match_topk_cross: too many images");
         return CMTIP_ERR_ARG;
     }
-    if (n_ref % 4 == 0 && aligned16(Dt) && aligned16(an))
-        hipLaunchKernelGGL(topk_cross_kernel<true>, dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0,
-                           as_stream(stream), Dt, n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand);
-    else
-        hipLaunchKernelGGL(topk_cross_kernel<false>, dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0,
-                           as_stream(stream), Dt, n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand);
+    const bool vec = n_ref % 4 == 0 && aligned16(Dt) && aligned16(an);
+#define CMTIP_TOPK_GO(V, L)                                                                                    \
+    hipLaunchKernelGGL((topk_cross_kernel<V, L>), dim3((unsigned)blocks), dim3(64 * kCrossWaves), 0,           \
+                       as_stream(stream), Dt, n_ref, (uint32_t)ref_offset, n_img, an, bn, gamma, k, cand)
+    // each lane keeps a list as long as k, so the merged top k is exact
+    if (k <= kTopK) {
+        if (vec) CMTIP_TOPK_GO(true, kTopK); else CMTIP_TOPK_GO(false, kTopK);
+    } else {
+        if (vec) CMTIP_TOPK_GO(true, kTopKMax); else CMTIP_TOPK_GO(false, kTopKMax);
+    }
+#undef CMTIP_TOPK_GO
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -1508,7 +1516,7 @@ extern "C" int cmtip_match_rescore_masked(const float* A_all, int64_t n_ref_tota
                                           const uint8_t* mask, int64_t n_img, int64_t P, const float* w2,
                                           const double* gamma_of_ref, int k, const unsigned long long* cand,
                                           unsigned long long* best, void* stream) {
-    if (k < 1 || k > kTopK || n_img < 0 || P < 1 || n_ref_total < 0 ||
+    if (k < 1 || k > kTopKMax || n_img < 0 || P < 1 || n_ref_total < 0 ||
         (n_img > 0 && (!A_all || !B || !w2 || !gamma_of_ref || !cand || !best))) {
         set_error("match_rescore: bad arguments");
         return CMTIP_ERR_ARG;
@@ -1521,9 +1529,12 @@ extern "C" int cmtip_match_rescore_masked(const float* A_all, int64_t n_ref_tota
     const bool quads = P % 4 == 0 && ((reinterpret_cast<uintptr_t>(A_all) | reinterpret_cast<uintptr_t>(B) |
                                         reinterpret_cast<uintptr_t>(w2)) & 15) == 0 &&
                        (reinterpret_cast<uintptr_t>(mask) & 3) == 0;
-    if (quads)
-        hipLaunchKernelGGL(rescore4_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
+    if (quads && k <= kTopK)
+        hipLaunchKernelGGL(rescore4_kernel<kTopK>, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
                            n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);
+    else if (quads)
+        hipLaunchKernelGGL(rescore4_kernel<kTopKMax>, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream),
+                           A_all, n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);
     else
         hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)n_img), dim3(kThreads), 0, as_stream(stream), A_all,
                            n_ref_total, B, P, w2, gamma_of_ref, k, cand, best, mask);

@@ -77,15 +77,16 @@ def test_masked_matching_m151(side):
     # the bright centre shrinks the best-to-second gaps ~100 x (median 5.6e-4 relative at
     # 128 x 128); with y^2 split into two bfloat16 parts the BF16 top-8 still holds the
     # best reference (a single bfloat16 y^2 missed it for 7 % of the images), so both
-    # forms must give the FP64 assignment outside the tie band
+    # forms must give the FP64 assignment outside the tie band; 10000 references x 10000
+    # images per binning
     d = dev()
-    M, n_ref, n_img = 151, 3000, 800
+    M, n_ref, n_img = 151, 10000, 10000
     s, ext = synth.detector(side)
     ac = torch.from_numpy(synth.gaussian_ac(M, 5000, seed=0)).to(d)
     ref = synth.random_quaternions(n_ref, seed=3)
     q_img = synth.random_quaternions(n_img, seed=4)
     sl = alignment.Slicer(ac, s, ext)
-    say(f"masked matching M=151 side={side}")
+    say(f"masked matching M=151 side={side} {n_img} x {n_ref}")
     clean = sl.slice(q_img).clamp(min=0)
     torch.manual_seed(5)
     imgs = torch.poisson(clean * (10.0 / float(clean.mean())))
@@ -109,9 +110,18 @@ def test_masked_matching_m151(side):
     want, d01 = _fp64_best(A, imgs, w2, gamma, mask=mask_d)
     ok = ((d01[1] - d01[0]) > 1e-5 * d01[0].abs()).cpu().numpy()
     got, w = idx.cpu().numpy(), want.cpu().numpy()
+    say(f"  outside the tie band {ok.mean():.4f}; fp32 agreement {(got == w).mean():.5f}, "
+        f"bf16 {(idx16.cpu().numpy() == w).mean():.5f}")
     assert ok.mean() > 0.9
     assert np.array_equal(got[ok], w[ok]), f"{(got[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
     got16 = idx16.cpu().numpy()
+    bad = np.flatnonzero(ok & (got16 != w))
+    for i in bad[:5]:
+        # FP64 distance of the BF16 choice against the best and second best
+        mw = mask_d[i].double() * w2.double()
+        dd = ((gamma * A[[got16[i], w[i]]].double() - imgs[i].double()) ** 2 * mw).sum(1)
+        say(f"  bf16 mismatch image {i}: d2(bf16 pick) {float(dd[0]):.9e}, d2(best) {float(dd[1]):.9e}, "
+            f"second {float(d01[1, i]):.9e}, relative gap {float((dd[0] - dd[1]) / dd[1]):.2e}")
     assert np.array_equal(got16[ok], w[ok]), f"bf16: {(got16[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
 
 
