@@ -270,6 +270,7 @@ int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t ref_offset,
  * writes the FP32 cross-term block Dt[i * n_ref + r] = <A_r, B_i> (bfloat16 rows of K
  * elements, FP32 accumulation) with the hipBLASLt GEMM; cmtip_match_topk_cross then
  * merges d2 = gamma^2 an[r] - 2 gamma Dt[i, r] + bn[i] into the image's k candidates
+ * (1 <= k <= 16 here and in cmtip_match_rescore[_masked]; the fused kernel keeps <= 8)
  * (one wave per image, no atomics; ties keep the smallest reference index).  Replaces
  * the same sklearn euclidean_distances + argmin (align.py:35,128-131). */
 int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const uint16_t* B, int64_t n_img, int64_t K,
