@@ -11,6 +11,20 @@
 // inverse axis 1, inverse axis 2 with the crop, support, scale and the rho*u term
 // folded into the store.  fourier_reg (:88-110) is the same five-pass shape on the
 // M^3 grid with the (ifftshifted) antisupport mask as the multiplier.
+//
+// The CG operator W = alpha conv + rho I + phi reg (autocorrelation.py:150-155) runs
+// as ONE five-pass sequence on half the spectrum:
+//  - fourier_reg folds into the Toeplitz multiplier when L is a multiple c of M (the
+//    reference's M_ups = 2M): the M-point DFT of the input equals the L-point DFT of
+//    the zero-padded input at multiples of c, so the antisupport becomes an L^3
+//    multiplier that is zero off the multiples of c.
+//  - u and the result are real and only the real part is kept (:59-62, :108-109), so
+//    a real multiplier G can be replaced by its symmetric part (G[k] + G[-k]) / 2,
+//    which makes every intermediate Hermitian: passes 1-4 keep only the first L/2 + 1
+//    positions of the last axis and pass 5 extends them by conjugate symmetry.
+//  - the CG direction update runs inside pass 1's load, p.q inside pass 5's store,
+//    and x / r updates, the r.r reduction and the loop check in one kernel, so an
+//    iteration is six launches.
 #include <math.h>
 
 #include <vector>
@@ -30,6 +44,11 @@ struct cmtip_acsolver_s {
     DevBuf scal;     // FP64 CG scalars
     DevBuf part;     // kRedBlocks doubles: per-block partial sums (fixed-order reductions)
     DevBuf ctrl;     // int32 CG state: stop, converged, iterations, maxiter
+    DevBuf Gh;       // L * L * (L/2 + 1) float: folded, symmetrised multiplier of W (half spectrum)
+    DevBuf pq_part;  // per-workgroup partial sums of p.q
+    DevBuf ticket;   // uint32 arrival counter of the update kernel (back to 0 after each launch)
+    double g_alpha = 0.0, g_phi = 0.0;
+    bool g_valid = false;  // Gh matches Fk, mask, g_alpha and g_phi
     int* host_state = nullptr;  // pinned host-mapped mirror: stop, converged, iterations, pad, rho (double)
     int* host_state_dev = nullptr;
     const float* support = nullptr;
@@ -157,8 +176,8 @@ __global__ void dot_partial_kernel(const float* __restrict__ a, const float* __r
     block_partial(s, part);
 }
 
-// out = sum of the kRedBlocks partials (fixed order); mode 1 also rotates rho.
-__global__ void reduce_partials_kernel(const double* __restrict__ part, double* __restrict__ s, int slot, int mode,
+// s[slot] = sum of the kRedBlocks partials (fixed order)
+__global__ void reduce_partials_kernel(const double* __restrict__ part, double* __restrict__ s, int slot,
                                        const int* __restrict__ ctrl) {
     if (stopped(ctrl)) return;
     __shared__ double red[kRedBlocks];
@@ -168,14 +187,7 @@ __global__ void reduce_partials_kernel(const double* __restrict__ part, double* 
         if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        if (mode == 1) {
-            s[S_RHO_PREV] = s[S_RHO];
-            s[S_RHO] = red[0];
-        } else {
-            s[slot] = red[0];
-        }
-    }
+    if (threadIdx.x == 0) s[slot] = red[0];
 }
 
 __global__ void set_scalars_kernel(double* __restrict__ s, int first, int count, double v) {
@@ -224,23 +236,33 @@ __global__ void copy_if_zero_rhs_kernel(const float* __restrict__ b, float* __re
     for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) x[i] = b[i];
 }
 
-// p = r (first iteration) or p = beta p + r with beta = rho / rho_prev
-__global__ void update_p_kernel(const float* __restrict__ r, float* __restrict__ p, const double* __restrict__ s,
-                                int64_t n, const int* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
-    const bool first = ctrl[C_ITERS] == 1;
-    const float beta = first ? 0.0f : (float)(s[S_RHO] / s[S_RHO_PREV]);
-    const int64_t stride = (int64_t)gridDim.x * kT;
-    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride)
-        p[i] = first ? r[i] : beta * p[i] + r[i];
-}
-
-// alpha = rho / (p.q); x += alpha p; r -= alpha q; per-block partials of r.r
-__global__ void update_xr_kernel(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
-                                 const float* __restrict__ q, const double* __restrict__ s, int64_t n,
-                                 double* __restrict__ part, const int* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
-    const float alpha = (float)(s[S_RHO] / s[S_PQ]);
+// alpha = rho / (p.q), with p.q the fixed-order sum of the npq partials (every
+// workgroup adds them in the same order, so all agree bit for bit); x += alpha p;
+// r -= alpha q; per-workgroup partials of r.r.  The last workgroup to arrive adds
+// those partials in a fixed order, rotates rho and runs the loop check of the next
+// iteration (cg_check_kernel's rule), then resets the arrival counter.
+__global__ void update_xr_check_kernel(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+                                       const float* __restrict__ q, double* __restrict__ s,
+                                       const double* __restrict__ pq_part, int npq, int64_t n,
+                                       double* __restrict__ part, int* __restrict__ ctrl,
+                                       unsigned* __restrict__ ticket) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double red[kT / 64];
+    __shared__ double pq_s;
+    __shared__ bool last;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < npq; i += kT) v += pq_part[i];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kT / 64; ++i) t += red[i];
+        pq_s = t;
+    }
+    __syncthreads();
+    const double pq = pq_s;
+    const float alpha = (float)(s[S_RHO] / pq);
     double acc = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kT;
     for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) {
@@ -249,7 +271,69 @@ __global__ void update_xr_kernel(float* __restrict__ x, float* __restrict__ r, c
         r[i] = rn;
         acc += (double)rn * (double)rn;
     }
-    block_partial(acc, part);
+    __syncthreads();  // red is reused
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kT / 64; ++i) t += red[i];
+        part[blockIdx.x] = t;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    __shared__ double tree[kRedBlocks];
+    for (int i = threadIdx.x; i < kRedBlocks; i += kT)
+        tree[i] = i < (int)gridDim.x ? __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+    __syncthreads();
+    for (int h = kRedBlocks / 2; h > 0; h >>= 1) {
+        for (int i = threadIdx.x; i < h; i += kT) tree[i] += tree[i + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s[S_PQ] = pq;
+        s[S_RHO_PREV] = s[S_RHO];
+        s[S_RHO] = tree[0];
+        if (ctrl[C_ITERS] >= ctrl[C_MAXITER]) {
+            ctrl[C_STOP] = 1;
+        } else if (sqrt(tree[0]) < s[S_ATOL]) {
+            ctrl[C_CONV] = 1;
+            ctrl[C_STOP] = 1;
+        } else {
+            ctrl[C_ITERS] += 1;
+        }
+        *ticket = 0u;
+    }
+}
+
+// The operator's multiplier on the half spectrum (p0, p1 < L, p2 <= L/2):
+// G = ka (K[p] + K[-p]) / 2 + ma (m[p / c] + m[-p / c]) / 2 where the second term is
+// present only at positions whose three indices are multiples of c (c = 0: absent).
+// K is the kernel spectrum (L^3), m the antisupport in FFT order (M^3); indices are
+// taken modulo the grid edge.
+__global__ void half_multiplier_kernel(const float* __restrict__ K, const float* __restrict__ m,
+                                       float* __restrict__ G, int L, int M, int c, double ka, double ma) {
+    const int64_t H = L / 2 + 1;
+    const int64_t total = (int64_t)L * L * H;
+    const int64_t stride = (int64_t)gridDim.x * kT;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < total; i += stride) {
+        const int p2 = (int)(i % H);
+        const int64_t t = i / H;
+        const int p1 = (int)(t % L);
+        const int p0 = (int)(t / L);
+        const int q0 = p0 ? L - p0 : 0, q1 = p1 ? L - p1 : 0, q2 = p2 ? L - p2 : 0;
+        const double k = 0.5 * ((double)K[((int64_t)p0 * L + p1) * L + p2] + (double)K[((int64_t)q0 * L + q1) * L + q2]);
+        double g = ka * k;
+        if (c > 0 && p0 % c == 0 && p1 % c == 0 && p2 % c == 0) {
+            const int j0 = p0 / c, j1 = p1 / c, j2 = p2 / c;
+            const int k0 = j0 ? M - j0 : 0, k1 = j1 ? M - j1 : 0, k2 = j2 ? M - j2 : 0;
+            g += ma * 0.5 * ((double)m[((int64_t)j0 * M + j1) * M + j2] + (double)m[((int64_t)k0 * M + k1) * M + k2]);
+        }
+        G[i] = (float)g;
+    }
 }
 
 // copy the loop state to pinned host memory (read by the host after a sync)
@@ -490,12 +574,158 @@ int reg_apply(cmtip_acsolver_s* s, const float* u, float* out, float out_scale, 
     return CMTIP_OK;
 }
 
+// c = L / M when fourier_reg folds into the L-grid multiplier, else 0
+int fold_factor(const cmtip_acsolver_s* s) { return s->L % s->M == 0 ? s->L / s->M : 0; }
+
+// Gh for W = alpha conv + rho I + phi reg (rebuilt when alpha, phi, the spectrum or the
+// antisupport changed)
+int ensure_multiplier(cmtip_acsolver_s* s, double alpha, double phi, hipStream_t st) {
+    if (s->g_valid && s->g_alpha == alpha && s->g_phi == phi) return CMTIP_OK;
+    const int M = s->M, L = s->L, c = fold_factor(s);
+    const double L3 = (double)L * L * L, M3 = (double)M * M * M;
+    const int64_t total = (int64_t)L * L * (L / 2 + 1);
+    hipLaunchKernelGGL(half_multiplier_kernel, dim3(nblk(total, 4096)), dim3(kT), 0, st, s->Fk.as<const float>(),
+                       s->mask.as<const float>(), s->Gh.as<float>(), L, M, c, alpha / L3, c ? phi / M3 : 0.0);
+    CMTIP_LAUNCH_CHECK();
+    s->g_alpha = alpha;
+    s->g_phi = phi;
+    s->g_valid = true;
+    return CMTIP_OK;
+}
+
+// out = S (.) Re(ifft(Gh fft(S (.) u))) + rho u in five passes over half the spectrum
+// of the last axis (H = L/2 + 1 positions): t1 is (M, M, H), t2 is (M, L, H).  With
+// cg set, u is r: pass 1 first replaces p by the new CG direction and transforms that,
+// pass 5 adds rho p and, when pq_parts is non-null, writes the p.q partials to pq_part
+// (their count to *pq_parts).
+int fast_apply(cmtip_acsolver_s* s, const float* u, float* out, float rho, bool cg, int* pq_parts,
+               hipStream_t st) {
+    const int M = s->M, L = s->L, H = L / 2 + 1;
+    const int64_t MM = (int64_t)M * M, MH = (int64_t)M * H, LH = (int64_t)L * H;
+    float2* t1 = s->tmp1.as<float2>();
+    float2* t2 = s->tmp2.as<float2>();
+    float* p = s->p.as<float>();
+    AxisDesc d;
+    // 1: axis 2 forward, lines (i0, i1), inputs [0, M), outputs [0, H)
+    d = AxisDesc();
+    d.in = u;
+    d.in_real = 1;
+    d.in_su = MM;
+    d.in_sv = M;
+    d.in_sa = 1;
+    d.ia = M;
+    d.in_mask = s->support;
+    if (cg) {
+        d.cg_p = p;
+        d.cg_scal = s->scal.as<const double>() + S_RHO;  // rho, rho_prev
+        d.cg_ctrl = s->ctrl.as<const int>();
+    }
+    d.out = t1;
+    d.out_su = MH;
+    d.out_sv = H;
+    d.out_sa = 1;
+    d.oa = H;
+    d.nu = M;
+    d.nv = M;
+    d.sign = -1;
+    d.skip = s->cur_skip;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 2: axis 1 forward, lines (i0, p2)
+    d = AxisDesc();
+    d.in = t1;
+    d.in_su = MH;
+    d.in_sv = 1;
+    d.in_sa = H;
+    d.ia = M;
+    d.out = t2;
+    d.out_su = LH;
+    d.out_sv = 1;
+    d.out_sa = H;
+    d.oa = L;
+    d.nu = M;
+    d.nv = H;
+    d.sign = -1;
+    d.skip = s->cur_skip;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 3: axis 0 forward, x Gh, inverse; lines (p1, p2); in place, outputs [0, M)
+    d = AxisDesc();
+    d.in = t2;
+    d.in_su = H;
+    d.in_sv = 1;
+    d.in_sa = LH;
+    d.ia = M;
+    d.out = t2;
+    d.out_su = H;
+    d.out_sv = 1;
+    d.out_sa = LH;
+    d.oa = M;
+    d.mid_r = s->Gh.as<const float>();
+    d.mid_su = H;
+    d.mid_sv = 1;
+    d.mid_sa = LH;
+    d.nu = L;
+    d.nv = H;
+    d.sign = -1;
+    d.skip = s->cur_skip;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 4: axis 1 inverse, lines (i0, p2), outputs [0, M)
+    d = AxisDesc();
+    d.in = t2;
+    d.in_su = LH;
+    d.in_sv = 1;
+    d.in_sa = H;
+    d.ia = L;
+    d.out = t1;
+    d.out_su = MH;
+    d.out_sv = 1;
+    d.out_sa = H;
+    d.oa = M;
+    d.nu = M;
+    d.nv = H;
+    d.sign = +1;
+    d.skip = s->cur_skip;
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    // 5: axis 2 inverse from the half spectrum (conjugate symmetry), real part, crop,
+    // support, + rho u
+    const float* src = cg ? p : u;
+    d = AxisDesc();
+    d.in = t1;
+    d.in_su = MH;
+    d.in_sv = H;
+    d.in_sa = 1;
+    d.ia = H;
+    d.ib = L - H;
+    d.in_herm = 1;
+    d.out = out;
+    d.out_mode = 1;
+    d.out_su = MM;
+    d.out_sv = M;
+    d.out_sa = 1;
+    d.oa = M;
+    d.out_mask = s->support;
+    d.add_src = src;
+    d.add_scale = rho;
+    d.nu = M;
+    d.nv = M;
+    d.sign = +1;
+    d.skip = s->cur_skip;
+    if (pq_parts) {
+        d.dot_src = src;
+        d.dot_part = s->pq_part.as<double>();
+        *pq_parts = (int)fft_axis_grid(s->fftL, d);
+    }
+    CMTIP_TRY(fft_axis_pass(s->fftL, d, st));
+    return CMTIP_OK;
+}
+
 int matvec(cmtip_acsolver_s* s, const float* u, float* out, double alpha, double rho, double phi,
            hipStream_t st) {
-    const double L3 = (double)s->L * s->L * s->L;
-    const double M3 = (double)s->M * s->M * s->M;
-    CMTIP_TRY(conv_apply(s, u, out, (float)(alpha / L3), u, (float)rho, st));
-    CMTIP_TRY(reg_apply(s, u, out, (float)(phi / M3), out, 1.0f, st));
+    CMTIP_TRY(ensure_multiplier(s, alpha, phi, st));
+    CMTIP_TRY(fast_apply(s, u, out, (float)rho, false, nullptr, st));
+    if (!fold_factor(s)) {
+        const double M3 = (double)s->M * s->M * s->M;
+        CMTIP_TRY(reg_apply(s, u, out, (float)(phi / M3), out, 1.0f, st));
+    }
     return CMTIP_OK;
 }
 
@@ -504,7 +734,7 @@ int dot(cmtip_acsolver_s* sv, const float* a, const float* b, int64_t n, double*
         const int* ctrl) {
     hipLaunchKernelGGL(dot_partial_kernel, dim3(kRedBlocks), dim3(kT), 0, st, a, b, n, sv->part.as<double>(), ctrl);
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kRedBlocks), 0, st, sv->part.as<const double>(), scal,
-                       slot, 0, ctrl);
+                       slot, ctrl);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -544,6 +774,13 @@ extern "C" int cmtip_acsolver_create(int M, int L, cmtip_acsolver* out) {
     if (st == CMTIP_OK) st = s->scal.ensure(sizeof(double) * S_COUNT);
     if (st == CMTIP_OK) st = s->part.ensure(sizeof(double) * kRedBlocks);
     if (st == CMTIP_OK) st = s->ctrl.ensure(sizeof(int) * 4);
+    if (st == CMTIP_OK) st = s->Gh.ensure(sizeof(float) * (size_t)L * L * (L / 2 + 1));
+    if (st == CMTIP_OK) st = s->pq_part.ensure(sizeof(double) * (size_t)(M * M > kRedBlocks ? M * M : kRedBlocks));
+    if (st == CMTIP_OK) st = s->ticket.ensure(sizeof(unsigned));
+    if (st == CMTIP_OK && hipMemset(s->ticket.p, 0, sizeof(unsigned)) != hipSuccess) {
+        set_error("acsolver_create: counter initialisation failed");
+        st = CMTIP_ERR_HIP;
+    }
     if (st == CMTIP_OK) {
         void* h = nullptr;
         if (hipHostMalloc(&h, 64, hipHostMallocMapped) != hipSuccess ||
@@ -602,6 +839,7 @@ extern "C" int cmtip_acsolver_set_antisupport(cmtip_acsolver s, const float* F_a
     hipLaunchKernelGGL(ifftshift3_kernel, dim3((unsigned)((M3 + kT - 1) / kT)), dim3(kT), 0, as_stream(stream),
                        F_antisupport, s->mask.as<float>(), M);
     CMTIP_LAUNCH_CHECK();
+    s->g_valid = false;
     return CMTIP_OK;
 }
 
@@ -613,6 +851,7 @@ extern "C" int cmtip_acsolver_set_spectrum(cmtip_acsolver s, const float* F_real
     }
     const size_t bytes = sizeof(float) * (size_t)s->L * s->L * s->L;
     CMTIP_HIP(hipMemcpyAsync(s->Fk.p, F_real, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    s->g_valid = false;
     return CMTIP_OK;
 }
 
@@ -630,6 +869,9 @@ extern "C" int cmtip_acsolver_destroy(cmtip_acsolver s) {
     s->scal.release();
     s->part.release();
     s->ctrl.release();
+    s->Gh.release();
+    s->pq_part.release();
+    s->ticket.release();
     if (s->host_state) (void)hipHostFree(s->host_state);
     if (s->ev_in) (void)hipEventDestroy(s->ev_in);
     if (s->ev_out) (void)hipEventDestroy(s->ev_out);
@@ -764,6 +1006,7 @@ extern "C" int cmtip_acsolver_set_kernel(cmtip_acsolver s, const float* ugrid_co
         d.sign = -1;
         rc = fft_axis_pass(s->fftL, d, st);
     }
+    s->g_valid = false;
     hipError_t e = hipStreamSynchronize(st);  // scratch buffers are freed below
     big1.release();
     big2.release();
@@ -805,26 +1048,29 @@ extern "C" int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* ou
     return matvec(s, u, out, alpha, rho, phi, as_stream(stream));
 }
 
-// One CG iteration (every kernel a no-op once ctrl[C_STOP] is set).
-static int cg_iteration(cmtip_acsolver_s* s, const float* d, float* x, double alpha, double rho, double phi,
-                        hipStream_t st) {
+// One CG iteration, the loop check of the next one at its end (every kernel a no-op
+// once ctrl[C_STOP] is set): p = r + beta p and q = W p (five passes, p.q partials in
+// the last), then x, r, rho and the check in one kernel.  Without the fold the M-grid
+// fourier_reg passes and a separate p.q reduction follow the five passes.
+static int cg_iteration(cmtip_acsolver_s* s, float* x, double rho, double phi, hipStream_t st) {
     const int64_t n = (int64_t)s->M * s->M * s->M;
-    double* sc = s->scal.as<double>();
     int* ctrl = s->ctrl.as<int>();
     float* r = s->r.as<float>();
     float* p = s->p.as<float>();
     float* q = s->q.as<float>();
-    (void)d;
-    hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(64), 0, st, (const double*)sc, ctrl);
-    hipLaunchKernelGGL(update_p_kernel, dim3(nblk(n, 2048)), dim3(kT), 0, st, (const float*)r, p, (const double*)sc,
-                       n, (const int*)ctrl);
-    CMTIP_LAUNCH_CHECK();
-    CMTIP_TRY(matvec(s, p, q, alpha, rho, phi, st));
-    CMTIP_TRY(dot(s, p, q, n, sc, S_PQ, st, ctrl));
-    hipLaunchKernelGGL(update_xr_kernel, dim3(kRedBlocks), dim3(kT), 0, st, x, r, (const float*)p, (const float*)q,
-                       (const double*)sc, n, s->part.as<double>(), (const int*)ctrl);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kRedBlocks), 0, st, s->part.as<const double>(), sc, 0, 1,
-                       (const int*)ctrl);
+    int npq = 0;
+    const bool fold = fold_factor(s) > 0;
+    CMTIP_TRY(fast_apply(s, r, q, (float)rho, true, fold ? &npq : nullptr, st));
+    if (!fold) {
+        const double M3 = (double)s->M * s->M * s->M;
+        CMTIP_TRY(reg_apply(s, p, q, (float)(phi / M3), q, 1.0f, st));
+        hipLaunchKernelGGL(dot_partial_kernel, dim3(kRedBlocks), dim3(kT), 0, st, (const float*)p, (const float*)q, n,
+                           s->pq_part.as<double>(), (const int*)ctrl);
+        npq = kRedBlocks;
+    }
+    hipLaunchKernelGGL(update_xr_check_kernel, dim3(kRedBlocks), dim3(kT), 0, st, x, r, (const float*)p,
+                       (const float*)q, s->scal.as<double>(), s->pq_part.as<const double>(), npq, n,
+                       s->part.as<double>(), ctrl, s->ticket.as<unsigned>());
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -867,13 +1113,21 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
                            s->r.as<float>(), n, (const int*)ctrl);
         rc = dot(s, s->r.as<const float>(), s->r.as<const float>(), n, sc, S_RHO, st, ctrl);
     }
+    if (rc == CMTIP_OK) {
+        // the loop check of the first iteration (each iteration ends with the next one's)
+        hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(64), 0, st, (const double*)sc, ctrl);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("acsolver_cg: launch failed");
+            rc = CMTIP_ERR_LAUNCH;
+        }
+    }
     // iterations: replay a captured block of kCgGraphIters iterations until the device stops
     hipGraphExec_t exec = nullptr;
     if (rc == CMTIP_OK && use_graphs() && maxiter > 0) {
         hipGraph_t graph = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             int crc = CMTIP_OK;
-            for (int k = 0; k < kCgGraphIters && crc == CMTIP_OK; ++k) crc = cg_iteration(s, d, x, alpha, rho, phi, st);
+            for (int k = 0; k < kCgGraphIters && crc == CMTIP_OK; ++k) crc = cg_iteration(s, x, rho, phi, st);
             hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
                                s->host_state_dev);
             const hipError_t e = hipStreamEndCapture(st, &graph);
@@ -894,7 +1148,7 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
                 break;
             }
         } else {
-            for (int k = 0; k < kCgGraphIters && rc == CMTIP_OK; ++k) rc = cg_iteration(s, d, x, alpha, rho, phi, st);
+            for (int k = 0; k < kCgGraphIters && rc == CMTIP_OK; ++k) rc = cg_iteration(s, x, rho, phi, st);
             hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
                                s->host_state_dev);
         }

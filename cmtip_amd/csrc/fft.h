@@ -80,11 +80,32 @@ struct AxisDesc {
     int mid_abs2 = 0;
     int64_t mid_sa = 0, mid_su = 0, mid_sv = 0;
 
+    // Hermitian half spectrum: with in_herm set, positions p in [n - ib, n) read
+    // conj(in[n - p]) (ig2 unused), so a line stored as its first n/2 + 1 values is
+    // extended to the full spectrum of a real signal.
+    int in_herm = 0;
+
+    // Load-phase CG direction update (real input only): p = r + beta p with
+    // beta = cg_scal[0] / cg_scal[1], or p = r when cg_ctrl[2] == 1 (first iteration);
+    // in points at r, the new p is written back to cg_p and transformed.
+    float* cg_p = nullptr;
+    const double* cg_scal = nullptr;
+    const int* cg_ctrl = nullptr;
+
+    // Real output modes: per-workgroup partial sums (double) of dot_src[off] * out[off]
+    // over the values stored, written to dot_part[blockIdx.x]; fft_axis_grid gives the
+    // number of workgroups.
+    const float* dot_src = nullptr;
+    double* dot_part = nullptr;
+
     // Optional device flag: when *skip != 0 the pass does nothing (lets a captured
     // sequence of passes be switched off from the device, e.g. after CG converged).
     const int* skip = nullptr;
 };
 
 int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream);
+
+// Number of workgroups fft_axis_pass launches for d (the length of dot_part it fills).
+int64_t fft_axis_grid(const Fft1D& plan, const AxisDesc& d);
 
 }  // namespace cmtip

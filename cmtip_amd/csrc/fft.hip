@@ -197,6 +197,14 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
     const uint32_t line0 = blockIdx.x * (uint32_t)a.lb;
     const uint32_t tot = (uint32_t)a.lb * n;
 
+    // CG direction update folded into the load: beta and the first-iteration flag
+    float cg_beta = 0.f;
+    bool cg_first = false;
+    if (IN_REAL && d.cg_p) {
+        cg_first = d.cg_ctrl[2] == 1;
+        cg_beta = cg_first ? 0.f : (float)(d.cg_scal[0] / d.cg_scal[1]);
+    }
+
     // ---- load (zero padding, input weights and masks folded in)
     for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
         uint32_t line, p;
@@ -205,18 +213,28 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
         float2 val = make_float2(0.f, 0.f);
         if (gl < a.nlines) {
             int g = -1;
-            if ((int)p < d.ia)
+            bool conj = false;
+            if ((int)p < d.ia) {
                 g = d.ig1 + (int)p;
-            else if ((int)p >= (int)n - d.ib)
-                g = d.ig2 + ((int)p - ((int)n - d.ib));
+            } else if ((int)p >= (int)n - d.ib) {
+                conj = d.in_herm != 0;
+                g = conj ? (int)n - (int)p : d.ig2 + ((int)p - ((int)n - d.ib));
+            }
             if (g >= 0) {
                 const uint32_t u = fdiv(gl, a.fnv);
                 const uint32_t v = gl - u * (uint32_t)d.nv;
                 const int64_t off = (int64_t)u * d.in_su + (int64_t)v * d.in_sv + (int64_t)g * d.in_sa;
-                if (IN_REAL)
-                    val = make_float2(reinterpret_cast<const float*>(d.in)[off], 0.f);
-                else
+                if (IN_REAL) {
+                    float x = reinterpret_cast<const float*>(d.in)[off];
+                    if (d.cg_p) {
+                        x = cg_first ? x : cg_beta * d.cg_p[off] + x;
+                        d.cg_p[off] = x;
+                    }
+                    val = make_float2(x, 0.f);
+                } else {
                     val = reinterpret_cast<const float2*>(d.in)[off];
+                    if (conj) val.y = -val.y;
+                }
                 float wgt = 1.f;
                 if (d.in_w) wgt *= d.in_w[p];
                 if (d.in_mask) wgt *= d.in_mask[off];
@@ -261,6 +279,7 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
     }
 
     // ---- store (cropping, output weights, masks and axpy folded in)
+    double dot_acc = 0.0;
     for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
         uint32_t line, p;
         split_elem(a, e, line, p);
@@ -300,8 +319,30 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
                 y = r;
             }
             o[off] = y;
+            if (d.dot_part) dot_acc += (double)d.dot_src[off] * (double)y;
         }
     }
+    if (OUT_MODE != 0 && OUT_MODE != 3 && d.dot_part) {
+        // fixed-order workgroup sum: wave shuffles, then the waves in index order
+        __shared__ double wsum[kThreads / 64];
+        for (int o = 32; o > 0; o >>= 1) dot_acc += __shfl_down(dot_acc, o, 64);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = dot_acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) t += wsum[w];
+            d.dot_part[blockIdx.x] = t;
+        }
+    }
+}
+
+// lines per workgroup: a power of two that keeps two LDS line buffers under budget
+int lines_per_group(int n) {
+    const int ldl = n | 1;
+    int lb = 1;
+    const int budget = lds_budget();
+    while (lb < 64 && 2 * (2 * lb) * ldl * (int)sizeof(float2) <= budget) lb *= 2;
+    return lb;
 }
 
 void factorize(int n, std::vector<int>& f) {
@@ -357,6 +398,11 @@ int fft1d_init(Fft1D& p, int n) {
 
 void fft1d_free(Fft1D& p) { p.tw.release(); }
 
+int64_t fft_axis_grid(const Fft1D& plan, const AxisDesc& d) {
+    const int64_t lb = lines_per_group(plan.n);
+    return ((int64_t)d.nu * d.nv + lb - 1) / lb;
+}
+
 int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     PassArgs a;
     a.d = d;
@@ -367,7 +413,8 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
     if (d.ia + d.ib > plan.n || d.oa + d.ob > plan.n ||
         ((d.mid_c != nullptr) + (d.mid_r != nullptr) + (d.mid_amp != nullptr) + (d.mid_abs2 != 0)) > 1 ||
         (d.upd_mode && (d.out_mode != 1 || !d.upd_support)) || d.out_mode < 0 || d.out_mode > 3 ||
-        (d.in_real && d.out_mode == 3)) {
+        (d.in_real && d.out_mode == 3) || (d.cg_p && (!d.in_real || !d.cg_scal || !d.cg_ctrl)) ||
+        (d.dot_part && (!d.dot_src || (d.out_mode != 1 && d.out_mode != 2))) || (d.in_herm && d.in_real)) {
         set_error("fft_axis_pass: bad segments (n=%d ia=%d ib=%d oa=%d ob=%d)", plan.n, d.ia, d.ib,
                   d.oa, d.ob);
         return CMTIP_ERR_ARG;
@@ -376,11 +423,8 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
         set_error("fft_axis_pass: too many lines");
         return CMTIP_ERR_ARG;
     }
-    // lines per workgroup: a power of two that keeps two LDS line buffers under budget
     const int ldl = plan.n | 1;
-    int lb = 1;
-    const int budget = lds_budget();
-    while (lb < 64 && 2 * (2 * lb) * ldl * (int)sizeof(float2) <= budget) lb *= 2;
+    const int lb = lines_per_group(plan.n);
     int lbshift = 0;
     while ((1 << lbshift) < lb) ++lbshift;
     a.lb = lb;
