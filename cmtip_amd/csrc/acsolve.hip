@@ -22,9 +22,9 @@
 //    a real multiplier G can be replaced by its symmetric part (G[k] + G[-k]) / 2,
 //    which makes every intermediate Hermitian: passes 1-4 keep only the first L/2 + 1
 //    positions of the last axis and pass 5 extends them by conjugate symmetry.
-//  - the CG direction update runs inside pass 1's load, p.q inside pass 5's store,
-//    and x / r updates, the r.r reduction and the loop check in one kernel, so an
-//    iteration is six launches.
+//  - the CG direction update runs inside pass 1's load and the p.q partials inside
+//    pass 5's store; one kernel updates x and r, one block reduces r.r and runs the
+//    loop check, so an iteration is seven launches.
 #include <math.h>
 
 #include <vector>
@@ -46,7 +46,6 @@ struct cmtip_acsolver_s {
     DevBuf ctrl;     // int32 CG state: stop, converged, iterations, maxiter
     DevBuf Gh;       // L * L * (L/2 + 1) float: folded, symmetrised multiplier of W (half spectrum)
     DevBuf pq_part;  // per-workgroup partial sums of p.q
-    DevBuf ticket;   // uint32 arrival counter of the update kernel (back to 0 after each launch)
     double g_alpha = 0.0, g_phi = 0.0;
     bool g_valid = false;  // Gh matches Fk, mask, g_alpha and g_phi
     int* host_state = nullptr;  // pinned host-mapped mirror: stop, converged, iterations, pad, rho (double)
@@ -238,18 +237,14 @@ __global__ void copy_if_zero_rhs_kernel(const float* __restrict__ b, float* __re
 
 // alpha = rho / (p.q), with p.q the fixed-order sum of the npq partials (every
 // workgroup adds them in the same order, so all agree bit for bit); x += alpha p;
-// r -= alpha q; per-workgroup partials of r.r.  The last workgroup to arrive adds
-// those partials in a fixed order, rotates rho and runs the loop check of the next
-// iteration (cg_check_kernel's rule), then resets the arrival counter.
-__global__ void update_xr_check_kernel(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
-                                       const float* __restrict__ q, double* __restrict__ s,
-                                       const double* __restrict__ pq_part, int npq, int64_t n,
-                                       double* __restrict__ part, int* __restrict__ ctrl,
-                                       unsigned* __restrict__ ticket) {
+// r -= alpha q; per-workgroup partials of r.r.
+__global__ void update_xr_kernel(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+                                 const float* __restrict__ q, const double* __restrict__ s,
+                                 const double* __restrict__ pq_part, int npq, int64_t n,
+                                 double* __restrict__ part, const int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
     __shared__ double red[kT / 64];
     __shared__ double pq_s;
-    __shared__ bool last;
     double v = 0.0;
     for (int i = threadIdx.x; i < npq; i += kT) v += pq_part[i];
     v = wave_sum(v);
@@ -261,8 +256,7 @@ __global__ void update_xr_check_kernel(float* __restrict__ x, float* __restrict_
         pq_s = t;
     }
     __syncthreads();
-    const double pq = pq_s;
-    const float alpha = (float)(s[S_RHO] / pq);
+    const float alpha = (float)(s[S_RHO] / pq_s);
     double acc = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kT;
     for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride) {
@@ -271,41 +265,31 @@ __global__ void update_xr_check_kernel(float* __restrict__ x, float* __restrict_
         r[i] = rn;
         acc += (double)rn * (double)rn;
     }
-    __syncthreads();  // red is reused
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < kT / 64; ++i) t += red[i];
-        part[blockIdx.x] = t;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    __shared__ double tree[kRedBlocks];
-    for (int i = threadIdx.x; i < kRedBlocks; i += kT)
-        tree[i] = i < (int)gridDim.x ? __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+    block_partial(acc, part);
+}
+
+// rho_prev = rho, rho = sum of the kRedBlocks r.r partials (fixed order), then the loop
+// check of the next iteration (cg_check_kernel's rule)
+__global__ void rotate_check_kernel(const double* __restrict__ part, double* __restrict__ s, int* __restrict__ ctrl) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double red[kRedBlocks];
+    red[threadIdx.x] = part[threadIdx.x];
     __syncthreads();
     for (int h = kRedBlocks / 2; h > 0; h >>= 1) {
-        for (int i = threadIdx.x; i < h; i += kT) tree[i] += tree[i + h];
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        s[S_PQ] = pq;
-        s[S_RHO_PREV] = s[S_RHO];
-        s[S_RHO] = tree[0];
-        if (ctrl[C_ITERS] >= ctrl[C_MAXITER]) {
-            ctrl[C_STOP] = 1;
-        } else if (sqrt(tree[0]) < s[S_ATOL]) {
-            ctrl[C_CONV] = 1;
-            ctrl[C_STOP] = 1;
-        } else {
-            ctrl[C_ITERS] += 1;
-        }
-        *ticket = 0u;
+    if (threadIdx.x != 0) return;
+    const double rho = red[0];
+    s[S_RHO_PREV] = s[S_RHO];
+    s[S_RHO] = rho;
+    if (ctrl[C_ITERS] >= ctrl[C_MAXITER]) {
+        ctrl[C_STOP] = 1;
+    } else if (sqrt(rho) < s[S_ATOL]) {
+        ctrl[C_CONV] = 1;
+        ctrl[C_STOP] = 1;
+    } else {
+        ctrl[C_ITERS] += 1;
     }
 }
 
@@ -776,11 +760,6 @@ extern "C" int cmtip_acsolver_create(int M, int L, cmtip_acsolver* out) {
     if (st == CMTIP_OK) st = s->ctrl.ensure(sizeof(int) * 4);
     if (st == CMTIP_OK) st = s->Gh.ensure(sizeof(float) * (size_t)L * L * (L / 2 + 1));
     if (st == CMTIP_OK) st = s->pq_part.ensure(sizeof(double) * (size_t)(M * M > kRedBlocks ? M * M : kRedBlocks));
-    if (st == CMTIP_OK) st = s->ticket.ensure(sizeof(unsigned));
-    if (st == CMTIP_OK && hipMemset(s->ticket.p, 0, sizeof(unsigned)) != hipSuccess) {
-        set_error("acsolver_create: counter initialisation failed");
-        st = CMTIP_ERR_HIP;
-    }
     if (st == CMTIP_OK) {
         void* h = nullptr;
         if (hipHostMalloc(&h, 64, hipHostMallocMapped) != hipSuccess ||
@@ -871,7 +850,6 @@ extern "C" int cmtip_acsolver_destroy(cmtip_acsolver s) {
     s->ctrl.release();
     s->Gh.release();
     s->pq_part.release();
-    s->ticket.release();
     if (s->host_state) (void)hipHostFree(s->host_state);
     if (s->ev_in) (void)hipEventDestroy(s->ev_in);
     if (s->ev_out) (void)hipEventDestroy(s->ev_out);
@@ -1050,7 +1028,7 @@ extern "C" int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* ou
 
 // One CG iteration, the loop check of the next one at its end (every kernel a no-op
 // once ctrl[C_STOP] is set): p = r + beta p and q = W p (five passes, p.q partials in
-// the last), then x, r, rho and the check in one kernel.  Without the fold the M-grid
+// the last), then x and r, then rho and the check.  Without the fold the M-grid
 // fourier_reg passes and a separate p.q reduction follow the five passes.
 static int cg_iteration(cmtip_acsolver_s* s, float* x, double rho, double phi, hipStream_t st) {
     const int64_t n = (int64_t)s->M * s->M * s->M;
@@ -1068,9 +1046,11 @@ static int cg_iteration(cmtip_acsolver_s* s, float* x, double rho, double phi, h
                            s->pq_part.as<double>(), (const int*)ctrl);
         npq = kRedBlocks;
     }
-    hipLaunchKernelGGL(update_xr_check_kernel, dim3(kRedBlocks), dim3(kT), 0, st, x, r, (const float*)p,
-                       (const float*)q, s->scal.as<double>(), s->pq_part.as<const double>(), npq, n,
-                       s->part.as<double>(), ctrl, s->ticket.as<unsigned>());
+    hipLaunchKernelGGL(update_xr_kernel, dim3(kRedBlocks), dim3(kT), 0, st, x, r, (const float*)p, (const float*)q,
+                       (const double*)s->scal.as<double>(), s->pq_part.as<const double>(), npq, n, s->part.as<double>(),
+                       (const int*)ctrl);
+    hipLaunchKernelGGL(rotate_check_kernel, dim3(1), dim3(kRedBlocks), 0, st, s->part.as<const double>(),
+                       s->scal.as<double>(), ctrl);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

@@ -4,8 +4,9 @@
 // transforms on graphics processors", SC 2008): with Ns the product of the
 // radices already applied, butterfly j of a radix-R stage reads x[j + r*n/R],
 // twiddles by exp(sign*2*pi*i*r*(j mod Ns)/(Ns*R)), does an R-point DFT and
-// writes to (j div Ns)*Ns*R + (j mod Ns) + r*Ns.  Radices 4, 2, 3 and 5 have
-// hand-written butterflies; any other prime factor (e.g. 41 or 151, the
+// writes to (j div Ns)*Ns*R + (j mod Ns) + r*Ns.  Radices 4, 9, 2, 3 and 5 have
+// hand-written butterflies (9 as 3 x 3 in registers, so 162 = 9 * 9 * 2 and 81 = 9 * 9
+// take three and two stages); any other prime factor (e.g. 41 or 151, the
 // autocorrelation edge lengths whose Fourier-corner projection needs an exact
 // M-point DFT) runs a direct O(R^2) DFT stage.
 #include <math.h>
@@ -20,22 +21,25 @@ namespace cmtip {
 namespace {
 
 constexpr int kThreads = 256;
-// LDS per workgroup for the two line buffers.  Strided passes want many lines per
-// workgroup (consecutive lines are contiguous in memory, so more lines make longer
-// contiguous runs); CMTIP_FFT_LDS_KB overrides the budget for tuning.
+// LDS per workgroup for the two line buffers.  More lines per workgroup make longer
+// contiguous runs on strided axes, fewer let more workgroups stay resident: a pass is
+// one round of resident workgroups when it fits, and its time is then one
+// workgroup's latency.  32 KB (8 lines of 162, 16 of 81; 64-byte runs, 7 workgroups
+// per CU) measured best for the autocorrelation operator at M = 81 and equal to 64 KB
+// for the NUFFT grids; CMTIP_FFT_LDS_KB overrides it for tuning.
 int lds_budget() {
     static int b = 0;
     if (!b) {
         const char* e = getenv("CMTIP_FFT_LDS_KB");
-        const int kb = e ? atoi(e) : 64;
-        b = (kb >= 16 && kb <= 160 ? kb : 64) * 1024;
+        const int kb = e ? atoi(e) : 32;
+        b = (kb >= 16 && kb <= 128 ? kb : 32) * 1024;
     }
     return b;
 }
 
 struct PassArgs {
     AxisDesc d;
-    int n, nst, lb, lbshift, ldl, ax_contig;
+    int n, nst, lb, lbshift, ldl, ax_contig, ntw;
     uint32_t nlines;
     FastDiv fn, fnv;
     int R[kFftMaxStages], Ns[kFftMaxStages], twoff[kFftMaxStages], omoff[kFftMaxStages];
@@ -87,16 +91,40 @@ __device__ __forceinline__ void butterfly(float2* v, float sg) {
         v[4] = csub(m1, t1);
         v[2] = cadd(m2, t2);
         v[3] = csub(m2, t2);
+    } else if constexpr (R == 9) {
+        // 3 x 3: n = 3 n1 + n2, k = k1 + 3 k2; 3-point DFTs over n1, twiddles
+        // w9^(n2 k1), 3-point DFTs over n2
+        float2 a0[3] = {v[0], v[3], v[6]}, a1[3] = {v[1], v[4], v[7]}, a2[3] = {v[2], v[5], v[8]};
+        butterfly<3>(a0, sg);
+        butterfly<3>(a1, sg);
+        butterfly<3>(a2, sg);
+        const float c1 = 0.76604444311897803520f, s1 = 0.64278760968653932632f;   // 2 pi / 9
+        const float c2 = 0.17364817766693034885f, s2 = 0.98480775301220805936f;   // 4 pi / 9
+        const float c4 = -0.93969262078590838405f, s4 = 0.34202014332566873304f;  // 8 pi / 9
+        a1[1] = cmul(a1[1], make_float2(c1, sg * s1));
+        a1[2] = cmul(a1[2], make_float2(c2, sg * s2));
+        a2[1] = cmul(a2[1], make_float2(c2, sg * s2));
+        a2[2] = cmul(a2[2], make_float2(c4, sg * s4));
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            float2 b[3] = {a0[k1], a1[k1], a2[k1]};
+            butterfly<3>(b, sg);
+            v[k1] = b[0];
+            v[k1 + 3] = b[1];
+            v[k1 + 6] = b[2];
+        }
     }
 }
 
+constexpr bool has_butterfly(int R) { return R == 2 || R == 3 || R == 4 || R == 5 || R == 9; }
+
 template <int R>
-__device__ __forceinline__ void radix_stage(const PassArgs& a, int s, const float2* __restrict__ src,
-                                            float2* __restrict__ dst, float sg) {
+__device__ __forceinline__ void radix_stage(const PassArgs& a, const float2* __restrict__ twt, int s,
+                                            const float2* __restrict__ src, float2* __restrict__ dst, float sg) {
     const uint32_t nb = (uint32_t)(a.n / R);
     const uint32_t Ns = (uint32_t)a.Ns[s];
     const uint32_t total = (uint32_t)a.lb * nb;
-    const float2* tw = a.tw + a.twoff[s];
+    const float2* tw = twt + a.twoff[s];
     for (uint32_t b = threadIdx.x; b < total; b += kThreads) {
         const uint32_t line = fdiv(b, a.fnb[s]);
         const uint32_t j = b - line * nb;
@@ -121,15 +149,15 @@ __device__ __forceinline__ void radix_stage(const PassArgs& a, int s, const floa
     }
 }
 
-__device__ void generic_stage(const PassArgs& a, int s, const float2* __restrict__ src,
-                              float2* __restrict__ dst, float sg) {
+__device__ void generic_stage(const PassArgs& a, const float2* __restrict__ twt, int s,
+                              const float2* __restrict__ src, float2* __restrict__ dst, float sg) {
     const int R = a.R[s];
     const uint32_t nb = (uint32_t)(a.n / R);
     const uint32_t Ns = (uint32_t)a.Ns[s];
     const uint32_t n = (uint32_t)a.n;
     const uint32_t total = (uint32_t)a.lb * n;
-    const float2* tw = a.tw + a.twoff[s];
-    const float2* om = a.tw + a.omoff[s];
+    const float2* tw = twt + a.twoff[s];
+    const float2* om = twt + a.omoff[s];
     for (uint32_t b = threadIdx.x; b < total; b += kThreads) {
         const uint32_t line = fdiv(b, a.fn);
         const uint32_t rem = b - line * n;
@@ -156,17 +184,21 @@ __device__ void generic_stage(const PassArgs& a, int s, const float2* __restrict
     }
 }
 
-// Runs all stages; returns the buffer holding the result.
-__device__ float2* run_stages(const PassArgs& a, float2* b0, float2* b1, float sg) {
+// Runs all stages (twiddles read from twt); returns the buffer holding the result.
+__device__ float2* run_stages(const PassArgs& a, const float2* twt, float2* b0, float2* b1, float sg) {
     float2* src = b0;
     float2* dst = b1;
+    // kept rolled: unrolled over kFftMaxStages with every radix inlined in each copy
+    // the kernel outgrows the instruction cache
+#pragma unroll 1
     for (int s = 0; s < a.nst; ++s) {
         switch (a.R[s]) {
-            case 2: radix_stage<2>(a, s, src, dst, sg); break;
-            case 3: radix_stage<3>(a, s, src, dst, sg); break;
-            case 4: radix_stage<4>(a, s, src, dst, sg); break;
-            case 5: radix_stage<5>(a, s, src, dst, sg); break;
-            default: generic_stage(a, s, src, dst, sg); break;
+            case 2: radix_stage<2>(a, twt, s, src, dst, sg); break;
+            case 3: radix_stage<3>(a, twt, s, src, dst, sg); break;
+            case 4: radix_stage<4>(a, twt, s, src, dst, sg); break;
+            case 5: radix_stage<5>(a, twt, s, src, dst, sg); break;
+            case 9: radix_stage<9>(a, twt, s, src, dst, sg); break;
+            default: generic_stage(a, twt, s, src, dst, sg); break;
         }
         __syncthreads();
         float2* t = src;
@@ -186,16 +218,29 @@ __device__ __forceinline__ void split_elem(const PassArgs& a, uint32_t e, uint32
     }
 }
 
+// Elements per thread per step of the load and store phases: every global load of a
+// step is issued before any of its results is used, so a step waits for one memory
+// latency instead of kU.
+constexpr int kU = 4;
+// Mid-pass multipliers a thread prefetches into registers before the forward stages
+// (covers lb * n <= kMidPre * kThreads, which the default LDS budget guarantees).
+constexpr int kMidPre = 16;
+
 template <bool IN_REAL, int OUT_MODE, bool MID>
 __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
     if (a.d.skip && *a.d.skip) return;
     extern __shared__ float2 lds[];
     float2* b0 = lds;
     float2* b1 = lds + a.lb * a.ldl;
+    float2* twl = lds + 2 * a.lb * a.ldl;
     const AxisDesc& d = a.d;
     const uint32_t n = (uint32_t)a.n;
     const uint32_t line0 = blockIdx.x * (uint32_t)a.lb;
     const uint32_t tot = (uint32_t)a.lb * n;
+
+    // twiddle table to LDS (read by every butterfly) when it fits beside the lines
+    for (int i = threadIdx.x; i < a.ntw; i += kThreads) twl[i] = a.tw[i];
+    const float2* twt = a.ntw ? twl : a.tw;
 
     // CG direction update folded into the load: beta and the first-iteration flag
     float cg_beta = 0.f;
@@ -205,13 +250,18 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
         cg_beta = cg_first ? 0.f : (float)(d.cg_scal[0] / d.cg_scal[1]);
     }
 
-    // ---- load (zero padding, input weights and masks folded in)
-    for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
-        uint32_t line, p;
-        split_elem(a, e, line, p);
-        const uint32_t gl = line0 + line;
-        float2 val = make_float2(0.f, 0.f);
-        if (gl < a.nlines) {
+    // ---- load (zero padding, input weights and masks folded in); positions outside the
+    // input segments read element 0 and are zeroed, so the loads are unconditional
+    for (uint32_t e0 = threadIdx.x; e0 < tot; e0 += kU * kThreads) {
+        float2 val[kU];
+        uint32_t slot[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const uint32_t e = e0 + k * kThreads;
+            uint32_t line, p;
+            split_elem(a, e, line, p);
+            slot[k] = line * a.ldl + p;
+            const uint32_t gl = line0 + line;
             int g = -1;
             bool conj = false;
             if ((int)p < d.ia) {
@@ -220,46 +270,69 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
                 conj = d.in_herm != 0;
                 g = conj ? (int)n - (int)p : d.ig2 + ((int)p - ((int)n - d.ib));
             }
-            if (g >= 0) {
+            const bool ok = e < tot && gl < a.nlines && g >= 0;
+            int64_t off = 0;
+            if (ok) {
                 const uint32_t u = fdiv(gl, a.fnv);
                 const uint32_t v = gl - u * (uint32_t)d.nv;
-                const int64_t off = (int64_t)u * d.in_su + (int64_t)v * d.in_sv + (int64_t)g * d.in_sa;
-                if (IN_REAL) {
-                    float x = reinterpret_cast<const float*>(d.in)[off];
-                    if (d.cg_p) {
-                        x = cg_first ? x : cg_beta * d.cg_p[off] + x;
-                        d.cg_p[off] = x;
-                    }
-                    val = make_float2(x, 0.f);
-                } else {
-                    val = reinterpret_cast<const float2*>(d.in)[off];
-                    if (conj) val.y = -val.y;
+                off = (int64_t)u * d.in_su + (int64_t)v * d.in_sv + (int64_t)g * d.in_sa;
+            }
+            float2 x;
+            if (IN_REAL) {
+                float r = reinterpret_cast<const float*>(d.in)[off];
+                if (d.cg_p) {
+                    const float pv = d.cg_p[off];
+                    r = cg_first ? r : cg_beta * pv + r;
+                    if (ok) d.cg_p[off] = r;
                 }
-                float wgt = 1.f;
-                if (d.in_w) wgt *= d.in_w[p];
-                if (d.in_mask) wgt *= d.in_mask[off];
-                val = cscale(val, wgt);
+                x = make_float2(r, 0.f);
+            } else {
+                x = reinterpret_cast<const float2*>(d.in)[off];
+                if (conj) x.y = -x.y;
+            }
+            float wgt = ok ? 1.f : 0.f;
+            if (d.in_w) wgt *= d.in_w[ok ? p : 0];
+            if (d.in_mask) wgt *= d.in_mask[off];
+            val[k] = cscale(x, wgt);
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k)
+            if (e0 + k * kThreads < tot) b0[slot[k]] = val[k];
+    }
+
+    // mid-pass multipliers for this thread's elements, fetched while the forward stages run
+    float2 midv[MID ? kMidPre : 1];
+    const bool mid_pre = MID && tot <= (uint32_t)(kMidPre * kThreads);
+    if (MID && mid_pre && !d.mid_abs2) {
+#pragma unroll
+        for (int k = 0; k < kMidPre; ++k) {
+            const uint32_t e = threadIdx.x + k * kThreads;
+            midv[k] = make_float2(0.f, 0.f);
+            if (e < tot) {
+                uint32_t line, p;
+                split_elem(a, e, line, p);
+                const uint32_t gl = line0 + line;
+                if (gl < a.nlines) {
+                    const uint32_t u = fdiv(gl, a.fnv);
+                    const uint32_t v = gl - u * (uint32_t)d.nv;
+                    const int64_t off = (int64_t)u * d.mid_su + (int64_t)v * d.mid_sv + (int64_t)p * d.mid_sa;
+                    if (d.mid_c)
+                        midv[k] = d.mid_c[off];
+                    else
+                        midv[k].x = d.mid_amp ? d.mid_amp[off] : d.mid_r[off];
+                }
             }
         }
-        b0[line * a.ldl + p] = val;
     }
     __syncthreads();
 
     const float sg = (float)d.sign;
-    float2* res = run_stages(a, b0, b1, sg);
+    float2* res = run_stages(a, twt, b0, b1, sg);
 
     if (MID) {
-        for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
-            uint32_t line, p;
-            split_elem(a, e, line, p);
-            const uint32_t gl = line0 + line;
-            if (gl >= a.nlines) continue;
-            const uint32_t u = fdiv(gl, a.fnv);
-            const uint32_t v = gl - u * (uint32_t)d.nv;
-            const int64_t off = (int64_t)u * d.mid_su + (int64_t)v * d.mid_sv + (int64_t)p * d.mid_sa;
-            float2 x = res[line * a.ldl + p];
+        auto apply = [&](float2 x, float2 m) -> float2 {
             if (d.mid_amp) {
-                const float am = d.mid_amp[off];
+                const float am = m.x;
                 if (am >= 0.f) {
                     const float r = sqrtf(x.x * x.x + x.y * x.y);
                     x = r > 0.f ? cscale(x, am / r) : make_float2(am, 0.f);
@@ -267,59 +340,114 @@ __global__ void __launch_bounds__(kThreads) axis_pass_kernel(PassArgs a) {
             } else if (d.mid_abs2) {
                 x = make_float2(x.x * x.x + x.y * x.y, 0.f);
             } else if (d.mid_c) {
-                x = cmul(x, d.mid_c[off]);
+                x = cmul(x, m);
             } else {
-                x = cscale(x, d.mid_r[off]);
+                x = cscale(x, m.x);
             }
-            res[line * a.ldl + p] = x;
+            return x;
+        };
+        if (mid_pre) {
+#pragma unroll
+            for (int k = 0; k < kMidPre; ++k) {
+                const uint32_t e = threadIdx.x + k * kThreads;
+                if (e >= tot) break;
+                uint32_t line, p;
+                split_elem(a, e, line, p);
+                if (line0 + line >= a.nlines) continue;
+                float2& x = res[line * a.ldl + p];
+                x = apply(x, midv[k]);
+            }
+        } else {
+            for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
+                uint32_t line, p;
+                split_elem(a, e, line, p);
+                const uint32_t gl = line0 + line;
+                if (gl >= a.nlines) continue;
+                const uint32_t u = fdiv(gl, a.fnv);
+                const uint32_t v = gl - u * (uint32_t)d.nv;
+                const int64_t off = (int64_t)u * d.mid_su + (int64_t)v * d.mid_sv + (int64_t)p * d.mid_sa;
+                float2 m = make_float2(0.f, 0.f);
+                if (d.mid_c)
+                    m = d.mid_c[off];
+                else if (!d.mid_abs2)
+                    m.x = d.mid_amp ? d.mid_amp[off] : d.mid_r[off];
+                float2& x = res[line * a.ldl + p];
+                x = apply(x, m);
+            }
         }
         __syncthreads();
         float2* other = (res == b0) ? b1 : b0;
-        res = run_stages(a, res, other, -sg);
+        res = run_stages(a, twt, res, other, -sg);
     }
 
-    // ---- store (cropping, output weights, masks and axpy folded in)
+    // ---- store (cropping, output weights, masks and axpy folded in); the step's side
+    // inputs (masks, axpy source, dot source, the array updated in place) are loaded
+    // unconditionally from element 0 where the position is not stored
     double dot_acc = 0.0;
-    for (uint32_t e = threadIdx.x; e < tot; e += kThreads) {
-        uint32_t line, p;
-        split_elem(a, e, line, p);
-        const uint32_t gl = line0 + line;
-        if (gl >= a.nlines) continue;
-        int g = -1;
-        if ((int)p < d.oa)
-            g = d.og1 + (int)p;
-        else if ((int)p >= (int)n - d.ob)
-            g = d.og2 + ((int)p - ((int)n - d.ob));
-        if (g < 0) continue;
-        const uint32_t u = fdiv(gl, a.fnv);
-        const uint32_t v = gl - u * (uint32_t)d.nv;
-        const int64_t off = (int64_t)u * d.out_su + (int64_t)v * d.out_sv + (int64_t)g * d.out_sa;
-        float wgt = d.out_scale;
-        if (d.out_w) wgt *= d.out_w[p];
-        if (d.out_mask) wgt *= d.out_mask[off];
-        float2 x = cscale(res[line * a.ldl + p], wgt);
-        if (OUT_MODE == 0) {
-            reinterpret_cast<float2*>(d.out)[off] = x;
-        } else if (OUT_MODE == 3) {
-            reinterpret_cast<float*>(d.out)[off] = sqrtf(x.x * x.x + x.y * x.y);
-        } else {
-            float y = OUT_MODE == 1 ? x.x : x.y;
-            if (d.add_src) y += d.add_scale * d.add_src[off];
-            float* o = reinterpret_cast<float*>(d.out);
-            if (OUT_MODE == 1 && d.upd_mode) {
-                const bool keep = d.upd_support[off] > 0.f && y > 0.f;
-                float r;
-                if (d.upd_mode == 1) {
-                    r = keep ? y : 0.f;
-                    if (y > d.upd_rho_max) r = d.upd_rho_max;
-                } else {
-                    r = keep ? y : o[off] - d.upd_beta * y;
-                    if (y > d.upd_rho_max) r += 2.f * d.upd_beta * y - d.upd_rho_max;
-                }
-                y = r;
+    for (uint32_t e0 = threadIdx.x; e0 < tot; e0 += kU * kThreads) {
+        int64_t offs[kU];
+        bool oks[kU];
+        float wg[kU], addv[kU], dotv[kU], oldv[kU], supv[kU];
+        float2 xs[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const uint32_t e = e0 + k * kThreads;
+            uint32_t line, p;
+            split_elem(a, e, line, p);
+            const uint32_t gl = line0 + line;
+            int g = -1;
+            if ((int)p < d.oa)
+                g = d.og1 + (int)p;
+            else if ((int)p >= (int)n - d.ob)
+                g = d.og2 + ((int)p - ((int)n - d.ob));
+            const bool ok = e < tot && gl < a.nlines && g >= 0;
+            int64_t off = 0;
+            if (ok) {
+                const uint32_t u = fdiv(gl, a.fnv);
+                const uint32_t v = gl - u * (uint32_t)d.nv;
+                off = (int64_t)u * d.out_su + (int64_t)v * d.out_sv + (int64_t)g * d.out_sa;
             }
-            o[off] = y;
-            if (d.dot_part) dot_acc += (double)d.dot_src[off] * (double)y;
+            offs[k] = off;
+            oks[k] = ok;
+            float w = d.out_scale;
+            if (d.out_w) w *= d.out_w[ok ? p : 0];
+            if (d.out_mask) w *= d.out_mask[off];
+            wg[k] = w;
+            xs[k] = ok ? res[line * a.ldl + p] : make_float2(0.f, 0.f);
+            if (OUT_MODE == 1 || OUT_MODE == 2) {
+                addv[k] = d.add_src ? d.add_src[off] : 0.f;
+                dotv[k] = d.dot_part ? d.dot_src[off] : 0.f;
+                oldv[k] = (OUT_MODE == 1 && d.upd_mode) ? reinterpret_cast<const float*>(d.out)[off] : 0.f;
+                supv[k] = (OUT_MODE == 1 && d.upd_mode) ? d.upd_support[off] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            if (!oks[k]) continue;
+            const int64_t off = offs[k];
+            const float2 x = cscale(xs[k], wg[k]);
+            if (OUT_MODE == 0) {
+                reinterpret_cast<float2*>(d.out)[off] = x;
+            } else if (OUT_MODE == 3) {
+                reinterpret_cast<float*>(d.out)[off] = sqrtf(x.x * x.x + x.y * x.y);
+            } else {
+                float y = OUT_MODE == 1 ? x.x : x.y;
+                if (d.add_src) y += d.add_scale * addv[k];
+                if (OUT_MODE == 1 && d.upd_mode) {
+                    const bool keep = supv[k] > 0.f && y > 0.f;
+                    float r;
+                    if (d.upd_mode == 1) {
+                        r = keep ? y : 0.f;
+                        if (y > d.upd_rho_max) r = d.upd_rho_max;
+                    } else {
+                        r = keep ? y : oldv[k] - d.upd_beta * y;
+                        if (y > d.upd_rho_max) r += 2.f * d.upd_beta * y - d.upd_rho_max;
+                    }
+                    y = r;
+                }
+                reinterpret_cast<float*>(d.out)[off] = y;
+                if (d.dot_part) dot_acc += (double)dotv[k] * (double)y;
+            }
         }
     }
     if (OUT_MODE != 0 && OUT_MODE != 3 && d.dot_part) {
@@ -348,6 +476,7 @@ int lines_per_group(int n) {
 void factorize(int n, std::vector<int>& f) {
     f.clear();
     while (n % 4 == 0) { f.push_back(4); n /= 4; }
+    while (n % 9 == 0) { f.push_back(9); n /= 9; }
     while (n % 2 == 0) { f.push_back(2); n /= 2; }
     while (n % 3 == 0) { f.push_back(3); n /= 3; }
     while (n % 5 == 0) { f.push_back(5); n /= 5; }
@@ -383,7 +512,7 @@ int fft1d_init(Fft1D& p, int n) {
                 tw.push_back(make_float2((float)cos(ang), (float)sin(ang)));
             }
         p.omoff[s] = (int)tw.size();
-        if (R > 5)
+        if (!has_butterfly(R))
             for (int m = 0; m < R; ++m) {
                 const double ang = -2.0 * M_PI * (double)m / (double)R;
                 tw.push_back(make_float2((float)cos(ang), (float)sin(ang)));
@@ -391,6 +520,7 @@ int fft1d_init(Fft1D& p, int n) {
         Ns *= R;
     }
     if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    p.ntw = (int)tw.size();
     CMTIP_TRY(p.tw.ensure(tw.size() * sizeof(float2)));
     CMTIP_HIP(hipMemcpy(p.tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
     return CMTIP_OK;
@@ -443,16 +573,21 @@ int fft_axis_pass(const Fft1D& plan, const AxisDesc& d, hipStream_t stream) {
         a.fNs[s] = make_fastdiv((uint32_t)plan.Ns[s]);
     }
     a.tw = plan.tw.as<const float2>();
-    const size_t lds = (size_t)2 * lb * ldl * sizeof(float2);
+    a.ntw = ((size_t)2 * lb * ldl + plan.ntw) * sizeof(float2) <= 156 * 1024 ? plan.ntw : 0;
+    const size_t lds = ((size_t)2 * lb * ldl + a.ntw) * sizeof(float2);
     const unsigned grid = (unsigned)((nlines + lb - 1) / lb);
     const bool mid = d.mid_c || d.mid_r || d.mid_amp || d.mid_abs2;
 #define CMTIP_AXIS_LAUNCH(IR, OM, MD)                                                                       \
     do {                                                                                                    \
-        static bool attr_set_ = false;                                                                      \
-        if (!attr_set_ && lds > 64 * 1024) {                                                                \
-            (void)hipFuncSetAttribute((const void*)axis_pass_kernel<IR, OM, MD>,                            \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);              \
-            attr_set_ = true;                                                                               \
+        static size_t attr_bytes_ = 64 * 1024; /* dynamic LDS the kernel accepts */                         \
+        if (lds > attr_bytes_) {                                                                            \
+            if (hipFuncSetAttribute((const void*)axis_pass_kernel<IR, OM, MD>,                              \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {  \
+                (void)hipGetLastError();                                                                    \
+                set_error("fft_axis_pass: %zu bytes of LDS per workgroup refused", lds);                    \
+                return CMTIP_ERR_LAUNCH;                                                                    \
+            }                                                                                               \
+            attr_bytes_ = lds;                                                                              \
         }                                                                                                   \
         hipLaunchKernelGGL((axis_pass_kernel<IR, OM, MD>), dim3(grid), dim3(kThreads), lds, stream, a);    \
     } while (0)
