@@ -185,7 +185,8 @@ def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_
 
     With ``scaled`` the coordinates are already x = H / reciprocal_extent * pi.  With
     ``autotune`` the two spreads pick their bin and launch settings with the NUFFT
-    autotuner (first call per shape; later calls reuse the choice)."""
+    autotuner (first call per shape; later calls reuse the choice).  ``weights`` None
+    means unit weights."""
     if not use_recip_sym:
         raise NotImplementedError("the GPU solver keeps real vectors (use_recip_sym=True)")
     if scaled:
@@ -204,7 +205,9 @@ def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_
                           autotune=autotune)
     solver.set_kernel(ugrid_conv)
     # rhs: d = alambda * A^H (data * weights) + rlambda * x0 (:162-171)
-    nuvect_Db = _arrays.to_dev(data, torch.float32).reshape(-1) * _arrays.to_dev(weights, torch.float32).reshape(-1)
+    nuvect_Db = _arrays.to_dev(data, torch.float32).reshape(-1)
+    if weights is not None:  # None: unit weights, the product is the data itself
+        nuvect_Db = nuvect_Db * _arrays.to_dev(weights, torch.float32).reshape(-1)
     uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps, autotune=autotune).reshape(-1)
     if bool(torch.isnan(uvect_ADb).any()):
         print("Warning: nans in the adjoint calculation; intensities may be too large", flush=True)
@@ -255,7 +258,7 @@ def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, 
         x0 = _arrays.to_dev(ac_estimate, torch.float32).reshape(-1).clone()
         est_dtype = ac_estimate.dtype
         out_dtype = np.float32 if est_dtype in (np.float32, torch.float32) else np.float64
-    weights = torch.ones(N, dtype=torch.float32, device=dev)
+    weights = None  # the reference's np.ones(N) (:221): multiplying by it is the identity
     if rlambda is None:
         rlambda = Mtot / N / 1000
     W, d = setup_linops(H_, K_, L_, data, ac_support, weights, x0, M, Mtot, N, ext, alambda, rlambda, flambda,

@@ -613,17 +613,45 @@ __device__ __forceinline__ int spread_scale_exp(float cmax) {
     return 62 - 13 - e;  // kChunk = 2^12 points, one bit of margin
 }
 
-// out_bits[0] = bits of max |c| (non-negative floats order like their bits); out_bits[1]
-// is set to 1 when some c < 0 (the caller clears both)
+// out_bits[0] = bits of max |c| (non-negative floats order like their bits; NaNs are
+// ignored, as fmaxf does); out_bits[1] is set to 1 when some c < 0 (the caller clears
+// both).  16-byte aligned input is read as float4, four loads in flight per thread.
 __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsigned* __restrict__ out_bits) {
     float m = 0.0f;
     bool neg = false;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
-        const float v = c[i];
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * kBlock;
+    auto take = [&](float v) {
         m = fmaxf(m, fabsf(v));
         neg = neg || v < 0.0f;
+    };
+    int64_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(c) & 15) == 0) {
+        const float4* c4 = reinterpret_cast<const float4*>(c);
+        const int64_t n4 = count >> 2;
+        int64_t i = tid;
+        for (; i + 3 * nth < n4; i += 4 * nth) {
+            float4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = c4[i + k * nth];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                take(v[k].x);
+                take(v[k].y);
+                take(v[k].z);
+                take(v[k].w);
+            }
+        }
+        for (; i < n4; i += nth) {
+            const float4 v = c4[i];
+            take(v.x);
+            take(v.y);
+            take(v.z);
+            take(v.w);
+        }
+        done = n4 << 2;
     }
+    for (int64_t i = done + tid; i < count; i += nth) take(c[i]);
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
     const bool any_neg = __ballot(neg) != 0;
     if ((threadIdx.x & 63) == 0) {
