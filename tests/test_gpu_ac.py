@@ -134,6 +134,55 @@ def _solver_system(M=11):
     return W, d, rl
 
 
+def _random_w(M, L, seed, positive=False):
+    """A solver with a random real spectrum (not symmetric), a random antisupport mask
+    and a random support, and the oracle's W = alpha conv + rho I + phi fourier_reg
+    (autocorrelation.py:150-155) for the same terms."""
+    rng = np.random.default_rng(seed)
+    F = rng.random((L,) * 3) if positive else rng.standard_normal((L,) * 3)
+    anti = rng.random((M,) * 3) > 0.5
+    S = (rng.random((M,) * 3) > 0.3).astype(np.float32)
+    solver = A.ACSolver(M, L)
+    solver.set_support(torch.as_tensor(S))
+    solver.set_spectrum(F.astype(np.float32))
+    solver.set_antisupport(anti.astype(np.float32))
+    F32 = F.astype(np.float32).astype(np.float64)
+
+    def W(u, alpha, rho, phi):
+        u = np.asarray(u, dtype=np.float64)
+        return (alpha * OAC.core_problem_convolution(u, M, F32, L, S) + rho * u
+                + phi * OAC.fourier_reg(u, S, anti, M))
+    return solver, W
+
+
+@pytest.mark.parametrize("M,L", [(11, 22), (9, 18), (10, 30), (11, 24), (12, 26)])
+def test_w_operator_half_spectrum_vs_oracle(M, L):
+    # the CG operator runs on half the spectrum with a symmetrised multiplier; L a
+    # multiple of M (fold factor 2 and 3) folds fourier_reg into it, other L (24 and 26
+    # here) run fourier_reg on the M grid after it
+    solver, W = _random_w(M, L, seed=M * 100 + L)
+    u = np.random.default_rng(L).standard_normal(M ** 3).astype(np.float32)
+    say(f"W matvec M={M} L={L}")
+    for alpha, rho, phi in ((0.7, 1e-2, 3.0), (1.0, 0.0, 0.0), (0.0, 0.5, 1e3)):
+        got = solver.matvec(torch.as_tensor(u, device=dev()), alpha, rho, phi).cpu().numpy()
+        assert rel_l2(got, W(u, alpha, rho, phi)) < 1e-5
+
+
+@pytest.mark.parametrize("M,L", [(10, 20), (10, 30), (10, 24)])
+def test_cg_solves_w_system_for_each_embedding(M, L):
+    # the device CG (fused direction update, p.q partials in the last pass, folded or
+    # separate fourier_reg) converges to the solution of the oracle's system
+    solver, W = _random_w(M, L, seed=7 * L, positive=True)
+    d = np.random.default_rng(5).standard_normal(M ** 3).astype(np.float32)
+    x = torch.zeros(M ** 3, device=dev())
+    say(f"cg M={M} L={L}")
+    alpha, rho, phi = 1.0, 0.05, 2.0
+    it, info = solver.cg(torch.as_tensor(d, device=dev()), x, alpha, rho, phi, 300, 1e-6)
+    assert info == 0 and 0 < it < 300
+    res = np.linalg.norm(W(x.cpu().numpy(), alpha, rho, phi) - d) / np.linalg.norm(d)
+    assert res < 1e-5
+
+
 def test_cg_graph_replay_matches_direct_launches(monkeypatch):
     W, d, rl = _solver_system()
     out = {}
