@@ -905,9 +905,10 @@ __global__ void __launch_bounds__(kSpreadBlock) interp_binned_kernel(
     }
 }
 
-// Slice moments in a fixed order: every tile (or wave) leaves its (sum, sum of squares)
-// in its own slot, stage 1 sums contiguous slot ranges per block with a fixed tree,
-// stage 2 adds the block sums in order, so the moments do not depend on scheduling.
+// Slice moments in a fixed order: every tile (or row of 16 lanes of a tile, or wave)
+// leaves its (sum, sum of squares) in its own slot, stage 1 sums contiguous slot ranges
+// per block with a fixed tree, stage 2 adds the block sums in order, so the moments do
+// not depend on scheduling.
 __global__ void __launch_bounds__(kBlock) moments_stage1_kernel(const double* __restrict__ part, int64_t nslots,
                                                                 double* __restrict__ stage) {
     __shared__ double red[2][kBlock];
@@ -1438,6 +1439,25 @@ __global__ void tile_scatter_kernel(int64_t ntiles, const int* __restrict__ tile
     tperm[offset[b] + atomicAdd(cursor + b, 1)] = (int)t;
 }
 
+// Sum of v over each row of 16 lanes, left in the row's last lane (15, 31, 47, 63):
+// an inclusive scan by DPP row shifts of 1, 2, 4 and 8 applied to both 32-bit halves
+// (lanes shifted in from outside the row read zero).  VALU only, no LDS traffic; the
+// whole wave must be active.
+template <int SH>
+__device__ __forceinline__ double row_shr_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 | SH, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 | SH, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double row16_sum(double v) {
+    v += row_shr_f64<1>(v);
+    v += row_shr_f64<2>(v);
+    v += row_shr_f64<4>(v);
+    v += row_shr_f64<8>(v);
+    return v;
+}
+
 template <int W>
 __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
     const float* __restrict__ g, int n, const EsKernel esk, float hi, float lo, TileRot rot, int nbx,
@@ -1532,17 +1552,15 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
                 if (out_bf16) out_bf16[j] = f32_to_bf16(acc);
             }
             if (part) {
-                // this tile's moments in its own slot (a fixed-order reduction follows)
+                // this tile's moments, one slot per row of 16 lanes (the tile order inside
+                // a bin varies from run to run, the slot of a tile does not); the row sums
+                // use DPP lane shifts, not LDS-routed shuffles
                 const double v = active ? (double)acc : 0.0;
-                double m1 = v, m2 = v * v;
-                for (int off = 32; off > 0; off >>= 1) {
-                    m1 += __shfl_down(m1, off, 64);
-                    m2 += __shfl_down(m2, off, 64);
-                }
-                if (lane == 0) {
-                    const int t = tperm[k];
-                    part[2 * (int64_t)t] = m1;
-                    part[2 * (int64_t)t + 1] = m2;
+                const double m1 = row16_sum(v), m2 = row16_sum(v * v);
+                if ((lane & 15) == 15) {
+                    const int64_t slot = 4 * (int64_t)tperm[k] + (lane >> 4);
+                    part[2 * slot] = m1;
+                    part[2 * slot + 1] = m2;
                 }
             }
         }
@@ -2527,12 +2545,14 @@ extern "C" int cmtip_slice_real(cmtip_nufft_plan p, const float* quats, int64_t 
         route = auto_route(p, P, n_orient);
     }
     p->route_used = route;
-    // moment partials: one slot per tile of 64 pixels (fused and tile routes) or per wave
-    // of the finishing pass (binned points); every slot is written by the launch
+    // moment partials: one slot per tile of 64 pixels (fused route), per row of 16 lanes
+    // of each tile (tile route) or per wave of the finishing pass (binned points); every
+    // slot is written by the launch
     double* part = nullptr;
     const int64_t total = n_orient * P;
     const int64_t finish_grid = std::min<int64_t>(4096, (total + kBlock - 1) / kBlock);
-    const int64_t nslots = route == 2 ? finish_grid * (kBlock / 64) : ((P + 63) / 64) * n_orient;
+    const int64_t ntiles = ((P + 63) / 64) * n_orient;
+    const int64_t nslots = route == 2 ? finish_grid * (kBlock / 64) : route == 3 ? 4 * ntiles : ntiles;
     if (moments) {
         CMTIP_TRY(p->part.ensure(sizeof(double) * 2 * (size_t)(nslots + kMomentBlocks)));
         part = p->part.as<double>();
