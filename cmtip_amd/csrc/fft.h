@@ -24,6 +24,7 @@ struct Fft1D {
     int twoff[kFftMaxStages];  // twiddle table offset of each stage
     int omoff[kFftMaxStages];  // R-th roots of unity table offset (generic radices only)
     DevBuf tw;                 // float2 twiddles for the e^{-i} direction
+    int ntw = 0;               // entries of tw
 };
 
 int fft1d_init(Fft1D& p, int n);
