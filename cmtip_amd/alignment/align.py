@@ -371,18 +371,18 @@ def match_orientations(generation, pixel_position_reciprocal, reciprocal_extent,
     precision "fp32": cross term in FP32 on MFMA, argmin fused.  "bf16": cross term in
     BF16 on MFMA with FP32 accumulation, the ``topk`` best per image rescored in FP32;
     ``cross`` "lib" takes the cross term from the hipBLASLt GEMM then reduces it to the
-    top-k, "fused" runs this library's MFMA kernel with the top-k in its epilogue.
+    top-k, "fused" runs this library's MFMA kernel with the top-k in its epilogue (both
+    keep up to 16 candidates; the library GEMM is the faster of the two at these shapes).
     masks (n_images, P), 1 = valid pixel: distances over each image's valid pixels only
     (the data std is then taken over valid pixels); not in the reference.  topk defaults
-    to 8, and to 16 for masked images with the library cross term: masking the bright
+    to 8, and to 16 for masked images in BF16: masking the bright
     centre shrinks the gaps between the best references about 100 x, so more of them
     fall inside the BF16 error (at 10000 x 10000 masked images the 8 best missed the
-    true best for 1 image in 10^4, the 16 best for none).  The fused kernel keeps at
-    most 8.
+    true best for 1 image in 10^4, the 16 best for none).
     """
     assert precision in ("fp32", "bf16")
     if topk is None:
-        topk = 16 if (masks is not None and cross == "lib" and precision == "bf16") else 8
+        topk = 16 if (masks is not None and precision == "bf16") else 8
     n_ref_orientations = _round_up_refs(n_ref_orientations, nbatches)
     ref = _ref_set(n_ref_orientations, ref_orientations, true_orientations, seed)
     as_torch = _arrays.is_torch(slices_, ac)

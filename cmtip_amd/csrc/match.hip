@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <utility>
 
 #include "common.h"
 
@@ -57,15 +58,17 @@ __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsig
 // down to a multiple of 8; the remainder keeps v = b), so the workgroups resident on one
 // XCD at a time hold consecutive positions; the grouped order walks kGroupM tile rows
 // before the next tile column, so those positions form a compact kGroupM-row block of
-// tiles whose A and B panels the XCD's L2 shares.
+// tiles whose A and B panels the XCD's L2 shares.  (The BF16 direct-to-LDS kernel walks
+// 4-row groups: with its 256 x 256 tiles, 4 x 8 blocks measured 3 % faster than 8 x 4.)
 constexpr int kGroupM = 8;
+template <int G = kGroupM>
 __device__ __forceinline__ void tile_of(int b, int tiles_m, int tiles, int* tm, int* tn) {
     const int n8 = tiles & ~7;
     const int v = b < n8 ? (b & 7) * (n8 >> 3) + (b >> 3) : b;
-    const int per_group = kGroupM * (tiles / tiles_m);  // tiles_n columns of kGroupM rows
+    const int per_group = G * (tiles / tiles_m);  // tiles_n columns of G rows
     const int g = v / per_group;
-    const int first = g * kGroupM;
-    const int gm = min(tiles_m - first, kGroupM);
+    const int first = g * G;
+    const int gm = min(tiles_m - first, G);
     const int r = v - g * per_group;
     *tm = first + r % gm;
     *tn = r / gm;
@@ -555,7 +558,7 @@ __global__ void __launch_bounds__(kThreads, BK == 16 ? 3 : 2)
     __shared__ __align__(16) float As[2][BM * SL::LD];
     __shared__ __align__(16) float Bs[2][BN * SL::LD];
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
+    tile_of<>(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -677,65 +680,18 @@ __global__ void __launch_bounds__(kThreads, BK == 16 ? 3 : 2)
 
 // ---------------------------------------------------------------- BF16 MFMA top-k
 //
-// Same tiling with v_mfma_f32_16x16x32_bf16 (gfx950): A[i][k] in lane i + 16 (k / 8),
-// element k % 8 of an 8-wide bf16 vector; B likewise; D as above.  K slabs of 64.
-// The epilogue writes the tile's d2 to LDS transposed, two threads per image scan
-// 64 references each keeping their 8 smallest keys, and insert up to k of them into
-// the image's global list with an atomicMin cascade: atomicMin(slot s, v) leaves the
-// smaller value in the slot and the larger one moves on to slot s + 1, so slot s ends
-// as the (s+1)-th smallest key ever inserted, whatever the interleaving.
+// The cross term on v_mfma_f32_16x16x32_bf16 (gfx950): A[i][k] in lane i + 16 (k / 8),
+// element k % 8 of an 8-wide bf16 vector; B likewise; D as above.  The epilogue keeps
+// the tile's smallest keys per image and inserts them into the image's global list with
+// an atomicMin cascade: atomicMin(slot s, v) leaves the smaller value in the slot and the
+// larger one moves on to slot s + 1, so slot s ends as the (s+1)-th smallest key ever
+// inserted, whatever the interleaving.
 
 constexpr int BK16 = 64;
 constexpr int LD16 = BK16 + 8;  // padded row stride (bf16), keeps 16-byte alignment
 
-__device__ __forceinline__ void load_slab_bf16(const uint16_t* __restrict__ X, int64_t rows, int64_t P,
-                                               int64_t row0, int64_t k0, uint4 (&reg)[4]) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t r = row0 + t / 8 + 32 * i;
-        const int64_t k = k0 + (t % 8) * 8;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (r < rows) {
-            const uint16_t* src = X + r * P + k;
-            if (k + 7 < P && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
-                v = *reinterpret_cast<const uint4*>(src);
-            } else {
-                uint16_t e[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) e[q] = (k + q < P) ? src[q] : (uint16_t)0;
-                v.x = (uint32_t)e[0] | ((uint32_t)e[1] << 16);
-                v.y = (uint32_t)e[2] | ((uint32_t)e[3] << 16);
-                v.z = (uint32_t)e[4] | ((uint32_t)e[5] << 16);
-                v.w = (uint32_t)e[6] | ((uint32_t)e[7] << 16);
-            }
-        }
-        reg[i] = v;
-    }
-}
-
-// Fast form for full slabs of 16-byte aligned rows (P % 8 == 0, aligned base), rows past
-// the end clamped to the last row (the epilogue skips their products)
-__device__ __forceinline__ void load_slab_bf16_full(const uint16_t* __restrict__ X, int64_t rows, int64_t P,
-                                                    int64_t row0, int64_t k0, uint4 (&reg)[4]) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        int64_t r = row0 + t / 8 + 32 * i;
-        r = r < rows ? r : rows - 1;
-        reg[i] = *reinterpret_cast<const uint4*>(X + r * P + k0 + (t % 8) * 8);
-    }
-}
-
-__device__ __forceinline__ void store_slab_bf16(uint16_t* __restrict__ S, const uint4 (&reg)[4]) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        *reinterpret_cast<uint4*>(S + (t / 8 + 32 * i) * LD16 + (t % 8) * 8) = reg[i];
-}
-
-constexpr int kTopK = 8;       // candidates per image of the fused kernel
-constexpr int kTopKMax = 16;   // of the library-GEMM top-k pass and the rescoring
+constexpr int kTopK = 8;       // candidate lists of up to 8 keep 8 per image (registers)
+constexpr int kTopKMax = 16;   // the most candidates per image (top-k kernels and rescoring)
 
 template <int L>
 __device__ __forceinline__ void topk_insert(unsigned long long (&lst)[L], unsigned long long v) {
@@ -756,107 +712,422 @@ __device__ __forceinline__ void global_insert(unsigned long long* slots, int k, 
     }
 }
 
-constexpr size_t kBf16Lds = 2 * (size_t)(BM + BN) * LD16 * sizeof(uint16_t);
-static_assert(kBf16Lds >= (size_t)BM * BN * sizeof(float), "epilogue reuses the operand LDS");
+// ---------------------------------------------------------------- BF16 MFMA top-k, register-staged slabs
+//
+// The general form (any P, any alignment; the direct-to-LDS kernel below takes aligned
+// rows with P a multiple of 64).  A workgroup tile of BMt references x BNt images streams
+// (BMt + BNt) rows of K through LDS for 2 BMt BNt K flop, so the tile size sets the
+// L2 -> CU traffic the MFMA rate demands: a 128 x 128 tile at the BF16 peak (~4070 flop
+// per clock per CU) needs 64 B per clock per CU, 256 x 256 half of that.  WM x WN waves
+// (two per SIMD) each own TM x TN 16 x 16 MFMA tiles; slabs of 64 columns, A rows then B
+// rows in one LDS stage (pitch 72 bf16: 16-byte reads of 8 consecutive rows cover all 32
+// banks), double buffered, the next slab's global loads held in registers while the
+// current one is multiplied.
+//
+// Epilogue (both kernels): the tile's d2 go to LDS in groups of kThreads / 4 images
+// (pitch BMt + 4 floats), four threads per image each scan a quarter of the references
+// (interleaved, so the 32 lanes of a half-wave read 32 distinct banks) keeping their L
+// smallest keys, and insert them into the image's global list with global_insert.
+template <int WM, int WN, int TM, int TN>
+struct XTile {
+    static constexpr int kWaves = WM * WN;
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int BMt = 16 * WM * TM;                     // references per tile
+    static constexpr int BNt = 16 * WN * TN;                     // images per tile
+    static constexpr int kRows = BMt + BNt;                      // staged rows per slab
+    static constexpr int kChunksA = BMt * (BK16 / 8);            // 16-byte chunks of A per slab
+    static constexpr int kChunks = kRows * (BK16 / 8);
+    static constexpr int kPerThread = (kChunks + kThreads - 1) / kThreads;
+    static constexpr int kStage = kRows * LD16;                  // bf16 per LDS stage
+    static constexpr int kGroup = kThreads / 4;                  // images per epilogue pass
+    static constexpr int kDPitch = BMt + 4;                      // floats per image row of d2
+    static constexpr size_t kStageBytes = 2 * (size_t)kStage * sizeof(uint16_t);
+    static constexpr size_t kDBytes = (size_t)kGroup * kDPitch * sizeof(float);
+    static constexpr size_t kLds = kStageBytes > kDBytes ? kStageBytes : kDBytes;
+    static_assert(kChunksA % kThreads == 0, "A chunks of a slab split evenly over the threads");
+    static_assert(kLds <= 160 * 1024, "LDS of one workgroup");
+};
 
-template <bool ALIGNED>
-__global__ void __launch_bounds__(kThreads, 2)
-    match_topk_bf16_kernel(const uint16_t* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
+// 16-byte chunk u of this thread for slab k0: rows of A first, then rows of B; rows past
+// the end are clamped to the last row (the epilogue skips their products).  FULL: the
+// whole slab lies inside P and rows are 16-byte aligned; otherwise columns past P read 0.
+template <class T, bool FULL>
+__device__ __forceinline__ void xtile_load(const uint16_t* __restrict__ A, int64_t n_ref,
+                                           const uint16_t* __restrict__ B, int64_t n_img, int64_t P, int64_t m0,
+                                           int64_t n0, int64_t k0, uint4 (&reg)[T::kPerThread]) {
+#pragma unroll
+    for (int u = 0; u < T::kPerThread; ++u) {
+        const int c = threadIdx.x + u * T::kThreads;
+        if (T::kChunks % T::kThreads != 0 && u == T::kPerThread - 1 && c >= T::kChunks) {
+            reg[u] = make_uint4(0u, 0u, 0u, 0u);
+            continue;
+        }
+        const int row = c >> 3, kc = c & 7;
+        const bool isA = u * T::kThreads < T::kChunksA;  // known at compile time
+        const uint16_t* X = isA ? A : B;
+        const int64_t rows = isA ? n_ref : n_img;
+        int64_t r = isA ? m0 + row : n0 + (row - T::BMt);
+        r = r < rows ? r : rows - 1;
+        const int64_t k = k0 + kc * 8;
+        const uint16_t* src = X + r * P + k;
+        if (FULL) {
+            reg[u] = *reinterpret_cast<const uint4*>(src);
+        } else if (k + 7 < P && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+            reg[u] = *reinterpret_cast<const uint4*>(src);
+        } else {
+            uint16_t e[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) e[q] = (k + q < P) ? src[q] : (uint16_t)0;
+            reg[u] = make_uint4((uint32_t)e[0] | ((uint32_t)e[1] << 16), (uint32_t)e[2] | ((uint32_t)e[3] << 16),
+                                (uint32_t)e[4] | ((uint32_t)e[5] << 16), (uint32_t)e[6] | ((uint32_t)e[7] << 16));
+        }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void xtile_store(uint16_t* __restrict__ S, const uint4 (&reg)[T::kPerThread]) {
+#pragma unroll
+    for (int u = 0; u < T::kPerThread; ++u) {
+        const int c = threadIdx.x + u * T::kThreads;
+        if (T::kChunks % T::kThreads != 0 && u == T::kPerThread - 1 && c >= T::kChunks) continue;
+        *reinterpret_cast<uint4*>(S + (c >> 3) * LD16 + (c & 7) * 8) = reg[u];
+    }
+}
+
+template <int WM, int WN, int TM, int TN, int L, bool ALIGNED>
+__global__ void __launch_bounds__(64 * WM * WN, 1)
+    cross_topk_bf16_kernel(const uint16_t* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
                            const uint16_t* __restrict__ Bw, int64_t n_img, int64_t P,
                            const float* __restrict__ an, const float* __restrict__ bn,
                            const double* __restrict__ gamma, int k, unsigned long long* __restrict__ cand,
                            int tiles_m) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kBf16Lds];
-    uint16_t* As0 = reinterpret_cast<uint16_t*>(lds_raw);
-    uint16_t* Bs0 = As0 + 2 * BM * LD16;
+    using T = XTile<WM, WN, TM, TN>;
+    extern __shared__ __attribute__((aligned(16))) unsigned char xlds[];
+    uint16_t* S0 = reinterpret_cast<uint16_t*>(xlds);
     int tm, tn;
-    tile_of(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
-    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    tile_of<>(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
+    const int64_t m0 = (int64_t)tm * T::BMt, n0 = (int64_t)tn * T::BNt;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave % WM, wn = wave / WM;
     const int li = lane & 15, lk = lane >> 4;
 
-    floatx4 acc[4][4];
+    floatx4 acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < TN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    uint4 ra[4], rb[4];
     const int nk = (int)((P + BK16 - 1) / BK16);
     const int nfull = ALIGNED ? (int)(P / BK16) : 0;
-    auto load = [&](int kt) {
-        if (kt < nfull) {
-            load_slab_bf16_full(A, n_ref, P, m0, (int64_t)kt * BK16, ra);
-            load_slab_bf16_full(Bw, n_img, P, n0, (int64_t)kt * BK16, rb);
-        } else {
-            load_slab_bf16(A, n_ref, P, m0, (int64_t)kt * BK16, ra);
-            load_slab_bf16(Bw, n_img, P, n0, (int64_t)kt * BK16, rb);
-        }
+    uint4 ra[T::kPerThread];
+    auto load = [&](int kt, uint4 (&reg)[T::kPerThread]) {
+        if (kt < nfull)
+            xtile_load<T, true>(A, n_ref, Bw, n_img, P, m0, n0, (int64_t)kt * BK16, reg);
+        else
+            xtile_load<T, false>(A, n_ref, Bw, n_img, P, m0, n0, (int64_t)kt * BK16, reg);
     };
-    load(0);
-    for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        uint16_t* As = As0 + buf * BM * LD16;
-        uint16_t* Bs = Bs0 + buf * BN * LD16;
-        store_slab_bf16(As, ra);
-        store_slab_bf16(Bs, rb);
-        __syncthreads();
-        if (kt + 1 < nk) load(kt + 1);
-        const uint16_t* as = As + (wm * 64 + li) * LD16 + lk * 8;
-        const uint16_t* bs = Bs + (wn * 64 + li) * LD16 + lk * 8;
+    auto compute = [&](int kt) {
+        const uint16_t* S = S0 + (kt & 1) * T::kStage;
+        const uint16_t* as = S + (wm * TM * 16 + li) * LD16 + lk * 8;
+        const uint16_t* bs = S + (T::BMt + wn * TN * 16 + li) * LD16 + lk * 8;
 #pragma unroll
         for (int kk = 0; kk < BK16 / 32; ++kk) {
-            bf16x8 af[4], bf[4];
+            bf16x8 af[TM], bf[TN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8*>(as + i * 16 * LD16 + kk * 32);
+            for (int i = 0; i < TM; ++i) af[i] = *reinterpret_cast<const bf16x8*>(as + i * 16 * LD16 + kk * 32);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = *reinterpret_cast<const bf16x8*>(bs + j * 16 * LD16 + kk * 32);
+            for (int j = 0; j < TN; ++j) bf[j] = *reinterpret_cast<const bf16x8*>(bs + j * 16 * LD16 + kk * 32);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
+    };
+    // slab kt + 1 is loaded into registers while slab kt is multiplied
+    load(0, ra);
+    xtile_store<T>(S0, ra);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load(kt + 1, ra);
+        compute(kt);
+        if (kt + 1 < nk) xtile_store<T>(S0 + ((kt + 1) & 1) * T::kStage, ra);
+        __syncthreads();
     }
-    __syncthreads();  // operand LDS is reused below
 
-    // d2 tile, transposed: D[col][row]
-    float* D = reinterpret_cast<float*>(lds_raw);
+    // d2 = gamma^2 an[r] - 2 gamma cross + bn[c] into LDS, image-major, 128 images a pass
+    float* D = reinterpret_cast<float*>(xlds);
     const float g = gamma ? (float)gamma[0] : 1.0f;
     const float g2 = g * g, m2g = -2.0f * g;
+    float anr[TM][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int cl = wn * 64 + j * 16 + li;
-        const int64_t c = n0 + cl;
-        const float bnc = c < n_img ? bn[c] : 0.f;
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int v = 0; v < 4; ++v) {
+            const int64_t r = m0 + wm * TM * 16 + i * 16 + lk * 4 + v;
+            anr[i][v] = r < n_ref ? an[r] : 0.f;
+        }
+    for (int g0 = 0; g0 < T::BNt; g0 += T::kGroup) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int rl = wm * 64 + i * 16 + lk * 4 + v;
-                const int64_t r = m0 + rl;
-                const float anr = r < n_ref ? an[r] : 0.f;
-                D[cl * BM + rl] = fmaf(g2, anr, fmaf(m2g, acc[i][j][v], bnc));
+        for (int j = 0; j < TN; ++j) {
+            const int cl = wn * TN * 16 + j * 16 + li;
+            const int q = cl - g0;
+            if (q < 0 || q >= T::kGroup) continue;
+            const int64_t c = n0 + cl;
+            const float bnc = c < n_img ? bn[c] : 0.f;
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    D[q * T::kDPitch + wm * TM * 16 + i * 16 + lk * 4 + v] =
+                        fmaf(g2, anr[i][v], fmaf(m2g, acc[i][j][v], bnc));
+        }
+        __syncthreads();
+        const int q = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const int64_t c = n0 + g0 + q;
+        if (g0 + q < T::BNt && c < n_img) {
+            unsigned long long lst[L];
+#pragma unroll
+            for (int s = 0; s < L; ++s) lst[s] = kEmpty;
+            const float* drow = D + q * T::kDPitch;
+            const int64_t rmax = n_ref - m0;  // references of this tile: rl < rmax
+            for (int e = 0; e < T::BMt / 4; ++e) {
+                const int rl = 4 * e + part;
+                if (rl < rmax) topk_insert(lst, pack_key(drow[rl], (uint32_t)(m0 + rl) + ref_offset));
             }
+            unsigned long long* slots = cand + c * k;
+            for (int s = 0; s < k; ++s) {
+                if (lst[s] == kEmpty || lst[s] >= slots[k - 1]) break;
+                global_insert(slots, k, lst[s]);
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    const int cl = threadIdx.x >> 1, half = threadIdx.x & 1;
-    const int64_t c = n0 + cl;
-    if (c < n_img) {
-        unsigned long long lst[kTopK];
+}
+
+// ---------------------------------------------------------------- BF16 top-k, direct-to-LDS pipeline
+//
+// The default BF16 kernel for aligned rows with P a multiple of 64: 256 references x 256
+// images per workgroup, four waves (one per SIMD) each owning 128 x 128 (8 x 8 MFMA
+// tiles, 256 accumulators in AGPRs), which halves the LDS operand reads per flop of a
+// 128 x 64 wave tile (16 16-byte reads per 64 MFMAs).  Operand slabs of 32 columns are
+// fetched by global_load_lds_dwordx4, which writes each lane's 16 bytes straight to LDS
+// (a wave instruction fills 1 KB: 16 rows of 64 bytes), so no registers stage them; S = 4
+// stages of 32 KB.  Step kt: wait until this wave's loads of slab kt + 1 have landed and
+// its LDS reads of slab kt are done, barrier (then every wave's have), request slab
+// kt + S into the stage slab kt used, read slab kt + 1's operands into the second
+// register set interleaved with slab kt's 64 MFMAs from the first (one read per four
+// MFMAs, fixed with sched_group_barrier).  Three slabs (96 KB per CU) stay in flight.
+// Rows are 64 bytes (four 16-byte chunks); chunk c of row r is stored at position
+// c ^ f(r), f(r) = ((r >> 1) + (r >> 3)) & 3, so the 16-byte operand reads of 8 (or 16)
+// consecutive rows at one chunk hit distinct banks.
+//
+// Measured at 10000 x 5000 x 16384 (and 51200 x 20480 x 16384): the loop is bound by
+// the global -> LDS traffic, 16.8 MB per tile: the copies alone take as long as the
+// whole kernel, and they run at about 9 TB/s with the tile order above; with every tile
+// reading the same L2-resident rows the same loop reaches 1.7 PFLOP/s.  The XCD-aware
+// order is worth 1.7x at the larger shape (plain order 648 TFLOP/s).
+constexpr int BK32 = 32;
+
+template <int WM, int WN, int TM, int TN, int S>
+struct DTile {
+    static constexpr int kWaves = WM * WN;
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int BMt = 16 * WM * TM;
+    static constexpr int BNt = 16 * WN * TN;
+    static constexpr int kRows = BMt + BNt;
+    static constexpr int kBlocks = kRows / 16;                   // 1 KB wave loads per slab
+    static constexpr int kMinPerWave = kBlocks / kWaves;         // loads per slab of every wave (at least)
+    static constexpr int kStageBytes = kRows * BK32 * 2;
+    static constexpr int kGroup = kThreads / 4;
+    static constexpr int kDPitch = BMt + 4;
+    static constexpr size_t kDBytes = (size_t)kGroup * kDPitch * sizeof(float);
+    static constexpr size_t kLds = (size_t)S * kStageBytes > kDBytes ? (size_t)S * kStageBytes : kDBytes;
+    static_assert(kRows % 16 == 0 && kMinPerWave >= 1, "whole 16-row blocks, one per wave at least");
+    static_assert(S >= 3 && kMinPerWave * (S - 2) < 64, "vmcnt range");
+    static_assert(kLds <= 160 * 1024, "LDS of one workgroup");
+};
+
+__device__ __forceinline__ int dswz(int r) { return ((r >> 1) + (r >> 3)) & 3; }
+
+// Scheduling order of one pipeline step: R - 1 rounds of (Q MFMAs, one LDS read), the last
+// read, then the remaining MFMAs, so the next slab's operand reads spread over this
+// slab's MFMAs and the last one has MFMAs behind it before the step's barrier.
+template <int Q, int... I>
+__device__ __forceinline__ void sched_rounds(std::integer_sequence<int, I...>) {
+    ((__builtin_amdgcn_sched_group_barrier(0x008, Q, 0), __builtin_amdgcn_sched_group_barrier(0x100, 1, 0),
+      (void)I),
+     ...);
+}
+
+template <int R, int Q, int REST>
+__device__ __forceinline__ void sched_reads_mfma() {
+    sched_rounds<Q>(std::make_integer_sequence<int, R - 1>{});
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // the last read ahead of the last MFMAs
+    __builtin_amdgcn_sched_group_barrier(0x008, Q + REST, 0);
+}
+
+// s_waitcnt with vmcnt <= N (expcnt, lgkmcnt left alone)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int WM, int WN, int TM, int TN, int S, int L>
+__global__ void __launch_bounds__(64 * WM * WN, 1)
+    cross_topk_bf16_dma_kernel(const uint16_t* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
+                               const uint16_t* __restrict__ Bw, int64_t n_img, int64_t P,
+                               const float* __restrict__ an, const float* __restrict__ bn,
+                               const double* __restrict__ gamma, int k, unsigned long long* __restrict__ cand,
+                               int tiles_m) {
+    using T = DTile<WM, WN, TM, TN, S>;
+    extern __shared__ __attribute__((aligned(16))) unsigned char xlds[];
+    int tm, tn;
+    tile_of<4>(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
+    const int64_t m0 = (int64_t)tm * T::BMt, n0 = (int64_t)tn * T::BNt;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave % WM, wn = wave / WM;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nk = (int)(P / BK32);
+
+    // this lane's global row pointer and LDS block of each of its wave's loads
+    constexpr int kMaxPerWave = (T::kBlocks + T::kWaves - 1) / T::kWaves;
+    const uint16_t* src[kMaxPerWave];
+    int nblk = 0;
 #pragma unroll
-        for (int s = 0; s < kTopK; ++s) lst[s] = kEmpty;
-        for (int q = 0; q < BM / 2; ++q) {
-            const int rl = half * (BM / 2) + q;
-            const int64_t r = m0 + rl;
-            if (r < n_ref) topk_insert(lst, pack_key(D[cl * BM + rl], (uint32_t)r + ref_offset));
+    for (int u = 0; u < kMaxPerWave; ++u) {
+        const int b = wave + u * T::kWaves;
+        const int r = 16 * b + (lane >> 2);
+        const int c = (lane & 3) ^ dswz(r);
+        const bool isA = r < T::BMt;
+        int64_t gr = isA ? m0 + r : n0 + (r - T::BMt);
+        const int64_t rows = isA ? n_ref : n_img;
+        gr = gr < rows ? gr : rows - 1;
+        src[u] = (isA ? A : Bw) + gr * P + 8 * c;
+        nblk += b < T::kBlocks ? 1 : 0;
+    }
+    auto issue = [&](int kt) {
+        unsigned char* stage = xlds + (kt % S) * T::kStageBytes;
+#pragma unroll
+        for (int u = 0; u < kMaxPerWave; ++u) {
+            if (u < nblk)
+                __builtin_amdgcn_global_load_lds((const void*)(src[u] + (int64_t)kt * BK32),
+                                                 (__attribute__((address_space(3))) void*)(stage +
+                                                     (wave + u * T::kWaves) * 1024),
+                                                 16, 0, 0);
         }
-        unsigned long long* slots = cand + c * k;
-        for (int s = 0; s < k; ++s) {
-            if (lst[s] == kEmpty || lst[s] >= slots[k - 1]) break;
-            global_insert(slots, k, lst[s]);
+    };
+
+    floatx4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // LDS byte offsets of this lane's operand rows within a stage
+    int aoff[TM], boff[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int r = wm * TM * 16 + i * 16 + li;
+        aoff[i] = r * 64 + 16 * (lk ^ dswz(r));
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int r = T::BMt + wn * TN * 16 + j * 16 + li;
+        boff[j] = r * 64 + 16 * (lk ^ dswz(r));
+    }
+    bf16x8 af[2][TM], bfr[2][TN];
+    auto read = [&](int kt, int set) {
+        const unsigned char* stage = xlds + (kt % S) * T::kStageBytes;
+#pragma unroll
+        for (int i = 0; i < TM; ++i) af[set][i] = *reinterpret_cast<const bf16x8*>(stage + aoff[i]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bfr[set][j] = *reinterpret_cast<const bf16x8*>(stage + boff[j]);
+    };
+    auto mma = [&](int set) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[set][i], bfr[set][j], acc[i][j], 0, 0, 0);
+    };
+
+    // prologue: slabs 0 .. S - 1 requested, slab 0's operands read
+    for (int s = 0; s < S && s < nk; ++s) issue(s);
+    wait_vm<0>();
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    read(0, 0);
+    // nk is even: slab kt from set 0, slab kt + 1 from set 1 (static register sets).  The
+    // operand reads are unconditional (past the last slab they re-read a landed stage
+    // and go unused), so no branch makes the compiler wait for them before the MFMAs.
+    for (int kt = 0; kt < nk; kt += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int s = kt + h;
+            // this wave's loads of slab s + 1 landed (slabs s + 2 .. s + S - 1 may still be
+            // in flight) and its reads of slab s done, so after the barrier slab s's stage is
+            // free in every wave
+            if (s + S - 1 < nk)
+                wait_vm<T::kMinPerWave * (S - 2)>();
+            else
+                wait_vm<0>();
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (s + S < nk) issue(s + S);
+            read(s + 1, h ^ 1);
+            mma(h);
+            sched_reads_mfma<TM + TN, (TM * TN) / (TM + TN), TM * TN - (TM + TN) * ((TM * TN) / (TM + TN))>();
         }
+    }
+    wait_vm<0>();
+    __syncthreads();  // the stages are reused below
+
+    float* D = reinterpret_cast<float*>(xlds);
+    const float g = gamma ? (float)gamma[0] : 1.0f;
+    const float g2 = g * g, m2g = -2.0f * g;
+    for (int g0 = 0; g0 < T::BNt; g0 += T::kGroup) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int cl = wn * TN * 16 + j * 16 + li;
+            const int q = cl - g0;
+            if (q < 0 || q >= T::kGroup) continue;
+            const int64_t c = n0 + cl;
+            const float bnc = c < n_img ? bn[c] : 0.f;
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    // an[r] reloaded per image group (L1 hits) to keep registers for the lists
+                    const int rl = wm * TM * 16 + i * 16 + lk * 4 + v;
+                    const float anr = m0 + rl < n_ref ? an[m0 + rl] : 0.f;
+                    D[q * T::kDPitch + rl] = fmaf(g2, anr, fmaf(m2g, acc[i][j][v], bnc));
+                }
+        }
+        __syncthreads();
+        const int q = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const int64_t c = n0 + g0 + q;
+        if (g0 + q < T::BNt && c < n_img) {
+            unsigned long long lst[L];
+#pragma unroll
+            for (int s = 0; s < L; ++s) lst[s] = kEmpty;
+            const float* drow = D + q * T::kDPitch;
+            const int64_t rmax = n_ref - m0;
+            for (int e = 0; e < T::BMt / 4; ++e) {
+                const int rl = 4 * e + part;
+                if (rl < rmax) topk_insert(lst, pack_key(drow[rl], (uint32_t)(m0 + rl) + ref_offset));
+            }
+            unsigned long long* slots = cand + c * k;
+            for (int s = 0; s < k; ++s) {
+                if (lst[s] == kEmpty || lst[s] >= slots[k - 1]) break;
+                global_insert(slots, k, lst[s]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1484,23 +1755,94 @@ extern "C" int cmtip_match_topk_cross(const float* Dt, int64_t n_ref, int64_t re
     return CMTIP_OK;
 }
 
+namespace {
+
+// Launchers: one workgroup per tile (the LDS of one tile fills a CU), dynamic LDS raised
+// to what the kernel needs; L = 8 list entries for k <= 8, else 16.
+template <int WM, int WN, int TM, int TN>
+struct XTileLaunch {
+    using T = XTile<WM, WN, TM, TN>;
+    static int64_t tiles_m(int64_t n_ref) { return (n_ref + T::BMt - 1) / T::BMt; }
+    static int64_t tiles(int64_t n_ref, int64_t n_img) { return tiles_m(n_ref) * ((n_img + T::BNt - 1) / T::BNt); }
+    template <int L, bool AL>
+    static void go(const uint16_t* A, int64_t n_ref, uint32_t off, const uint16_t* Bw, int64_t n_img, int64_t P,
+                   const float* an, const float* bn, const double* gamma, int k, unsigned long long* cand,
+                   hipStream_t st) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void*)cross_topk_bf16_kernel<WM, WN, TM, TN, L, AL>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::kLds);
+            attr = true;
+        }
+        hipLaunchKernelGGL((cross_topk_bf16_kernel<WM, WN, TM, TN, L, AL>), dim3((unsigned)tiles(n_ref, n_img)),
+                           dim3(T::kThreads), T::kLds, st, A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand,
+                           (int)tiles_m(n_ref));
+    }
+    static void run(bool aligned, const uint16_t* A, int64_t n_ref, uint32_t off, const uint16_t* Bw, int64_t n_img,
+                    int64_t P, const float* an, const float* bn, const double* gamma, int k,
+                    unsigned long long* cand, hipStream_t st) {
+        if (k <= kTopK) {
+            if (aligned) go<kTopK, true>(A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand, st);
+            else go<kTopK, false>(A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand, st);
+        } else {
+            if (aligned) go<kTopKMax, true>(A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand, st);
+            else go<kTopKMax, false>(A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand, st);
+        }
+    }
+};
+
+template <int WM, int WN, int TM, int TN, int S>
+struct DTileLaunch {
+    using T = DTile<WM, WN, TM, TN, S>;
+    static int64_t tiles_m(int64_t n_ref) { return (n_ref + T::BMt - 1) / T::BMt; }
+    static int64_t tiles(int64_t n_ref, int64_t n_img) { return tiles_m(n_ref) * ((n_img + T::BNt - 1) / T::BNt); }
+    template <int L>
+    static void go(const uint16_t* A, int64_t n_ref, uint32_t off, const uint16_t* Bw, int64_t n_img, int64_t P,
+                   const float* an, const float* bn, const double* gamma, int k, unsigned long long* cand,
+                   hipStream_t st) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void*)cross_topk_bf16_dma_kernel<WM, WN, TM, TN, S, L>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::kLds);
+            attr = true;
+        }
+        hipLaunchKernelGGL((cross_topk_bf16_dma_kernel<WM, WN, TM, TN, S, L>), dim3((unsigned)tiles(n_ref, n_img)),
+                           dim3(T::kThreads), T::kLds, st, A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand,
+                           (int)tiles_m(n_ref));
+    }
+    static void run(const uint16_t* A, int64_t n_ref, uint32_t off, const uint16_t* Bw, int64_t n_img, int64_t P,
+                    const float* an, const float* bn, const double* gamma, int k, unsigned long long* cand,
+                    hipStream_t st) {
+        if (k <= kTopK) go<kTopK>(A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand, st);
+        else go<kTopKMax>(A, n_ref, off, Bw, n_img, P, an, bn, gamma, k, cand, st);
+    }
+};
+
+// 256 x 256 tiles for both: measured at 10000 x 5000 x 16384, the register-staged kernel
+// 2.25-2.31 ms with 256 x 256, 2.15-2.16 with 256 x 160 or 256 x 192 tiles (8 waves);
+// the direct-to-LDS kernel 1.80 ms (four waves of 128 x 128; eight waves of 128 x 64,
+// 1.76-1.99 ms depending on the stage count, spill at five stages).
+using XTileBF16 = XTileLaunch<2, 4, 8, 4>;
+using DTileBF16 = DTileLaunch<2, 2, 8, 8, 4>;
+
+}  // namespace
+
 extern "C" int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t ref_offset, const uint16_t* Bw,
                                      int64_t n_img, int64_t P, const float* an, const float* bn,
                                      const double* gamma, int k, unsigned long long* cand, void* stream) {
-    if (k < 1 || k > kTopK) {
-        set_error("match_topk_bf16: k must be in [1, %d]", kTopK);
+    if (k < 1 || k > kTopKMax) {
+        set_error("match_topk_bf16: k must be in [1, %d]", kTopKMax);
         return CMTIP_ERR_ARG;
     }
     int64_t tiles_m, tiles;
     CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, cand, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
-    if (P % 8 == 0 && aligned16(A) && aligned16(Bw))
-        hipLaunchKernelGGL(match_topk_bf16_kernel<true>, dim3((unsigned)tiles), dim3(kThreads), 0, as_stream(stream),
-                           A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, (int)tiles_m);
+    const bool aligned = P % 8 == 0 && aligned16(A) && aligned16(Bw);
+    const hipStream_t st = as_stream(stream);
+    if (aligned && P % (2 * BK32) == 0)
+        DTileBF16::run(A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, st);
     else
-        hipLaunchKernelGGL(match_topk_bf16_kernel<false>, dim3((unsigned)tiles), dim3(kThreads), 0,
-                           as_stream(stream), A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand,
-                           (int)tiles_m);
+        XTileBF16::run(aligned, A, n_ref, (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, k, cand, st);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

@@ -255,8 +255,10 @@ int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref_offset, co
 int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int64_t n_b, int64_t P,
                      const float* an, const float* bn, float* D, void* stream);
 
-/* BF16 path: cross term in bfloat16 on MFMA with FP32 accumulation; keeps the k
- * (1 <= k <= 8) smallest approximate distances per image in cand (n_img x k packed
+/* BF16 path: cross term in bfloat16 on MFMA with FP32 accumulation (this library's
+ * kernels: 256 x 256 tiles, direct-to-LDS operand pipeline for aligned rows with K a
+ * multiple of 64, register-staged otherwise); keeps the k (1 <= k <= 16) smallest
+ * approximate distances per image in cand (n_img x k packed
  * keys as above, initialised to all ones bits before the first batch), then
  * cmtip_match_rescore recomputes each candidate's distance in FP32 from the float32
  * slices (d2 = sum_p w_p^2 (gamma a - b)^2 accumulated in FP32) and writes the argmin
@@ -270,7 +272,7 @@ int cmtip_match_topk_bf16(const uint16_t* A, int64_t n_ref, int64_t ref_offset,
  * writes the FP32 cross-term block Dt[i * n_ref + r] = <A_r, B_i> (bfloat16 rows of K
  * elements, FP32 accumulation) with the hipBLASLt GEMM; cmtip_match_topk_cross then
  * merges d2 = gamma^2 an[r] - 2 gamma Dt[i, r] + bn[i] into the image's k candidates
- * (1 <= k <= 16 here and in cmtip_match_rescore[_masked]; the fused kernel keeps <= 8)
+ * (1 <= k <= 16 here, in cmtip_match_topk_bf16 and in cmtip_match_rescore[_masked])
  * (one wave per image, no atomics; ties keep the smallest reference index).  Replaces
  * the same sklearn euclidean_distances + argmin (align.py:35,128-131). */
 int cmtip_match_cross_bf16(const uint16_t* A, int64_t n_ref, const uint16_t* B, int64_t n_img, int64_t K,

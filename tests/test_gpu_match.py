@@ -180,6 +180,53 @@ def test_match_kernels_vs_fp64(n_ref, n_img, P):
     assert same > 0.9
 
 
+@pytest.mark.parametrize("n_ref,n_img,P,k", [(700, 300, 4096, 8), (517, 1111, 2048, 16), (10000, 5000, 16384, 16)])
+def test_bf16_kernel_paths_agree(n_ref, n_img, P, k):
+    # the direct-to-LDS kernel (aligned rows, P a multiple of 64) and the register-staged
+    # kernel (rows one element off 16-byte alignment) accumulate the same MFMA sequence,
+    # so their candidate lists are identical; both keep the same candidates as the library
+    # GEMM + top-k pass apart from near-ties of the approximate distances; reference
+    # offsets carry into the keys
+    from cmtip_amd._lib import check, lib, ptr, stream
+    d = dev()
+    L = lib()
+    g = torch.Generator(device=d).manual_seed(11)
+    A = torch.randn((n_ref, P), device=d, generator=g)
+    B = torch.randn((n_img, P), device=d, generator=g)
+    A16 = A.to(torch.bfloat16).view(torch.int16)
+    B16 = B.to(torch.bfloat16).view(torch.int16)
+    # unaligned copies: rows start one element past a 16-byte boundary
+    A16u = torch.empty(n_ref * P + 1, dtype=torch.int16, device=d)[1:].view(n_ref, P)
+    B16u = torch.empty(n_img * P + 1, dtype=torch.int16, device=d)[1:].view(n_img, P)
+    A16u.copy_(A16)
+    B16u.copy_(B16)
+    an = (A.double() ** 2).sum(1).float()
+    bn = (B.double() ** 2).sum(1).float()
+    off = 3
+
+    def run(a, b):
+        cand = torch.full((n_img, k), -1, dtype=torch.int64, device=d)
+        check(L.cmtip_match_topk_bf16(ptr(a), n_ref, off, ptr(b), n_img, P, ptr(an), ptr(bn), None, k, ptr(cand),
+                                      stream()))
+        return cand.cpu().numpy().view(np.uint64)
+
+    say(f"bf16 kernel paths {n_ref}x{n_img}x{P} k={k}")
+    c_dma, c_reg = run(A16, B16), run(A16u, B16u)
+    np.testing.assert_array_equal(c_dma, c_reg)
+    assert np.all(np.diff(c_dma.astype(np.float64), axis=1) >= 0)
+    idx = (c_dma & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    assert idx.min() >= off and idx.max() < n_ref + off
+    assert all(len(set(r)) == k for r in idx[:200])
+    say("  library GEMM + top-k")
+    cand = torch.full((n_img, k), -1, dtype=torch.int64, device=d)
+    Dt = torch.empty((n_img, n_ref), device=d)
+    check(L.cmtip_match_cross_bf16(ptr(A16), n_ref, ptr(B16), n_img, P, ptr(Dt), stream()))
+    check(L.cmtip_match_topk_cross(ptr(Dt), n_ref, off, n_img, ptr(an), ptr(bn), None, k, ptr(cand), stream()))
+    c_lib = cand.cpu().numpy().view(np.uint64)
+    same = np.mean([len(set(a) & set(b)) / k for a, b in zip(idx, (c_lib & np.uint64(0xFFFFFFFF)).astype(np.int64))])
+    assert same > 0.9
+
+
 @pytest.mark.parametrize("split,n_ref", [(180, 300), (181, 300), (1000, 2601)])
 def test_topk_cross_exact(split, n_ref):
     # keys formed and merged exactly as numpy forms them in float32 (gamma = 1), over two

@@ -57,6 +57,9 @@ def test_bf16_top8_at_full_size():
     del clean
     q, idx = alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref, precision="bf16",
                                          return_index=True)
+    say("  the same with this library's BF16 kernel (cross='fused')")
+    qf, idxf = alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref, precision="bf16",
+                                           return_index=True, cross="fused")
     A = sl.slice(torch.from_numpy(ref).to(d))
     P = side * side
     w2 = torch.full((P,), 1.0 / P ** 2, device=d)
@@ -69,6 +72,8 @@ def test_bf16_top8_at_full_size():
     say(f"agreement with FP64 argmin: {agree:.5f} overall, images outside the tie band: {ok.mean():.4f}")
     assert ok.mean() > 0.9
     assert np.array_equal(got[ok], w[ok]), f"{(got[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
+    gf = idxf.cpu().numpy()
+    assert np.array_equal(gf[ok], w[ok]), f"fused: {(gf[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
 
 
 @pytest.mark.parametrize("side", [128, 96])
@@ -103,6 +108,9 @@ def test_masked_matching_m151(side):
                                          return_index=True, masks=masks)
     q16, idx16 = alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref,
                                              precision="bf16", return_index=True, masks=masks)
+    say("  bf16 with this library's kernel (cross='fused', 16 candidates)")
+    q16f, idx16f = alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref,
+                                               precision="bf16", return_index=True, masks=masks, cross="fused")
     A = sl.slice(torch.from_numpy(ref).to(d))
     w2 = torch.full((P,), 1.0 / P ** 2, device=d)
     valid = imgs[mask_d.bool()].double()
@@ -123,6 +131,8 @@ def test_masked_matching_m151(side):
         say(f"  bf16 mismatch image {i}: d2(bf16 pick) {float(dd[0]):.9e}, d2(best) {float(dd[1]):.9e}, "
             f"second {float(d01[1, i]):.9e}, relative gap {float((dd[0] - dd[1]) / dd[1]):.2e}")
     assert np.array_equal(got16[ok], w[ok]), f"bf16: {(got16[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
+    g16f = idx16f.cpu().numpy()
+    assert np.array_equal(g16f[ok], w[ok]), f"bf16 fused: {(g16f[ok] != w[ok]).sum()} mismatches of {ok.sum()}"
 
 
 def test_solve_m151_autotuned():
