@@ -471,35 +471,60 @@ __global__ void center_rows_sq_kernel(const float* __restrict__ X, int64_t P, co
 
 // ---------------------------------------------------------------- FP32 MFMA argmin
 //
-// 256 threads = 4 waves as 2 (refs) x 2 (images); each wave owns a 64 x 64 block
-// computed as 4 x 4 tiles of v_mfma_f32_16x16x4f32.  Operand layouts of that
-// instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and D[i][j] in
-// lane j + 16 (i / 4), register i % 4.  K is staged through double-buffered LDS in
-// slabs of BK = 32 columns, each loaded from global memory two slabs ahead.  The order
-// of the K terms inside a slab is free (the same for A and B): lane group lk (lanes
-// 16 lk .. 16 lk + 15) takes slab columns 8 lk .. 8 lk + 7 in its eight MFMA steps, so a
-// lane reads its operands as two 16-byte LDS reads per tile row per slab instead of
-// eight 4-byte reads.  The LDS row pitch of 36 floats keeps those reads 16-byte
-// aligned, and 8 rows of one lane group start on 8 distinct 4-bank groups.
+// WM x WN waves; wave (wm, wn) owns TM x TN tiles of v_mfma_f32_16x16x4f32 (16 references
+// x 16 images each), so a workgroup covers BMt = 16 WM TM references by BNt = 16 WN TN
+// images.  Operand layouts of that instruction: A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k, and D[i][j] in lane j + 16 (i / 4), register i % 4.  K is staged through
+// double-buffered LDS in slabs of BK = 32 columns, each loaded from global memory two
+// slabs ahead (two register sets).  The order of the K terms inside a slab is free (the
+// same for A and B): lane group lk (lanes 16 lk .. 16 lk + 15) takes slab columns
+// 8 lk .. 8 lk + 7 in its eight MFMA steps, so a lane reads its operands as two 16-byte
+// LDS reads per tile row per slab instead of eight 4-byte reads.  The LDS row pitch of
+// 36 floats keeps those reads 16-byte aligned, and 8 rows of one lane group start on 8
+// distinct 4-bank groups.
+//
+// The launch uses 128 x 128 tiles (four waves of 64 x 64, two workgroups per CU).  At
+// 10000 references x 5000 images its 3160 tiles take 6.17 rounds of 512 resident
+// workgroups; the 88 tiles of the seventh round are split along K (SplitTail below) so
+// that round is not mostly idle.  A 256 x 160 tile (eight waves of 64 x 80, one
+// workgroup per CU: exactly 5 rounds) measured slower, 16.7 against 13.7 ms: with one
+// workgroup per CU nothing covers its barrier waits.
 
-// K slab width of the FP32 kernel (16 or 32 columns) and the padded LDS row pitch
-template <int BK>
-struct F32Slab {
-    static constexpr int LD = BK + 4;      // a multiple of 4 floats
-    static constexpr int TPR = BK / 4;     // threads (float4 each) per staged row
-    static constexpr int RPP = kThreads / TPR;  // rows per staging pass
-    static constexpr int NR = BM / RPP;    // float4 per thread per operand
+template <int WM, int WN, int TM, int TN, int BK>
+struct F32Tile {
+    static constexpr int kWaves = WM * WN;
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int BMt = 16 * WM * TM;
+    static constexpr int BNt = 16 * WN * TN;
+    static constexpr int LD = BK + 4;                                   // a multiple of 4 floats
+    static constexpr int TPR = BK / 4;                                  // threads (float4 each) per row
+    static constexpr int RPP = kThreads / TPR;                          // rows per staging pass
+    static constexpr int NA = (BMt + RPP - 1) / RPP;                    // float4 per thread, A
+    static constexpr int NB = (BNt + RPP - 1) / RPP;                    // float4 per thread, B
+    static constexpr int kWgPerCu = kWaves == 4 ? 2 : 1;
+    static_assert(kThreads % TPR == 0, "whole rows per staging pass");
 };
 
-template <int BK>
+// float4 u of this thread for the slab at column k0 of an operand of `rows` rows staged
+// from row0 (NR float4 per thread); rows past the tile (the last pass of a tile whose
+// rows do not divide evenly) are skipped, rows past the end of the operand read zeros, or
+// with FULL (a whole slab of 16-byte aligned rows) are clamped to the last row (their
+// products land in rows / columns the epilogue skips), so those loads carry no branches.
+template <class T, int NR, int TILE_ROWS, bool FULL>
 __device__ __forceinline__ void load_slab_f32(const float* __restrict__ X, int64_t rows, int64_t P,
-                                              int64_t row0, int64_t k0, float4 (&reg)[F32Slab<BK>::NR]) {
-    using S = F32Slab<BK>;
+                                              int64_t row0, int64_t k0, float4 (&reg)[NR]) {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < S::NR; ++i) {
-        const int64_t r = row0 + t / S::TPR + S::RPP * i;
-        const int64_t k = k0 + (t % S::TPR) * 4;
+    for (int i = 0; i < NR; ++i) {
+        const int lr = t / T::TPR + T::RPP * i;
+        if (TILE_ROWS % T::RPP != 0 && i == NR - 1 && lr >= TILE_ROWS) continue;
+        int64_t r = row0 + lr;
+        const int64_t k = k0 + (t % T::TPR) * 4;
+        if (FULL) {
+            r = r < rows ? r : rows - 1;
+            reg[i] = *reinterpret_cast<const float4*>(X + r * P + k);
+            continue;
+        }
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (r < rows) {
             const float* src = X + r * P + k;
@@ -516,29 +541,15 @@ __device__ __forceinline__ void load_slab_f32(const float* __restrict__ X, int64
     }
 }
 
-// Fast form for full slabs of 16-byte aligned rows (P % 4 == 0, aligned base): rows past
-// the end are clamped to the last row (their products land in rows / columns the
-// epilogue skips), so the loads carry no branches.
-template <int BK>
-__device__ __forceinline__ void load_slab_f32_full(const float* __restrict__ X, int64_t rows, int64_t P,
-                                                   int64_t row0, int64_t k0, float4 (&reg)[F32Slab<BK>::NR]) {
-    using S = F32Slab<BK>;
+template <class T, int NR, int TILE_ROWS>
+__device__ __forceinline__ void store_slab_f32(float* __restrict__ S_, const float4 (&reg)[NR]) {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < S::NR; ++i) {
-        int64_t r = row0 + t / S::TPR + S::RPP * i;
-        r = r < rows ? r : rows - 1;
-        reg[i] = *reinterpret_cast<const float4*>(X + r * P + k0 + (t % S::TPR) * 4);
+    for (int i = 0; i < NR; ++i) {
+        const int lr = t / T::TPR + T::RPP * i;
+        if (TILE_ROWS % T::RPP != 0 && i == NR - 1 && lr >= TILE_ROWS) continue;
+        *reinterpret_cast<float4*>(S_ + lr * T::LD + (t % T::TPR) * 4) = reg[i];
     }
-}
-
-template <int BK>
-__device__ __forceinline__ void store_slab_f32(float* __restrict__ S_, const float4 (&reg)[F32Slab<BK>::NR]) {
-    using S = F32Slab<BK>;
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int i = 0; i < S::NR; ++i)
-        *reinterpret_cast<float4*>(S_ + (t / S::TPR + S::RPP * i) * S::LD + (t % S::TPR) * 4) = reg[i];
 }
 
 __device__ __forceinline__ float f4_at(const float4& v, int c) {
@@ -547,92 +558,141 @@ __device__ __forceinline__ float f4_at(const float4& v, int c) {
 
 // WRITE_DIST: instead of the argmin, write the full matrix D[r, i] = sqrt(max(d2, 0))
 // (float32, row-major n_ref x n_img) as sklearn's euclidean_distances returns it.
-template <bool WRITE_DIST, bool ALIGNED, int BK, int PF>
-__global__ void __launch_bounds__(kThreads, BK == 16 ? 3 : 2)
+// Split tail: the last `rem` tiles of the launch (the partial last round of resident
+// workgroups) are each cut into `split` runs of consecutive K slabs, taken by workgroups
+// full .. full + rem * split - 1.  A run writes its partial tile to ws (per lane, its
+// accumulator registers) and counts itself in cnt[tile]; the run that completes the count
+// adds the split partials in run order (the same order every launch, so the result is
+// deterministic), resets the counter and finishes the tile.  split = 1: no tail split.
+struct SplitTail {
+    int full, rem, split;
+    float* ws;
+    int* cnt;
+};
+
+template <int WM, int WN, int TM, int TN, bool WRITE_DIST, bool ALIGNED>
+__global__ void __launch_bounds__(64 * WM * WN, (WM * WN == 4 ? 2 : 1))
     match_argmin_f32_kernel(const float* __restrict__ A, int64_t n_ref, uint32_t ref_offset,
                             const float* __restrict__ Bw, int64_t n_img, int64_t P,
                             const float* __restrict__ an, const float* __restrict__ bn,
                             const double* __restrict__ gamma, unsigned long long* __restrict__ best,
-                            float* __restrict__ dist, int tiles_m) {
-    using SL = F32Slab<BK>;
-    __shared__ __align__(16) float As[2][BM * SL::LD];
-    __shared__ __align__(16) float Bs[2][BN * SL::LD];
+                            float* __restrict__ dist, int tiles_m, SplitTail sp) {
+    constexpr int BK = 32;
+    using T = F32Tile<WM, WN, TM, TN, BK>;
+    __shared__ __align__(16) float As[2][T::BMt * T::LD];
+    __shared__ __align__(16) float Bs[2][T::BNt * T::LD];
+    __shared__ int last_run;
+    const int tiles = sp.full + sp.rem;
+    int b = blockIdx.x, run = 0, tail = -1;
+    if (b >= sp.full) {
+        tail = (b - sp.full) / sp.split;
+        run = (b - sp.full) - tail * sp.split;
+        b = sp.full + tail;
+    }
     int tm, tn;
-    tile_of<>(blockIdx.x, tiles_m, gridDim.x, &tm, &tn);
-    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    tile_of<>(b, tiles_m, tiles, &tm, &tn);
+    const int64_t m0 = (int64_t)tm * T::BMt, n0 = (int64_t)tn * T::BNt;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave % WM, wn = wave / WM;
     const int li = lane & 15, lk = lane >> 4;
 
-    floatx4 acc[4][4];
+    floatx4 acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < TN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    const int nk = (int)((P + BK - 1) / BK);
+    const int nk_all = (int)((P + BK - 1) / BK);
+    // this workgroup's slabs [k0, k0 + nk) (all of them unless the tile is split)
+    const int k0 = tail < 0 ? 0 : (int)((int64_t)nk_all * run / sp.split);
+    const int nk = tail < 0 ? nk_all : (int)((int64_t)nk_all * (run + 1) / sp.split) - k0;
     // slabs [0, nfull) are whole BK-column slabs; with ALIGNED they load without branches
     const int nfull = ALIGNED ? (int)(P / BK) : 0;
-    auto load = [&](int kt, float4 (&ra)[SL::NR], float4 (&rb)[SL::NR]) {
+    auto load = [&](int u, float4 (&ra)[T::NA], float4 (&rb)[T::NB]) {
+        const int kt = k0 + u;
         if (kt < nfull) {
-            load_slab_f32_full<BK>(A, n_ref, P, m0, (int64_t)kt * BK, ra);
-            load_slab_f32_full<BK>(Bw, n_img, P, n0, (int64_t)kt * BK, rb);
+            load_slab_f32<T, T::NA, T::BMt, true>(A, n_ref, P, m0, (int64_t)kt * BK, ra);
+            load_slab_f32<T, T::NB, T::BNt, true>(Bw, n_img, P, n0, (int64_t)kt * BK, rb);
         } else {
-            load_slab_f32<BK>(A, n_ref, P, m0, (int64_t)kt * BK, ra);
-            load_slab_f32<BK>(Bw, n_img, P, n0, (int64_t)kt * BK, rb);
+            load_slab_f32<T, T::NA, T::BMt, false>(A, n_ref, P, m0, (int64_t)kt * BK, ra);
+            load_slab_f32<T, T::NB, T::BNt, false>(Bw, n_img, P, n0, (int64_t)kt * BK, rb);
         }
+    };
+    auto store = [&](int buf, const float4 (&ra)[T::NA], const float4 (&rb)[T::NB]) {
+        store_slab_f32<T, T::NA, T::BMt>(As[buf], ra);
+        store_slab_f32<T, T::NB, T::BNt>(Bs[buf], rb);
     };
     auto compute = [&](int buf) {
         // lane group lk takes the slab's columns lk * BK / 4 .. (lk + 1) * BK / 4 - 1
-        const float* as = As[buf] + (wm * 64 + li) * SL::LD + lk * (BK / 4);
-        const float* bs = Bs[buf] + (wn * 64 + li) * SL::LD + lk * (BK / 4);
+        const float* as = As[buf] + (wm * TM * 16 + li) * T::LD + lk * (BK / 4);
+        const float* bs = Bs[buf] + (wn * TN * 16 + li) * T::LD + lk * (BK / 4);
 #pragma unroll
         for (int h = 0; h < BK / 16; ++h) {
-            float4 av[4], bv[4];
+            float4 av[TM], bv[TN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = *reinterpret_cast<const float4*>(as + i * 16 * SL::LD + 4 * h);
+            for (int i = 0; i < TM; ++i) av[i] = *reinterpret_cast<const float4*>(as + i * 16 * T::LD + 4 * h);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = *reinterpret_cast<const float4*>(bs + j * 16 * SL::LD + 4 * h);
+            for (int j = 0; j < TN; ++j) bv[j] = *reinterpret_cast<const float4*>(bs + j * 16 * T::LD + 4 * h);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < TM; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < TN; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_at(av[i], c), f4_at(bv[j], c),
                                                                          acc[i][j], 0, 0, 0);
         }
     };
-    if (PF == 1) {
-        float4 ra[SL::NR], rb[SL::NR];
-        load(0, ra, rb);
-        for (int kt = 0; kt < nk; ++kt) {
-            const int buf = kt & 1;
-            store_slab_f32<BK>(As[buf], ra);
-            store_slab_f32<BK>(Bs[buf], rb);
+    // two register sets: slab kt + 2 is requested while slab kt is computed, so each load
+    // has two slabs of MFMA work to arrive
+    float4 ra[T::NA], rb[T::NB], rc[T::NA], rd[T::NB];
+    load(0, ra, rb);
+    if (nk > 1) load(1, rc, rd);
+    for (int kt = 0; kt < nk; kt += 2) {
+        store(0, ra, rb);
+        __syncthreads();
+        if (kt + 2 < nk) load(kt + 2, ra, rb);
+        compute(0);
+        if (kt + 1 < nk) {
+            store(1, rc, rd);
             __syncthreads();
-            if (kt + 1 < nk) load(kt + 1, ra, rb);
-            compute(buf);
+            if (kt + 3 < nk) load(kt + 3, rc, rd);
+            compute(1);
         }
-    } else {
-        // two register sets: slab kt + 2 is requested while slab kt is computed, so each
-        // load has two slabs of MFMA work to arrive
-        float4 ra[SL::NR], rb[SL::NR], rc[SL::NR], rd[SL::NR];
-        load(0, ra, rb);
-        if (nk > 1) load(1, rc, rd);
-        for (int kt = 0; kt < nk; kt += 2) {
-            store_slab_f32<BK>(As[0], ra);
-            store_slab_f32<BK>(Bs[0], rb);
-            __syncthreads();
-            if (kt + 2 < nk) load(kt + 2, ra, rb);
-            compute(0);
-            if (kt + 1 < nk) {
-                store_slab_f32<BK>(As[1], rc);
-                store_slab_f32<BK>(Bs[1], rd);
-                __syncthreads();
-                if (kt + 3 < nk) load(kt + 3, rc, rd);
-                compute(1);
-            }
+    }
+
+    if (tail >= 0) {
+        // partial tile of a split run: lane-major registers, then count the run
+        float4* wsp = reinterpret_cast<float4*>(sp.ws) + ((int64_t)(tail * sp.split + run) * T::kThreads +
+                                                        threadIdx.x) * (TM * TN);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                wsp[i * TN + j] = make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) last_run = atomicAdd(sp.cnt + tail, 1) == sp.split - 1;
+        __syncthreads();
+        if (!last_run) return;
+        __threadfence();
+        if (threadIdx.x == 0) sp.cnt[tail] = 0;  // ready for the next launch
+        const float4* w0 = reinterpret_cast<const float4*>(sp.ws) + ((int64_t)(tail * sp.split) * T::kThreads +
+                                                                    threadIdx.x) * (TM * TN);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < sp.split; ++r) {
+            const float4* wr = w0 + (int64_t)r * T::kThreads * (TM * TN);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const float4 v = wr[i * TN + j];
+                    acc[i][j] += floatx4{v.x, v.y, v.z, v.w};
+                }
         }
     }
 
@@ -640,15 +700,15 @@ __global__ void __launch_bounds__(kThreads, BK == 16 ? 3 : 2)
     const float g2 = g * g, m2g = -2.0f * g;
     if (WRITE_DIST) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t c = n0 + wn * 64 + j * 16 + li;
+        for (int j = 0; j < TN; ++j) {
+            const int64_t c = n0 + wn * TN * 16 + j * 16 + li;
             if (c >= n_img) continue;
             const float bnc = bn[c];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int64_t r = m0 + wm * 64 + i * 16 + lk * 4 + v;
+                    const int64_t r = m0 + wm * TM * 16 + i * 16 + lk * 4 + v;
                     if (r < n_ref) {
                         const float d2 = fmaf(g2, an[r], fmaf(m2g, acc[i][j][v], bnc));
                         dist[r * n_img + c] = sqrtf(fmaxf(d2, 0.0f));
@@ -658,15 +718,15 @@ __global__ void __launch_bounds__(kThreads, BK == 16 ? 3 : 2)
         return;
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int64_t c = n0 + wn * 64 + j * 16 + li;
+    for (int j = 0; j < TN; ++j) {
+        const int64_t c = n0 + wn * TN * 16 + j * 16 + li;
         const float bnc = c < n_img ? bn[c] : 0.f;
         unsigned long long key = kEmpty;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t r = m0 + wm * 64 + i * 16 + lk * 4 + v;
+                const int64_t r = m0 + wm * TM * 16 + i * 16 + lk * 4 + v;
                 if (r < n_ref) {
                     const float d2 = fmaf(g2, an[r], fmaf(m2g, acc[i][j][v], bnc));
                     key = umin64(key, pack_key(d2, (uint32_t)r + ref_offset));
@@ -1472,14 +1532,103 @@ static int check_match_args(int64_t n_ref, int64_t ref_offset, int64_t n_img, in
 }
 
 // The FP32 kernel runs with 32-column slabs and two slabs of prefetch: measured at
-// 10000 x 5000 x 16384, 13.98 ms against 14.21 ms with one slab of prefetch, and 14.69 /
-// 14.97 ms with 16-column slabs (three workgroups per CU, one / two slabs of prefetch).
-template <bool WD, class... Args>
-static void launch_f32(bool aligned, unsigned tiles, hipStream_t st, Args... args) {
-    if (aligned)
-        hipLaunchKernelGGL((match_argmin_f32_kernel<WD, true, 32, 2>), dim3(tiles), dim3(kThreads), 0, st, args...);
-    else
-        hipLaunchKernelGGL((match_argmin_f32_kernel<WD, false, 32, 2>), dim3(tiles), dim3(kThreads), 0, st, args...);
+// 10000 x 5000 x 16384 with 128 x 128 tiles, 13.98 ms against 14.21 ms with one slab of
+// prefetch, and 14.69 / 14.97 ms with 16-column slabs (three workgroups per CU).
+// Shapes are picked by the work the launch is paid for: rounds of resident workgroups x
+// the tile area one CU holds per round (2 x 128 x 128 or 256 x 160).
+// Split-K workspace of the FP32 kernel, one per (device, stream): partial tiles and
+// self-resetting run counters.
+struct SplitWs {
+    float* ws = nullptr;
+    int* cnt = nullptr;
+    size_t ws_floats = 0;
+};
+std::mutex g_split_mu;
+std::map<std::pair<int, hipStream_t>, SplitWs> g_split;
+constexpr int kSplitMaxTiles = 4096;
+
+template <int WM, int WN, int TM, int TN>
+struct F32Launch {
+    using T = F32Tile<WM, WN, TM, TN, 32>;
+    static int64_t tiles_m(int64_t n_ref) { return (n_ref + T::BMt - 1) / T::BMt; }
+    static int64_t tiles(int64_t n_ref, int64_t n_img) { return tiles_m(n_ref) * ((n_img + T::BNt - 1) / T::BNt); }
+    // the partial last round of resident workgroups is split along K when it is less than
+    // half full: rem tiles x split runs fill up to one round; each run keeps >= 8 slabs
+    static SplitTail plan(int64_t n_ref, int64_t n_img, int64_t P, int cus) {
+        const int64_t t = tiles(n_ref, n_img), slots = (int64_t)cus * T::kWgPerCu;
+        const int64_t rem = t % slots, nk = (P + 31) / 32;
+        SplitTail sp{(int)t, 0, 1, nullptr, nullptr};
+        if (t < slots || rem == 0 || 2 * rem > slots || rem > kSplitMaxTiles) return sp;
+        const int64_t split = std::min<int64_t>(std::min<int64_t>(slots / rem, 8), nk / 8);
+        if (split < 2) return sp;
+        sp.full = (int)(t - rem);
+        sp.rem = (int)rem;
+        sp.split = (int)split;
+        return sp;
+    }
+    template <bool WD>
+    static int run(bool aligned, hipStream_t st, const float* A, int64_t n_ref, uint32_t off, const float* B,
+                   int64_t n_img, int64_t P, const float* an, const float* bn, const double* gamma,
+                   unsigned long long* best, float* dist, int cus) {
+        SplitTail sp = plan(n_ref, n_img, P, cus);
+        if (sp.split > 1) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess) {
+                set_error("match: hipGetDevice failed");
+                return CMTIP_ERR_HIP;
+            }
+            const size_t need = (size_t)sp.rem * sp.split * T::kThreads * TM * TN * 4;
+            std::lock_guard<std::mutex> lk(g_split_mu);
+            SplitWs& w = g_split[std::make_pair(dev, st)];
+            if (!w.cnt) {
+                if (hipMalloc(&w.cnt, sizeof(int) * kSplitMaxTiles) != hipSuccess ||
+                    hipMemset(w.cnt, 0, sizeof(int) * kSplitMaxTiles) != hipSuccess) {
+                    set_error("match: split-K counter allocation failed");
+                    return CMTIP_ERR_ALLOC;
+                }
+            }
+            if (w.ws_floats < need) {
+                if (w.ws) (void)hipFree(w.ws);
+                w.ws = nullptr;
+                w.ws_floats = 0;
+                if (hipMalloc(&w.ws, need * sizeof(float)) != hipSuccess) {
+                    set_error("match: split-K workspace allocation failed");
+                    return CMTIP_ERR_ALLOC;
+                }
+                w.ws_floats = need;
+            }
+            sp.ws = w.ws;
+            sp.cnt = w.cnt;
+        }
+        const unsigned grid = (unsigned)(sp.full + sp.rem * sp.split);
+        const int tm = (int)tiles_m(n_ref);
+        if (aligned)
+            hipLaunchKernelGGL((match_argmin_f32_kernel<WM, WN, TM, TN, WD, true>), dim3(grid), dim3(T::kThreads), 0,
+                               st, A, n_ref, off, B, n_img, P, an, bn, gamma, best, dist, tm, sp);
+        else
+            hipLaunchKernelGGL((match_argmin_f32_kernel<WM, WN, TM, TN, WD, false>), dim3(grid), dim3(T::kThreads),
+                               0, st, A, n_ref, off, B, n_img, P, an, bn, gamma, best, dist, tm, sp);
+        return CMTIP_OK;
+    }
+};
+using F32Small = F32Launch<2, 2, 4, 4>;  // 128 x 128, two workgroups per CU
+
+int match_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <bool WD>
+static int launch_f32(bool aligned, hipStream_t st, const float* A, int64_t n_ref, uint32_t off, const float* B,
+                      int64_t n_img, int64_t P, const float* an, const float* bn, const double* gamma,
+                      unsigned long long* best, float* dist) {
+    return F32Small::run<WD>(aligned, st, A, n_ref, off, B, n_img, P, an, bn, gamma, best, dist, match_cus());
 }
 
 extern "C" int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref_offset, const float* Bw,
@@ -1488,8 +1637,8 @@ extern "C" int cmtip_match_argmin_f32(const float* A, int64_t n_ref, int64_t ref
     int64_t tiles_m, tiles;
     CMTIP_TRY(check_match_args(n_ref, ref_offset, n_img, P, A, Bw, an, bn, best, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
-    launch_f32<false>(P % 4 == 0 && aligned16(A) && aligned16(Bw), (unsigned)tiles, as_stream(stream), A, n_ref,
-                      (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best, (float*)nullptr, (int)tiles_m);
+    CMTIP_TRY(launch_f32<false>(P % 4 == 0 && aligned16(A) && aligned16(Bw), as_stream(stream), A, n_ref,
+                                (uint32_t)ref_offset, Bw, n_img, P, an, bn, gamma, best, (float*)nullptr));
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }
@@ -1499,8 +1648,8 @@ extern "C" int cmtip_eudist_f32(const float* A, int64_t n_a, const float* B, int
     int64_t tiles_m, tiles;
     CMTIP_TRY(check_match_args(n_a, 0, n_b, P, A, B, an, bn, D, &tiles_m, &tiles));
     if (tiles == 0) return CMTIP_OK;
-    launch_f32<true>(P % 4 == 0 && aligned16(A) && aligned16(B), (unsigned)tiles, as_stream(stream), A, n_a, 0u, B,
-                     n_b, P, an, bn, (const double*)nullptr, (unsigned long long*)nullptr, D, (int)tiles_m);
+    CMTIP_TRY(launch_f32<true>(P % 4 == 0 && aligned16(A) && aligned16(B), as_stream(stream), A, n_a, 0u, B, n_b, P,
+                               an, bn, (const double*)nullptr, (unsigned long long*)nullptr, D));
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

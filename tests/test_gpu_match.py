@@ -180,6 +180,38 @@ def test_match_kernels_vs_fp64(n_ref, n_img, P):
     assert same > 0.9
 
 
+@pytest.mark.parametrize("n_ref,n_img,P", [(3840, 2560, 2048), (3800, 2500, 2051)])
+def test_fp32_split_tail_deterministic(n_ref, n_img, P):
+    # 600 tiles of 128 x 128 on 512 resident workgroups: the 88 tiles of the last round are
+    # split along K into 5 runs whose partials are added in run order; the argmin matches
+    # FP64 outside the tie band and repeated calls (counters reset by the kernel) give
+    # identical keys; aligned (P % 4 == 0) and scalar (P odd) loads
+    from cmtip_amd._lib import check, lib, ptr, stream
+    A, B, w2, gamma, D = _kernel_case(n_ref, n_img, P, 3)
+    want = torch.argmin(D, dim=0).cpu().numpy()
+    L = lib()
+    d = dev()
+    Bw = torch.empty_like(B)
+    Bw16 = torch.empty(B.shape, dtype=torch.int16, device=d)
+    check(L.cmtip_match_prepare_images(ptr(B), n_img, P, ptr(w2), ptr(Bw), ptr(Bw16), stream()))
+    an = torch.empty(n_ref, device=d)
+    bn = torch.empty(n_img, device=d)
+    check(L.cmtip_weighted_norms(ptr(A), n_ref, P, ptr(w2), ptr(an), stream()))
+    check(L.cmtip_weighted_norms(ptr(B), n_img, P, ptr(w2), ptr(bn), stream()))
+    say(f"fp32 split tail {n_ref}x{n_img}x{P}")
+    keys = []
+    for _ in range(3):
+        best = torch.full((n_img,), -1, dtype=torch.int64, device=d)
+        check(L.cmtip_match_argmin_f32(ptr(A), n_ref, 0, ptr(Bw), n_img, P, ptr(an), ptr(bn), ptr(gamma),
+                                       ptr(best), stream()))
+        keys.append(best.cpu().numpy())
+    assert np.array_equal(keys[0], keys[1]) and np.array_equal(keys[0], keys[2])
+    got = keys[0] & 0xFFFFFFFF
+    ok = _gap_ok(D, 1e-5)
+    assert ok.mean() > 0.5
+    assert np.array_equal(got[ok], want[ok]), f"mismatches: {(got[ok] != want[ok]).sum()} of {ok.sum()}"
+
+
 @pytest.mark.parametrize("n_ref,n_img,P,k", [(700, 300, 4096, 8), (517, 1111, 2048, 16), (10000, 5000, 16384, 16)])
 def test_bf16_kernel_paths_agree(n_ref, n_img, P, k):
     # the direct-to-LDS kernel (aligned rows, P a multiple of 64) and the register-staged
