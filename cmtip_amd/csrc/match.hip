@@ -788,6 +788,45 @@ __device__ __forceinline__ void global_insert(unsigned long long* slots, int k, 
 // (pitch BMt + 4 floats), four threads per image each scan a quarter of the references
 // (interleaved, so the 32 lanes of a half-wave read 32 distinct banks) keeping their L
 // smallest keys, and insert them into the image's global list with global_insert.
+// Top-k of one image's row of d2 in LDS (BMT references of the tile, first reference
+// index ref0): the image's four lanes (4q .. 4q + 3 of one wave, part = lane & 3) scan
+// references part, part + 4, ... keeping their L smallest keys, the four lists are merged
+// by k rounds of a 4-lane minimum (each key belongs to one lane), and lane 0 inserts the
+// merged keys into the image's global list until one is no better than its k-th entry.
+// Every lane of the wave must call this (the merge shuffles); valid = 0 scans nothing.
+template <int L, int BMT>
+__device__ __forceinline__ void tile_topk_insert(const float* drow, bool valid, int64_t rmax, uint32_t ref0, int k,
+                                                 unsigned long long* slots) {
+    const int part = threadIdx.x & 3;
+    unsigned long long lst[L];
+#pragma unroll
+    for (int s = 0; s < L; ++s) lst[s] = kEmpty;
+    if (valid)
+        for (int e = 0; e < BMT / 4; ++e) {
+            const int rl = 4 * e + part;
+            if (rl < rmax) topk_insert(lst, pack_key(drow[rl], ref0 + (uint32_t)rl));
+        }
+    bool done = !valid || part != 0;
+    unsigned long long kth = done ? 0ull : slots[k - 1];
+    for (int s = 0; s < k; ++s) {
+        unsigned long long m = umin64(lst[0], shfl_xor_u64(lst[0], 1));
+        m = umin64(m, shfl_xor_u64(m, 2));
+        if (lst[0] == m && m != kEmpty) {
+#pragma unroll
+            for (int t = 0; t < L - 1; ++t) lst[t] = lst[t + 1];
+            lst[L - 1] = kEmpty;
+        }
+        if (!done) {
+            if (m == kEmpty || m >= kth) {
+                done = true;
+            } else {
+                global_insert(slots, k, m);
+                kth = slots[k - 1];
+            }
+        }
+    }
+}
+
 template <int WM, int WN, int TM, int TN>
 struct XTile {
     static constexpr int kWaves = WM * WN;
@@ -944,24 +983,10 @@ __global__ void __launch_bounds__(64 * WM * WN, 1)
                         fmaf(g2, anr[i][v], fmaf(m2g, acc[i][j][v], bnc));
         }
         __syncthreads();
-        const int q = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const int q = threadIdx.x >> 2;
         const int64_t c = n0 + g0 + q;
-        if (g0 + q < T::BNt && c < n_img) {
-            unsigned long long lst[L];
-#pragma unroll
-            for (int s = 0; s < L; ++s) lst[s] = kEmpty;
-            const float* drow = D + q * T::kDPitch;
-            const int64_t rmax = n_ref - m0;  // references of this tile: rl < rmax
-            for (int e = 0; e < T::BMt / 4; ++e) {
-                const int rl = 4 * e + part;
-                if (rl < rmax) topk_insert(lst, pack_key(drow[rl], (uint32_t)(m0 + rl) + ref_offset));
-            }
-            unsigned long long* slots = cand + c * k;
-            for (int s = 0; s < k; ++s) {
-                if (lst[s] == kEmpty || lst[s] >= slots[k - 1]) break;
-                global_insert(slots, k, lst[s]);
-            }
-        }
+        tile_topk_insert<L, T::BMt>(D + q * T::kDPitch, g0 + q < T::BNt && c < n_img, n_ref - m0,
+                                    (uint32_t)m0 + ref_offset, k, cand + c * k);
         __syncthreads();
     }
 }
@@ -1169,24 +1194,10 @@ __global__ void __launch_bounds__(64 * WM * WN, 1)
                 }
         }
         __syncthreads();
-        const int q = threadIdx.x >> 2, part = threadIdx.x & 3;
+        const int q = threadIdx.x >> 2;
         const int64_t c = n0 + g0 + q;
-        if (g0 + q < T::BNt && c < n_img) {
-            unsigned long long lst[L];
-#pragma unroll
-            for (int s = 0; s < L; ++s) lst[s] = kEmpty;
-            const float* drow = D + q * T::kDPitch;
-            const int64_t rmax = n_ref - m0;
-            for (int e = 0; e < T::BMt / 4; ++e) {
-                const int rl = 4 * e + part;
-                if (rl < rmax) topk_insert(lst, pack_key(drow[rl], (uint32_t)(m0 + rl) + ref_offset));
-            }
-            unsigned long long* slots = cand + c * k;
-            for (int s = 0; s < k; ++s) {
-                if (lst[s] == kEmpty || lst[s] >= slots[k - 1]) break;
-                global_insert(slots, k, lst[s]);
-            }
-        }
+        tile_topk_insert<L, T::BMt>(D + q * T::kDPitch, g0 + q < T::BNt && c < n_img, n_ref - m0,
+                                    (uint32_t)m0 + ref_offset, k, cand + c * k);
         __syncthreads();
     }
 }
