@@ -20,6 +20,7 @@ CMTIP_PTS_REAL = 2
 CMTIP_PTS_ONES = 4
 CMTIP_PTS_IMAG = 8
 CMTIP_EXEC_ADJOINT = 16
+CMTIP_PTS_BOUND = 32
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -37,6 +38,7 @@ SIGNATURES = {
     "cmtip_nufft_makeplan": [_i, _i, _i, _d, ctypes.POINTER(_vp)],
     "cmtip_nufft_info": [_vp, _pi, _pi, _pd],
     "cmtip_nufft_setpts": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "cmtip_nufft_setpts_strengths": [_vp, _i64, _vp, _vp, _vp, _vp, _vp],
     "cmtip_nufft_points_max_abs": [_vp, ctypes.POINTER(ctypes.c_float), _vp],
     "cmtip_nufft_execute": [_vp, _vp, _vp, _i, _vp, _f, _vp],
     "cmtip_nufft_destroy": [_vp],

@@ -31,6 +31,7 @@ struct cmtip_nufft_plan_s {
     Fft1D fft;
     DevBuf deconv;   // n floats
     DevBuf fine;     // n^3 float2 (used as n^3 float when real)
+    DevBuf fine64;   // n^3 int64: fixed-point fine grid of real spreads (converted into fine)
     DevBuf tmpA;     // M * n * n float2
     DevBuf tmpB;     // M * M * n float2
     DevBuf rinv;     // 9 doubles per orientation
@@ -69,8 +70,9 @@ struct cmtip_nufft_plan_s {
     DevBuf perm;      // N int32: sorted position -> original index
     DevBuf sorted;    // N float4 (x, y, z, original index as int bits) in sorted order
     DevBuf chunks;    // max_chunks x int4 (bin, start, end, 0)
-    DevBuf ctrl;      // int32: [0] number of chunks, [1] work counter
+    DevBuf ctrl;      // int32: [0] number of chunks, [1] work counter, [2] bits of max |c|, [3] some c < 0
     int64_t max_chunks = 0;
+    int bound = 0;    // 1: the records carry real strengths (setpts_strengths), not indices
 };
 
 namespace {
@@ -270,31 +272,55 @@ struct PtsArrays {
 // |x| for the coordinate range check, NaN counted as out of range (+inf)
 __device__ __forceinline__ float range_abs(float v) { return v == v ? fabsf(v) : __int_as_float(0x7f800000); }
 
-// max |coordinate| of a wave's points into *amax (non-negative float bits order like
-// the values)
-__device__ __forceinline__ void wave_amax(float m, unsigned* amax) {
+
+// Workgroup maxima of non-negative float bits (and an OR of flags) through LDS, then one
+// global atomic per workgroup: per-wave atomics on one address serialise (16 k of them
+// cost ~0.3 ms per pass over 81.92 M points).
+__device__ __forceinline__ void block_max_or(float m, unsigned flag, unsigned* gmax, unsigned* gflag) {
+    __shared__ unsigned s_max, s_flag;
+    if (threadIdx.x == 0) {
+        s_max = 0u;
+        s_flag = 0u;
+    }
+    __syncthreads();
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(m));
+    const bool any = __ballot(flag != 0u) != 0;
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&s_max, __float_as_uint(m));
+        if (any) atomicOr(&s_flag, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (gmax) atomicMax(gmax, s_max);
+        if (gflag && s_flag) atomicOr(gflag, 1u);
+    }
 }
 
 template <class Src>
 __global__ void bin_count_kernel(Src pts, int64_t N, int n, int W, int bs, int nbx, float hi, float lo,
-                                 int* __restrict__ bin_of, int* __restrict__ count, unsigned* __restrict__ amax) {
+                                 int* __restrict__ bin_of, int* __restrict__ count, unsigned* __restrict__ amax,
+                                 const float* __restrict__ cstr, unsigned* __restrict__ cmax) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = j < N;
     int b = 0;
-    float m = 0.0f;
+    float m = 0.0f, cm = 0.0f;
+    bool neg = false;
     if (valid) {
         float t, x, y, z;
         pts.at(j, &x, &y, &z);
         if (amax) m = fmaxf(range_abs(x), fmaxf(range_abs(y), range_abs(z)));
+        if (cstr) {
+            cm = fabsf(cstr[j]);
+            neg = cstr[j] < 0.0f;
+        }
         const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
         const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
         const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
         b = (bx * nbx + by) * nbx + bz;
         bin_of[j] = b;
     }
-    if (amax) wave_amax(m, amax);
+    if (amax) block_max_or(m, 0u, amax, nullptr);
+    if (cstr) block_max_or(cm, neg ? 1u : 0u, cmax, cmax + 1);
     wave_bin_add(count, b, valid);
 }
 
@@ -369,7 +395,7 @@ template <bool COPY>
 __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                    const float* __restrict__ z, int64_t N, const int* __restrict__ bin_of,
                                    const int* __restrict__ offset, int* __restrict__ cursor, int* __restrict__ perm,
-                                   float4* __restrict__ sorted) {
+                                   float4* __restrict__ sorted, const float* __restrict__ cstr) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = j < N;
     const int b = valid ? bin_of[j] : 0;
@@ -377,7 +403,7 @@ __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __r
     if (!valid) return;
     const int pos = offset[b] + slot;
     if (COPY)
-        sorted[pos] = make_float4(x[j], y[j], z[j], __int_as_float((int)j));
+        sorted[pos] = make_float4(x[j], y[j], z[j], cstr ? cstr[j] : __int_as_float((int)j));
     else
         perm[pos] = (int)j;
 }
@@ -391,21 +417,31 @@ __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __r
 constexpr int kBinBlock = 1024;
 constexpr int64_t kLocalBinsMax = 36864;  // 144 KB of LDS counters
 
+// cstr (optional): real strengths bound to the points at setpts; their max |c| and
+// whether any is negative go to cmax[0], cmax[1] (the spread's fixed-point scale)
 template <class Src>
 __global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int64_t N, int64_t per_block, int n,
                                                                    int W, int bs, int nbx, int nbins, float hi,
                                                                    float lo, int* __restrict__ bin_of,
                                                                    int* __restrict__ count,
-                                                                   unsigned* __restrict__ amax) {
+                                                                   unsigned* __restrict__ amax,
+                                                                   const float* __restrict__ cstr,
+                                                                   unsigned* __restrict__ cmax) {
     extern __shared__ int hist[];
     for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
     __syncthreads();
     const int64_t j0 = (int64_t)blockIdx.x * per_block, j1 = min(N, j0 + per_block);
-    float m = 0.0f;
+    float m = 0.0f, cm = 0.0f;
+    bool neg = false;
     for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
         float t, x, y, z;
         pts.at(j, &x, &y, &z);
         if (amax) m = fmaxf(m, fmaxf(range_abs(x), fmaxf(range_abs(y), range_abs(z))));
+        if (cstr) {
+            const float v = cstr[j];
+            cm = fmaxf(cm, fabsf(v));
+            neg = neg || v < 0.0f;
+        }
         const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
         const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
         const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
@@ -413,17 +449,20 @@ __global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int
         bin_of[j] = b;
         atomicAdd(hist + b, 1);
     }
-    if (amax) wave_amax(m, amax);
+    if (amax) block_max_or(m, 0u, amax, nullptr);
+    if (cstr) block_max_or(cm, neg ? 1u : 0u, cmax, cmax + 1);
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += kBinBlock)
         if (hist[i]) atomicAdd(count + i, hist[i]);
 }
 
+// COPY: one 16-byte record per point, (x, y, z, original index as int bits), or with
+// cstr (x, y, z, strength)
 template <bool COPY>
 __global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int64_t N,
     int64_t per_block, int nbins, const int* __restrict__ bin_of, const int* __restrict__ offset,
-    int* __restrict__ cursor, int* __restrict__ perm, float4* __restrict__ sorted) {
+    int* __restrict__ cursor, int* __restrict__ perm, float4* __restrict__ sorted, const float* __restrict__ cstr) {
     extern __shared__ int hist[];
     for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
     __syncthreads();
@@ -439,7 +478,7 @@ __global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
     for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
         const int pos = atomicAdd(hist + bin_of[j], 1);
         if (COPY)
-            sorted[pos] = make_float4(x[j], y[j], z[j], __int_as_float((int)j));
+            sorted[pos] = make_float4(x[j], y[j], z[j], cstr ? cstr[j] : __int_as_float((int)j));
         else
             perm[pos] = (int)j;
     }
@@ -652,12 +691,7 @@ __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsign
         done = n4 << 2;
     }
     for (int64_t i = done + tid; i < count; i += nth) take(c[i]);
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    const bool any_neg = __ballot(neg) != 0;
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(out_bits, __float_as_uint(m));
-        if (any_neg) atomicOr(out_bits + 1, 1u);
-    }
+    block_max_or(m, neg ? 1u : 0u, out_bits, out_bits + 1);
 }
 
 // Fixed-point value of a non-negative float x < 2^63, rounded to nearest: the two 32-bit
@@ -676,7 +710,8 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     typename Cell<CPLX>::T* __restrict__ grid, int n, const EsKernel esk, float hi, float lo, int bs, int nbx,
     const float4* __restrict__ sorted, const typename Cell<CPLX>::T* __restrict__ c,
     const unsigned* __restrict__ cmax_bits,
-    const int4* __restrict__ chunks, int* __restrict__ ctrl) {
+    const int4* __restrict__ chunks, int* __restrict__ ctrl, int bound, unsigned long long* __restrict__ fixed,
+    int shift) {
     constexpr int NC = CPLX ? 2 : 1;  // integer accumulators per cell
     extern __shared__ unsigned long long acc[];
     // LDS layout x * SSP + y * SP + z (spread_pitch)
@@ -689,8 +724,13 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     __shared__ uint16_t dealt[kChunk];  // chunk-local point indices grouped by class
     __shared__ uint8_t cls_of[kChunk];
     const int total = ctrl[0];
-    const int sc = spread_scale_exp(c ? __uint_as_float(*cmax_bits) : 1.0f);
-    const bool nonneg = !CPLX && (c == nullptr || cmax_bits[1] == 0u);
+    // bound: real strengths carried in the records (setpts_strengths); their max |c| and
+    // sign were recorded at setpts
+    const bool has_c = c != nullptr || bound;
+    // fixed (real spreads): the chunks' integer sums are added into a 64-bit fine grid,
+    // exactly and in any order, so the scale leaves room for all points (shift)
+    const int sc = spread_scale_exp(has_c ? __uint_as_float(*cmax_bits) : 1.0f) - (fixed ? shift : 0);
+    const bool nonneg = !CPLX && (!has_c || cmax_bits[1] == 0u);
     const double down = ldexp(1.0, -sc);
     for (;;) {
         if (threadIdx.x == 0) chunk_id = atomicAdd(ctrl + 1, 1);
@@ -771,7 +811,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                 vr = ldexpf(cv.x, sc);
                 vi = ldexpf(cv.y, sc);
             } else {
-                vr = ldexpf(c ? reinterpret_cast<const float*>(c)[src] : 1.0f, sc);
+                vr = ldexpf(bound ? q.w : (c ? reinterpret_cast<const float*>(c)[src] : 1.0f), sc);
             }
             unsigned long long* base = acc + NC * (lx * SSP + ly * SP + lz);
             if (nonneg) {
@@ -812,6 +852,11 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             gy = gy >= n ? gy - n : gy;
             gz = gz >= n ? gz - n : gz;
             const uint32_t off = (uint32_t)((gx * n + gy) * n + gz);
+            if (!CPLX && fixed) {
+                const unsigned long long v = acc[li];
+                if (v != 0ull) atomicAdd(fixed + off, v);
+                continue;
+            }
             float* dst = reinterpret_cast<float*>(grid) + (size_t)NC * off;
 #pragma unroll
             for (int q = 0; q < NC; ++q) {
@@ -821,6 +866,17 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         }
         __syncthreads();
     }
+}
+
+// Fixed-point fine grid of a real spread to float: value = int64 sum x 2^-(sc - shift), the
+// scale the spread used (from the same max |c| in device memory).
+__global__ void fixed_to_float_kernel(const unsigned long long* __restrict__ fixed, int64_t count,
+                                      const unsigned* __restrict__ cmax_bits, int has_c, int shift,
+                                      float* __restrict__ out) {
+    const int sc = spread_scale_exp(has_c ? __uint_as_float(*cmax_bits) : 1.0f) - shift;
+    const double down = ldexp(1.0, -sc);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBlock)
+        out[i] = (float)((double)(long long)fixed[i] * down);
 }
 
 // Sorted point k of the interpolation: coordinates and output index.  SortedCopies:
@@ -1642,11 +1698,12 @@ struct InterpLaunch {
 
 template <int W>
 struct SpreadLaunch {
-    static int run(cmtip_nufft_plan_s* p, bool cplx, const void* c, hipStream_t st) {
+    // bound: real strengths carried in the point records, max |c| recorded at setpts
+    static int run(cmtip_nufft_plan_s* p, bool cplx, const void* c, bool bound, hipStream_t st) {
         if (p->npts == 0) return CMTIP_OK;
         CMTIP_TRY(reset_work_counter(p, st));
         unsigned* cmax = reinterpret_cast<unsigned*>(p->ctrl.as<int>() + 2);
-        if (c) {
+        if (c && !bound) {
             hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int*>(cmax),
                                (int64_t)2, 0);
             const int64_t cnt = (cplx ? 2 : 1) * p->npts;
@@ -1662,16 +1719,28 @@ struct SpreadLaunch {
             allow_large_lds(cplx ? (const void*)spread_binned_kernel<W, true> : (const void*)spread_binned_kernel<W, false>);
             attr_set[cplx] = true;
         }
-        if (cplx)
+        if (cplx) {
             hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadThreads), lds, st,
                                p->fine.as<float2>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
                                p->sorted.as<const float4>(), reinterpret_cast<const float2*>(c), cmax, ch,
-                               p->ctrl.as<int>());
-        else
-            hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
-                               p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
-                               p->sorted.as<const float4>(), reinterpret_cast<const float*>(c), cmax, ch,
-                               p->ctrl.as<int>());
+                               p->ctrl.as<int>(), 0, (unsigned long long*)nullptr, 0);
+            CMTIP_LAUNCH_CHECK();
+            return CMTIP_OK;
+        }
+        // real: the chunks' integer sums go into the zeroed 64-bit grid (exact, so the result
+        // does not depend on the order of the adds), scaled so that npts full-weight
+        // contributions stay below 2^63, then one pass converts it into the float fine grid
+        int bits = 0;
+        while (bits < 62 && (int64_t(1) << bits) <= p->npts) ++bits;  // npts < 2^bits
+        const int shift = std::max(0, bits - 14);
+        const int64_t n3 = (int64_t)p->n * p->n * p->n;
+        hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
+                           p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                           p->sorted.as<const float4>(), reinterpret_cast<const float*>(c), cmax, ch,
+                           p->ctrl.as<int>(), bound ? 1 : 0, p->fine64.as<unsigned long long>(), shift);
+        hipLaunchKernelGGL(fixed_to_float_kernel, dim3((unsigned)std::min<int64_t>(65536, (n3 + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, st, p->fine64.as<const unsigned long long>(), n3,
+                           (const unsigned*)cmax, (c || bound) ? 1 : 0, shift, p->fine.as<float>());
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
     }
@@ -1679,7 +1748,10 @@ struct SpreadLaunch {
 
 // Sort the registered points (or, with rot, the slicing route's rotated pixels) into
 // spatial bins (counting sort) and build the chunk list.
-int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nullptr) {
+// cstr (optional, registered points only): real strengths stored in the records in place of
+// the original indices, with their max |c| and sign recorded in ctrl[2], ctrl[3]
+int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nullptr,
+               const float* cstr = nullptr) {
     const int64_t N = p->npts;
     p->bs = p->bs_req > 0 ? p->bs_req : bin_size_for(p->w);
     p->chunk = p->chunk_req > 0 ? p->chunk_req : kChunk;
@@ -1699,6 +1771,9 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
     const unsigned gp = blocks_for(N);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(gb), dim3(kBlock), 0, st, p->count.as<int>(), p->nbins, 0);
+    unsigned* cmax = reinterpret_cast<unsigned*>(p->ctrl.as<int>() + 2);
+    if (cstr) hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->ctrl.as<int>() + 2, (int64_t)2, 0);
+    p->bound = cstr ? 1 : 0;
     // registered point arrays: their max |coordinate| for the caller's range check
     unsigned* amax = nullptr;
     p->amax_valid = 0;
@@ -1724,22 +1799,23 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
         if (rot)
             hipLaunchKernelGGL(bin_count_local_kernel<PtsRotated>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                *rot, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi, p->tc_lo,
-                               p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr);
+                               p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr, (const float*)nullptr,
+                               (unsigned*)nullptr);
         else
             hipLaunchKernelGGL(bin_count_local_kernel<PtsArrays>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                PtsArrays{p->x, p->y, p->z}, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi,
-                               p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), amax);
+                               p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), amax, cstr, cmax);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                            p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
         if (rot)
             hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, N, per_block, nb,
                                p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                               p->perm.as<int>(), (float4*)nullptr);
+                               p->perm.as<int>(), (float4*)nullptr, (const float*)nullptr);
         else
             hipLaunchKernelGGL(bin_scatter_local_kernel<true>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st, p->x,
                                p->y, p->z, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
-                               p->count.as<int>(), (int*)nullptr, p->sorted.as<float4>());
+                               p->count.as<int>(), (int*)nullptr, p->sorted.as<float4>(), cstr);
         hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                            p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
         CMTIP_LAUNCH_CHECK();
@@ -1747,21 +1823,23 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     }
     if (rot)
         hipLaunchKernelGGL(bin_count_kernel<PtsRotated>, dim3(gp), dim3(kBlock), 0, st, *rot, N, p->n, p->w, p->bs,
-                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr);
+                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr,
+                           (const float*)nullptr, (unsigned*)nullptr);
     else
         hipLaunchKernelGGL(bin_count_kernel<PtsArrays>, dim3(gp), dim3(kBlock), 0, st, PtsArrays{p->x, p->y, p->z}, N,
                            p->n, p->w, p->bs, p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(),
-                           amax);
+                           amax, cstr, cmax);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                        p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
     if (rot)
         hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(gp), dim3(kBlock), 0, st, (const float*)nullptr,
                            (const float*)nullptr, (const float*)nullptr, N, p->bin_of.as<const int>(),
-                           p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr);
+                           p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr,
+                           (const float*)nullptr);
     else
         hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
                            p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                           (int*)nullptr, p->sorted.as<float4>());
+                           (int*)nullptr, p->sorted.as<float4>(), cstr);
     hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                        p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
     CMTIP_LAUNCH_CHECK();
@@ -1810,7 +1888,7 @@ struct TileLaunch {
             hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), hlds, st,
                                (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, ntiles, per_block,
                                (int)(nbins + 1), p->bin_of.as<const int>(), p->offset.as<const int>(),
-                               p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr);
+                               p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr, (const float*)nullptr);
         else
             hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
                                p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
@@ -2129,8 +2207,9 @@ extern "C" int cmtip_nufft_get_bins(cmtip_nufft_plan p, int* bin_size, int* chun
     return CMTIP_OK;
 }
 
-extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y,
-                                  const float* z, void* stream) {
+namespace {
+int setpts_impl(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y, const float* z, const float* c,
+                bool bind, void* stream) {
     if (!valid_plan(p)) return CMTIP_ERR_ARG;
     if (npts < 0 || (npts > 0 && (!x || !y || !z))) {
         set_error("setpts: bad arguments");
@@ -2144,11 +2223,33 @@ extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float*
     p->x = x;
     p->y = y;
     p->z = z;
+    p->bound = 0;
     if (npts == 0) {
         p->amax_valid = 2;  // no points: the maximum is 0
+        p->bound = bind ? 1 : 0;
         return CMTIP_OK;
     }
-    return bin_points(p, as_stream(stream));
+    return bin_points(p, as_stream(stream), nullptr, c);
+}
+}  // namespace
+
+extern "C" int cmtip_nufft_setpts(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y,
+                                  const float* z, void* stream) {
+    return setpts_impl(p, npts, x, y, z, nullptr, false, stream);
+}
+
+extern "C" int cmtip_nufft_setpts_strengths(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y,
+                                            const float* z, const float* c, void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (p->type != 1) {
+        set_error("setpts_strengths: type-1 plans only");
+        return CMTIP_ERR_ARG;
+    }
+    if (npts > 0 && !c) {
+        set_error("setpts_strengths: null strengths");
+        return CMTIP_ERR_ARG;
+    }
+    return setpts_impl(p, npts, x, y, z, c, true, stream);
 }
 
 extern "C" int cmtip_nufft_points_max_abs(cmtip_nufft_plan p, float* max_abs, void* stream) {
@@ -2197,9 +2298,18 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     const bool grid_real = flags & CMTIP_GRID_REAL;
     const bool pts_ones = (flags & CMTIP_PTS_ONES) && type == 1;
     const bool pts_imag = (flags & CMTIP_PTS_IMAG) && type == 2;
-    const bool pts_real = (flags & CMTIP_PTS_REAL) || pts_ones || pts_imag;
-    if (pts_ones) c = nullptr;
-    if (!f || (p->npts > 0 && !c && !pts_ones)) {
+    const bool pts_bound = (flags & CMTIP_PTS_BOUND) != 0;
+    const bool pts_real = (flags & CMTIP_PTS_REAL) || pts_ones || pts_imag || pts_bound;
+    if (pts_bound && (type != 1 || pts_ones || !p->bound)) {
+        set_error("execute: CMTIP_PTS_BOUND needs a type-1 execute after setpts_strengths");
+        return CMTIP_ERR_STATE;
+    }
+    if (p->bound && !pts_bound && !pts_ones) {
+        set_error("execute: the registered records carry strengths (setpts_strengths); call setpts again");
+        return CMTIP_ERR_STATE;
+    }
+    if (pts_ones || pts_bound) c = nullptr;
+    if (!f || (p->npts > 0 && !c && !pts_ones && !pts_bound)) {
         set_error("execute: null data pointer");
         return CMTIP_ERR_ARG;
     }
@@ -2229,8 +2339,14 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
     }
     p->slice_ready = 0;
     const int64_t n3 = (int64_t)p->n * p->n * p->n;
-    CMTIP_TRY(fill_zero(p->fine.p, pts_real ? n3 : 2 * n3, st));
-    CMTIP_TRY(dispatch_w<SpreadLaunch>(p->w, p, !pts_real, (const void*)c, st));
+    if (pts_real) {
+        // real spreads accumulate in the 64-bit fixed-point grid (converted into fine)
+        CMTIP_TRY(p->fine64.ensure(sizeof(unsigned long long) * (size_t)n3));
+        CMTIP_TRY(fill_zero(p->fine64.p, 2 * n3, st));
+    } else {
+        CMTIP_TRY(fill_zero(p->fine.p, 2 * n3, st));
+    }
+    CMTIP_TRY(dispatch_w<SpreadLaunch>(p->w, p, !pts_real, (const void*)c, pts_bound, st));
     return fine_to_grid(p, pts_real, f, grid_real, support, scale, st);
 }
 

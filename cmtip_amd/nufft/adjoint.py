@@ -6,7 +6,7 @@ import numpy as np
 import torch
 
 from .. import _arrays
-from .._lib import CMTIP_GRID_REAL, CMTIP_PTS_ONES, CMTIP_PTS_REAL
+from .._lib import CMTIP_GRID_REAL, CMTIP_PTS_BOUND, CMTIP_PTS_ONES, CMTIP_PTS_REAL
 from .plan import DEFAULT_EPS, get_plan, tuned_setpts
 
 
@@ -36,8 +36,14 @@ def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, one
     sup = None if support is None else _arrays.to_dev(support, torch.float32)
     plan = get_plan(1, M, +1, eps)
     scale = float(1.0 / M ** 3) if scale is None else float(scale)
+    bind = not autotune and c is not None and (flags & CMTIP_PTS_REAL)
     if autotune:
         tuned_setpts(plan, x, y, z, c, f, flags, sup, scale)
+    elif bind:
+        # real strengths ride in the bin-sorted point records: no gather of c by index
+        plan.setpts_strengths(x, y, z, c)
+        flags |= CMTIP_PTS_BOUND
+        c = None
     else:
         plan.setpts(x, y, z)
     # the range check of adjoint.py:29, 66: the maximum is recorded while setpts bins

@@ -56,6 +56,18 @@ class NufftPlan:
         check(_lib.lib().cmtip_nufft_setpts(self._h, n, ptr(x), ptr(y), ptr(z), stream()), "cmtip_nufft_setpts")
         self._pts_key = key
 
+    def setpts_strengths(self, x, y, z, c):
+        """Type-1 plans: register the points with their real strengths c (float32, one per
+        point), carried in the bin-sorted records; execute with CMTIP_PTS_BOUND (c None)."""
+        n = x.numel()
+        assert y.numel() == n and z.numel() == n and c.numel() == n
+        for t in (x, y, z, c):
+            assert t.dtype == torch.float32 and t.is_cuda and t.is_contiguous()
+        self._pts = (x, y, z, c)
+        self._pts_key = None  # the records hold strengths: the next setpts bins again
+        check(_lib.lib().cmtip_nufft_setpts_strengths(self._h, n, ptr(x), ptr(y), ptr(z), ptr(c), stream()),
+              "cmtip_nufft_setpts_strengths")
+
     def points_max_abs(self):
         """Largest |coordinate| of the registered points (NaN as inf), recorded by setpts."""
         v = ctypes.c_float()

@@ -68,6 +68,17 @@ int cmtip_nufft_info(cmtip_nufft_plan plan, int* w, int* n_fine, double* beta);
 int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, const float* y,
                        const float* z, void* stream);
 
+/* Type-1 plans: register the points together with real strengths c (npts floats).  Each
+ * point's bin-sorted record carries its strength in place of its index, and max |c| and
+ * the sign of c are recorded while the points are binned, so a following execute with
+ * CMTIP_PTS_BOUND (c ignored) spreads without gathering c by index and without a pass
+ * over c.  No counterpart in the reference: cufinufft takes the strengths at execute
+ * (adjoint.py:75); this serves solve_ac's right-hand side (autocorrelation.py:162-168),
+ * whose strengths are known when the points are registered.  Until the next setpts the
+ * plan executes only CMTIP_PTS_BOUND or CMTIP_PTS_ONES (CMTIP_ERR_STATE otherwise). */
+int cmtip_nufft_setpts_strengths(cmtip_nufft_plan plan, int64_t npts, const float* x, const float* y,
+                                 const float* z, const float* c, void* stream);
+
 /* Largest |coordinate| of the point arrays registered by the last setpts (NaN counts as
  * +inf), recorded while the points were binned: the range check of forward.py:27,64 and
  * adjoint.py:29,66 without another pass over the points.  Synchronises the stream.
@@ -81,6 +92,8 @@ int cmtip_nufft_points_max_abs(cmtip_nufft_plan plan, float* max_abs, void* stre
                                all-ones kernel input of autocorrelation.py:143,146 */
 #define CMTIP_PTS_IMAG 8    /* type 2 only: c (float) receives Im(T2(f)), e.g. the
                                coordinate gradient of a real slice */
+#define CMTIP_PTS_BOUND 32  /* type 1 only: the real strengths registered with the points by
+                               cmtip_nufft_setpts_strengths (c is ignored) */
 #define CMTIP_EXEC_ADJOINT 16 /* run the adjoint of the plan's transform on its registered
                                points: a type-2 plan spreads (type 1, sign flipped), a
                                type-1 plan interpolates (type 2, sign flipped); c and f

@@ -224,8 +224,9 @@ def test_setpts_skips_only_unchanged_points():
 @pytest.mark.parametrize("real", [True, False])
 def test_adjoint_flag_matches_the_other_plan_type(real):
     # a plan's adjoint on its own bins gives what the plan of the other type gives on
-    # the same points (same bins, same sign): the interpolation bit for bit, the spread
-    # up to the order of the float atomics that flush its chunks into the fine grid
+    # the same points (same bins, same sign): the interpolation bit for bit, the real
+    # spread bit for bit (exact 64-bit fixed-point sums), the complex spread up to the
+    # order of the float atomics that flush its chunks into the fine grid
     from cmtip_amd._lib import CMTIP_EXEC_ADJOINT, CMTIP_GRID_REAL, CMTIP_PTS_REAL, CmtipError
     from cmtip_amd.nufft import NufftPlan
     d = dev()
@@ -250,7 +251,10 @@ def test_adjoint_flag_matches_the_other_plan_type(real):
     want_f = torch.empty_like(fo)
     p1.execute(c, want_f, flags, None, 1.0 / M ** 3)
     p2.execute(c, fo, flags | CMTIP_EXEC_ADJOINT, None, 1.0 / M ** 3)
-    assert rel_l2(fo, want_f) < 1e-6
+    if real:
+        assert torch.equal(fo, want_f)  # exact integer sums: independent of the bins
+    else:
+        assert rel_l2(fo, want_f) < 1e-6
     want_c = torch.empty_like(co)
     p2.execute(want_c, f, flags)
     p1.execute(co, f, flags | CMTIP_EXEC_ADJOINT)
@@ -280,3 +284,53 @@ def test_lds_atomic_probe_values():
         np.add.at(acc, (t + 64 * it) & 8191, it + 1)
     got = out.cpu().numpy().reshape(wg, 512)
     np.testing.assert_allclose(got, np.broadcast_to(acc[:512], (wg, 512)), rtol=1e-6)
+
+
+@pytest.mark.parametrize("M,N,signed", [(21, 4000, False), (21, 4000, True), (81, 2_000_000, False)])
+def test_setpts_strengths(M, N, signed):
+    # strengths carried in the point records (setpts_strengths + CMTIP_PTS_BOUND) give the
+    # spread of the same strengths gathered by index (bit for bit) and the exact sums; max
+    # |coordinate| is still
+    # recorded; executes that need indices or other strengths are refused until the next
+    # setpts, unit strengths are allowed
+    from cmtip_amd._lib import (CMTIP_EXEC_ADJOINT, CMTIP_GRID_REAL, CMTIP_PTS_BOUND, CMTIP_PTS_ONES,
+                                CMTIP_PTS_REAL, CmtipError)
+    from cmtip_amd.nufft import NufftPlan
+    d = dev()
+    rng = np.random.default_rng(M + N + signed)
+    xs, ys, zs = points(rng, N, lim=np.pi)
+    cs = (rng.standard_normal(N) if signed else rng.random(N) * 50).astype(np.float32)
+    x, y, z, c = (torch.as_tensor(v, device=d) for v in (xs, ys, zs, cs))
+    flags = CMTIP_GRID_REAL | CMTIP_PTS_REAL
+    p = NufftPlan(1, M, +1)
+    say(f"setpts_strengths M={M} N={N} signed={signed}")
+    want = torch.empty((M, M, M), device=d)
+    p.setpts(x, y, z)
+    p.execute(c, want, flags, None, 1.0 / M ** 3)
+    got = torch.empty_like(want)
+    p.setpts_strengths(x, y, z, c)
+    assert abs(p.points_max_abs() - float(np.abs(np.stack([xs, ys, zs])).max())) == 0.0
+    p.execute(None, got, flags | CMTIP_PTS_BOUND, None, 1.0 / M ** 3)
+    # real spreads add exact 64-bit integer sums into the fine grid: the same result bit
+    # for bit whatever the order of the adds or the point records' layout
+    assert torch.equal(got, want)
+    again = torch.empty_like(want)
+    p.execute(None, again, flags | CMTIP_PTS_BOUND, None, 1.0 / M ** 3)
+    assert torch.equal(again, got)
+    if N <= 4000:
+        exact = O.nudft1(cs.astype(np.complex128), xs, ys, zs, M).real / M ** 3
+        assert rel_l2(got, exact) < TOL
+    ones_b = torch.empty_like(want)
+    p.execute(None, ones_b, CMTIP_GRID_REAL | CMTIP_PTS_ONES, None, 1.0)
+    with pytest.raises(CmtipError):
+        p.execute(c, got, flags, None, 1.0)
+    with pytest.raises(CmtipError):
+        p.execute(torch.empty(N, device=d), want, flags | CMTIP_EXEC_ADJOINT)
+    p.setpts(x, y, z)
+    with pytest.raises(CmtipError):
+        p.execute(None, got, flags | CMTIP_PTS_BOUND, None, 1.0)
+    ones = torch.empty_like(want)
+    p.execute(None, ones, CMTIP_GRID_REAL | CMTIP_PTS_ONES, None, 1.0)
+    assert torch.equal(ones_b, ones)
+    with pytest.raises(CmtipError):
+        NufftPlan(2, M, -1).setpts_strengths(x, y, z, c)

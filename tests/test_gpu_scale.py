@@ -137,7 +137,7 @@ def test_masked_matching_m151(side):
 
 def test_solve_m151_autotuned():
     # M = 151 solve over 1000 images of 96 x 96 pixels (9.2 M points): spreads with
-    # autotuned bin settings give the same solution as the defaults
+    # autotuned bin settings give the same solution as the defaults, bit for bit
     from cmtip_amd import nufft
     from cmtip_amd.autocorrelation import solve_ac
     d = dev()
@@ -152,5 +152,8 @@ def test_solve_m151_autotuned():
     a1, i1 = solve_ac(0, s, ext, imgs.reshape(n_img, 1, -1), M, orientations=q, maxiter=10, rlambda=1e-4,
                       return_info=True, autotune=True)
     assert i0["iterations"] == i1["iterations"] == 10
-    assert rel_l2(a1, a0) < 1e-5
+    # real spreads sum exact 64-bit fixed-point contributions, so other bin settings give
+    # the same spreads bit for bit, and the CG (fixed-order FP64 reductions) the same solve
+    say(f"  rel L2 autotuned vs default {rel_l2(a1, a0):.3e}")
+    assert torch.equal(a1, a0)
     nufft.clear_plans()
