@@ -344,6 +344,14 @@ def main():
                                           1.0 / modes ** 3))
             res_nufft[label] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": Np / te / 1e6,
                                 "Mpts_s_with_setpts": Np / (te + ts) / 1e6}
+        # the solve's rhs form: real strengths bound to the point records at setpts
+        from cmtip_amd._lib import CMTIP_PTS_BOUND
+        p = nufft.get_plan(1, M, 1)
+        ts = tm(lambda: p.setpts_strengths(x, y, z, data))
+        o = torch.empty((M,) * 3, device=dev)
+        te = tm(lambda: p.execute(None, o, CMTIP_GRID_REAL | CMTIP_PTS_REAL | CMTIP_PTS_BOUND, None, 1.0 / M ** 3))
+        res_nufft["type1_real_M81_bound"] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": Np / te / 1e6,
+                                             "Mpts_s_with_setpts": Np / (te + ts) / 1e6}
         t2 = res_nufft["type2_real"]["execute_ms"] * 1e-3
         t1 = res_nufft["type1_real_M81"]["execute_ms"] * 1e-3
         # 20 B per point + 4 M^3 per transform (real form), execute only
@@ -554,6 +562,14 @@ def part_main(args, M, atoms, side, n_img, n_ref, dev):
             te = tm(lambda: p.execute(data, out, flags, None, 1.0 / modes ** 3))
         res[label] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": N / te / 1e6}
         log(f"[bench] {label}: setpts {ts * 1e3:.2f} ms, execute {te * 1e3:.2f} ms, {N / te / 1e6:.0f} Mpts/s")
+    # real strengths bound to the point records at setpts (the solve's rhs form)
+    from cmtip_amd._lib import CMTIP_PTS_BOUND
+    p = nufft.get_plan(1, M, 1)
+    ts = tm(lambda: p.setpts_strengths(x, y, z, data), 3)
+    out = torch.empty((M,) * 3, device=dev)
+    te = tm(lambda: p.execute(None, out, CMTIP_GRID_REAL | CMTIP_PTS_REAL | CMTIP_PTS_BOUND, None, 1.0 / M ** 3))
+    res["type1_real_bound"] = {"setpts_ms": ts * 1e3, "execute_ms": te * 1e3, "Mpts_s": N / te / 1e6}
+    log(f"[bench] type1_real_bound: setpts {ts * 1e3:.2f} ms, execute {te * 1e3:.2f} ms, {N / te / 1e6:.0f} Mpts/s")
     print(json.dumps({"part": "nufft", "M": M, **res}), flush=True)
 
 
