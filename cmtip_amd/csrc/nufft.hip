@@ -705,15 +705,26 @@ __device__ __forceinline__ unsigned long long fixed_nonneg(float x) {
     return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
 }
 
-template <int W, bool CPLX>
+// U32 (unit strengths only): 32-bit LDS accumulators at the scale 2^kOnes32Scale per
+// contribution (every weight product is <= 1, so a chunk's kChunk = 2^12 contributions to
+// a cell stay below 2^31), added to the 64-bit fine grid shifted to its scale: exact, so
+// the result is the same as the 64-bit path's up to the rounding of each weight product
+// to 2^-19 (at most 1e-6 per tap), and deterministic.  Half the LDS per subgrid and
+// twice the atomic rate of 64-bit LDS atomics.
+constexpr int kOnes32Scale = 19;
+
+template <int W, bool CPLX, bool U32 = false>
 __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     typename Cell<CPLX>::T* __restrict__ grid, int n, const EsKernel esk, float hi, float lo, int bs, int nbx,
     const float4* __restrict__ sorted, const typename Cell<CPLX>::T* __restrict__ c,
     const unsigned* __restrict__ cmax_bits,
     const int4* __restrict__ chunks, int* __restrict__ ctrl, int bound, unsigned long long* __restrict__ fixed,
     int shift) {
+    static_assert(!U32 || !CPLX, "32-bit accumulators for real unit strengths");
     constexpr int NC = CPLX ? 2 : 1;  // integer accumulators per cell
-    extern __shared__ unsigned long long acc[];
+    using AccT = typename std::conditional<U32, unsigned, unsigned long long>::type;
+    extern __shared__ unsigned long long acc_raw[];
+    AccT* acc = reinterpret_cast<AccT*>(acc_raw);
     // LDS layout x * SSP + y * SP + z (spread_pitch)
     const int S = bs + W - 1, S3 = S * S * S;
     int SP, SSP;
@@ -740,7 +751,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         const int4 ch = chunks[cid];
         const int b = ch.x;
         const int ox = (b / (nbx * nbx)) * bs, oy = ((b / nbx) % nbx) * bs, oz = (b % nbx) * bs;
-        for (int i = threadIdx.x; i < NC * L3; i += kSpreadThreads) acc[i] = 0ull;
+        for (int i = threadIdx.x; i < NC * L3; i += kSpreadThreads) acc[i] = (AccT)0;
         if (threadIdx.x < kSpreadClasses) {
             cls_count[threadIdx.x] = 0;
             cls_fill[threadIdx.x] = 0;
@@ -811,20 +822,31 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                 vr = ldexpf(cv.x, sc);
                 vi = ldexpf(cv.y, sc);
             } else {
-                vr = ldexpf(bound ? q.w : (c ? reinterpret_cast<const float*>(c)[src] : 1.0f), sc);
+                vr = U32 ? (float)(1 << kOnes32Scale)
+                         : ldexpf(bound ? q.w : (c ? reinterpret_cast<const float*>(c)[src] : 1.0f), sc);
             }
-            unsigned long long* base = acc + NC * (lx * SSP + ly * SP + lz);
-            if (nonneg) {
+            AccT* base = acc + NC * (lx * SSP + ly * SP + lz);
+            if (U32) {
+#pragma unroll
+                for (int a = 0; a < W; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < W; ++bb) {
+                        AccT* row = base + (a * SSP + bb * SP);
+                        const float pr = (wx[a] * wy[bb]) * vr;
+#pragma unroll
+                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (unsigned)rintf(wz[cc] * pr));
+                    }
+            } else if (nonneg) {
                 // real non-negative strengths (image counts, unit strengths): every
                 // contribution is >= 0, so the cheaper unsigned conversion applies
 #pragma unroll
                 for (int a = 0; a < W; ++a)
 #pragma unroll
                     for (int bb = 0; bb < W; ++bb) {
-                        unsigned long long* row = base + (a * SSP + bb * SP);
+                        AccT* row = base + (a * SSP + bb * SP);
                         const float pr = (wx[a] * wy[bb]) * vr;
 #pragma unroll
-                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, fixed_nonneg(wz[cc] * pr));
+                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (AccT)fixed_nonneg(wz[cc] * pr));
                     }
             } else {
 #pragma unroll
@@ -832,13 +854,13 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
 #pragma unroll
                     for (int bb = 0; bb < W; ++bb) {
                         const float wab = wx[a] * wy[bb];
-                        unsigned long long* row = base + NC * (a * SSP + bb * SP);
+                        AccT* row = base + NC * (a * SSP + bb * SP);
                         const float pr = wab * vr;
                         const float pi = wab * vi;
 #pragma unroll
                         for (int cc = 0; cc < W; ++cc) {
-                            atomicAdd(row + NC * cc, (unsigned long long)__float2ll_rn(wz[cc] * pr));
-                            if (CPLX) atomicAdd(row + NC * cc + 1, (unsigned long long)__float2ll_rn(wz[cc] * pi));
+                            atomicAdd(row + NC * cc, (AccT)__float2ll_rn(wz[cc] * pr));
+                            if (CPLX) atomicAdd(row + NC * cc + 1, (AccT)__float2ll_rn(wz[cc] * pi));
                         }
                     }
             }
@@ -853,7 +875,9 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             gz = gz >= n ? gz - n : gz;
             const uint32_t off = (uint32_t)((gx * n + gy) * n + gz);
             if (!CPLX && fixed) {
-                const unsigned long long v = acc[li];
+                // U32: the 2^kOnes32Scale sums moved to the grid's scale 2^sc (sc >= kOnes32Scale)
+                const unsigned long long v = U32 ? ((unsigned long long)acc[li] << (sc - kOnes32Scale))
+                                                 : (unsigned long long)acc[li];
                 if (v != 0ull) atomicAdd(fixed + off, v);
                 continue;
             }
@@ -1710,14 +1734,20 @@ struct SpreadLaunch {
             hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)std::min<int64_t>(4096, (cnt + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, st, reinterpret_cast<const float*>(c), cnt, cmax);
         }
-        const size_t lds = subgrid_lds(p->bs, W, true, cplx);
+        // unit strengths: 32-bit LDS accumulators (half the subgrid LDS); their flush shifts
+        // left by the grid's scale minus kOnes32Scale, which is >= 0 while npts < 2^43
+        const bool ones32 = !cplx && !c && !bound && p->npts < (int64_t(1) << 43);
+        const size_t lds = subgrid_lds(p->bs, W, true, cplx) / (ones32 ? 2 : 1);
         const int fit = (int)std::max<size_t>(1, std::min<size_t>(4, kMaxLds / (lds + kSpreadStaticLds)));
         const unsigned grid = (unsigned)(cu_count() * (p->wpc_req > 0 ? p->wpc_req : fit));
         const int4* ch = p->chunks.as<const int4>();
-        static bool attr_set[2] = {false, false};
-        if (!attr_set[cplx] && lds > kDefaultLds) {
-            allow_large_lds(cplx ? (const void*)spread_binned_kernel<W, true> : (const void*)spread_binned_kernel<W, false>);
-            attr_set[cplx] = true;
+        static bool attr_set[3] = {false, false, false};
+        const int form = cplx ? 1 : (ones32 ? 2 : 0);
+        if (!attr_set[form] && lds > kDefaultLds) {
+            allow_large_lds(form == 1 ? (const void*)spread_binned_kernel<W, true>
+                                      : (form == 2 ? (const void*)spread_binned_kernel<W, false, true>
+                                                   : (const void*)spread_binned_kernel<W, false>));
+            attr_set[form] = true;
         }
         if (cplx) {
             hipLaunchKernelGGL((spread_binned_kernel<W, true>), dim3(grid), dim3(kSpreadThreads), lds, st,
@@ -1734,10 +1764,16 @@ struct SpreadLaunch {
         while (bits < 62 && (int64_t(1) << bits) <= p->npts) ++bits;  // npts < 2^bits
         const int shift = std::max(0, bits - 14);
         const int64_t n3 = (int64_t)p->n * p->n * p->n;
-        hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
-                           p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
-                           p->sorted.as<const float4>(), reinterpret_cast<const float*>(c), cmax, ch,
-                           p->ctrl.as<int>(), bound ? 1 : 0, p->fine64.as<unsigned long long>(), shift);
+        if (ones32)
+            hipLaunchKernelGGL((spread_binned_kernel<W, false, true>), dim3(grid), dim3(kSpreadThreads), lds, st,
+                               p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->sorted.as<const float4>(), (const float*)nullptr, cmax, ch, p->ctrl.as<int>(), 0,
+                               p->fine64.as<unsigned long long>(), shift);
+        else
+            hipLaunchKernelGGL((spread_binned_kernel<W, false>), dim3(grid), dim3(kSpreadThreads), lds, st,
+                               p->fine.as<float>(), p->n, p->esk, p->tc_hi, p->tc_lo, p->bs, p->nbx,
+                               p->sorted.as<const float4>(), reinterpret_cast<const float*>(c), cmax, ch,
+                               p->ctrl.as<int>(), bound ? 1 : 0, p->fine64.as<unsigned long long>(), shift);
         hipLaunchKernelGGL(fixed_to_float_kernel, dim3((unsigned)std::min<int64_t>(65536, (n3 + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, st, p->fine64.as<const unsigned long long>(), n3,
                            (const unsigned*)cmax, (c || bound) ? 1 : 0, shift, p->fine.as<float>());

@@ -332,5 +332,10 @@ def test_setpts_strengths(M, N, signed):
     ones = torch.empty_like(want)
     p.execute(None, ones, CMTIP_GRID_REAL | CMTIP_PTS_ONES, None, 1.0)
     assert torch.equal(ones_b, ones)
+    # unit strengths spread with 32-bit accumulators (weight products rounded to 2^-19):
+    # the 64-bit spread of explicit ones agrees to that rounding
+    real1 = torch.empty_like(want)
+    p.execute(torch.ones(N, device=d), real1, flags, None, 1.0)
+    assert rel_l2(ones, real1) < 2e-6
     with pytest.raises(CmtipError):
         NufftPlan(2, M, -1).setpts_strengths(x, y, z, c)
