@@ -52,7 +52,6 @@ SIGNATURES = {
     "cmtip_slice_prepare": [_vp, _vp, _vp, _vp],
     "cmtip_slice_real": [_vp, _vp, _i64, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "cmtip_weighted_norms": [_vp, _i64, _i64, _vp, _vp, _vp],
-    "cmtip_moments": [_vp, _i64, _vp, _vp],
     "cmtip_match_prepare_images": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_image_stats": [_vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "cmtip_image_stats_noalloc": [_vp, _i64, _i64, _vp, _vp, _vp],

@@ -95,24 +95,6 @@ __global__ void weighted_norms_kernel(const float* __restrict__ X, int64_t P, co
     }
 }
 
-__global__ void moments_kernel(const float* __restrict__ X, int64_t count, double* __restrict__ out2) {
-    double s1 = 0.0, s2 = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * kThreads;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) {
-        const double v = (double)X[i];
-        s1 += v;
-        s2 += v * v;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        s1 += __shfl_down(s1, off, 64);
-        s2 += __shfl_down(s2, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        unsafeAtomicAdd(out2, s1);
-        unsafeAtomicAdd(out2 + 1, s2);
-    }
-}
-
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
     uint32_t u = __float_as_uint(f);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0;
@@ -1346,18 +1328,6 @@ extern "C" int cmtip_weighted_norms(const float* X, int64_t rows, int64_t P, con
     }
     hipLaunchKernelGGL(weighted_norms_kernel, dim3((unsigned)rows), dim3(kThreads), 0, as_stream(stream), X, P,
                        w2, out);
-    CMTIP_LAUNCH_CHECK();
-    return CMTIP_OK;
-}
-
-extern "C" int cmtip_moments(const float* X, int64_t count, double* out2, void* stream) {
-    if (count < 0 || !out2 || (count > 0 && !X)) {
-        set_error("moments: bad arguments");
-        return CMTIP_ERR_ARG;
-    }
-    if (count == 0) return CMTIP_OK;
-    hipLaunchKernelGGL(moments_kernel, dim3(grid1(count, 4096)), dim3(kThreads), 0, as_stream(stream), X, count,
-                       out2);
     CMTIP_LAUNCH_CHECK();
     return CMTIP_OK;
 }

@@ -642,14 +642,16 @@ __device__ __forceinline__ void weights_at(float t, const WideRow<W, READS>& wr,
 // Spreading accumulates in LDS in 64-bit fixed point: integer LDS atomics are an
 // order of magnitude faster than float ones on this hardware, and integer sums are
 // exact and independent of the order of the adds.  Every contribution (an FP32
-// product) is scaled by 2^sc and rounded to an integer; sc is chosen from max|c| so
-// that a cell's sum over a chunk of at most kChunk points (weights <= 1) stays below
-// 2^62.  The chunk's cells are then added to the fine grid as floats.
+// product) is scaled by 2^sc and rounded to a 32-bit integer (one rounding and one
+// conversion instruction, plus a sign extension for signed strengths); sc is chosen from
+// max|c| so that every contribution lies below 2^30 in magnitude.  A chunk of at most
+// kChunk = 2^12 points then sums to below 2^42 in any cell, and the 64-bit fine grid of a
+// real spread holds 2^33 points' contributions (beyond that the caller lowers sc).
 __device__ __forceinline__ int spread_scale_exp(float cmax) {
     if (!(cmax > 0.0f)) return 0;
     int e;
     frexpf(cmax, &e);  // cmax < 2^e
-    return 62 - 13 - e;  // kChunk = 2^12 points, one bit of margin
+    return 30 - e;
 }
 
 // out_bits[0] = bits of max |c| (non-negative floats order like their bits; NaNs are
@@ -694,20 +696,19 @@ __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsign
     block_max_or(m, neg ? 1u : 0u, out_bits, out_bits + 1);
 }
 
-// Fixed-point value of a non-negative float x < 2^63, rounded to nearest: the two 32-bit
-// halves from hi = floor(x / 2^32) and lo = x - hi 2^32, both exact (x below 2^32 is lo
-// itself; above, x is a multiple of 2^9 and its low half fits 24 bits).  Six
-// instructions where the general signed conversion takes eleven.
-__device__ __forceinline__ unsigned long long fixed_nonneg(float x) {
-    x = rintf(x);
-    const float hi = floorf(x * 2.3283064365386963e-10f);
-    const float lo = fmaf(hi, -4294967296.0f, x);
-    return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
+// Contribution x (|x| < 2^30) of a spread as a 64-bit fixed-point addend: the nearest
+// integer, converted exactly in 32 bits and sign-extended (zero-extended for
+// non-negative strengths).
+template <bool NONNEG>
+__device__ __forceinline__ unsigned long long fixed_addend(float x) {
+    if (NONNEG) return (unsigned long long)(uint32_t)rintf(x);
+    return (unsigned long long)(long long)(int)rintf(x);
 }
 
 // U32 (unit strengths only): 32-bit LDS accumulators at the scale 2^kOnes32Scale per
 // contribution (every weight product is <= 1, so a chunk's kChunk = 2^12 contributions to
-// a cell stay below 2^31), added to the 64-bit fine grid shifted to its scale: exact, so
+// a cell stay below 2^31), added to the 64-bit fine grid shifted to its scale (which is
+// at least 2^kOnes32Scale while npts < 2^43): exact, so
 // the result is the same as the 64-bit path's up to the rounding of each weight product
 // to 2^-19 (at most 1e-6 per tap), and deterministic.  Half the LDS per subgrid and
 // twice the atomic rate of 64-bit LDS atomics.
@@ -846,7 +847,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                         AccT* row = base + (a * SSP + bb * SP);
                         const float pr = (wx[a] * wy[bb]) * vr;
 #pragma unroll
-                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (AccT)fixed_nonneg(wz[cc] * pr));
+                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (AccT)fixed_addend<true>(wz[cc] * pr));
                     }
             } else {
 #pragma unroll
@@ -859,8 +860,8 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                         const float pi = wab * vi;
 #pragma unroll
                         for (int cc = 0; cc < W; ++cc) {
-                            atomicAdd(row + NC * cc, (AccT)__float2ll_rn(wz[cc] * pr));
-                            if (CPLX) atomicAdd(row + NC * cc + 1, (AccT)__float2ll_rn(wz[cc] * pi));
+                            atomicAdd(row + NC * cc, (AccT)fixed_addend<false>(wz[cc] * pr));
+                            if (CPLX) atomicAdd(row + NC * cc + 1, (AccT)fixed_addend<false>(wz[cc] * pi));
                         }
                     }
             }
@@ -1759,10 +1760,11 @@ struct SpreadLaunch {
         }
         // real: the chunks' integer sums go into the zeroed 64-bit grid (exact, so the result
         // does not depend on the order of the adds), scaled so that npts full-weight
-        // contributions stay below 2^63, then one pass converts it into the float fine grid
+        // contributions (each below 2^(30 - shift)) stay below 2^63, then one pass converts
+        // it into the float fine grid
         int bits = 0;
         while (bits < 62 && (int64_t(1) << bits) <= p->npts) ++bits;  // npts < 2^bits
-        const int shift = std::max(0, bits - 14);
+        const int shift = std::max(0, bits - 33);
         const int64_t n3 = (int64_t)p->n * p->n * p->n;
         if (ones32)
             hipLaunchKernelGGL((spread_binned_kernel<W, false, true>), dim3(grid), dim3(kSpreadThreads), lds, st,

@@ -252,7 +252,6 @@ int cmtip_match_prepare_images_norms(const float* B, int64_t n_img, int64_t P, c
                                      const float* c, float* Bw, uint16_t* Bw_bf16, float* norms, void* stream);
 int cmtip_weighted_norms(const float* X, int64_t rows, int64_t P, const float* w2, float* out,
                          void* stream);
-int cmtip_moments(const float* X, int64_t count, double* out2, void* stream);
 int cmtip_match_prepare_images(const float* B, int64_t n_img, int64_t P, const float* w2,
                                float* Bw, uint16_t* Bw_bf16, void* stream);
 int cmtip_fill_u64(unsigned long long* dst, int64_t count, unsigned long long value, void* stream);
