@@ -39,6 +39,7 @@ SIGNATURES = {
     "cmtip_nufft_info": [_vp, _pi, _pi, _pd],
     "cmtip_nufft_setpts": [_vp, _i64, _vp, _vp, _vp, _vp],
     "cmtip_nufft_setpts_strengths": [_vp, _i64, _vp, _vp, _vp, _vp, _vp],
+    "cmtip_nufft_setpts_rotated": [_vp, _vp, _i64, _vp, _i64, _f, _vp, _vp],
     "cmtip_nufft_points_max_abs": [_vp, ctypes.POINTER(ctypes.c_float), _vp],
     "cmtip_nufft_execute": [_vp, _vp, _vp, _i, _vp, _f, _vp],
     "cmtip_nufft_destroy": [_vp],

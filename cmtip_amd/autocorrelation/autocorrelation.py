@@ -180,16 +180,21 @@ class WOperator:
 
 
 def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_extent, alambda, rlambda, flambda,
-                 use_gpu=False, use_recip_sym=True, *, L_embed=None, eps=DEFAULT_EPS, scaled=False, autotune=False):
+                 use_gpu=False, use_recip_sym=True, *, L_embed=None, eps=DEFAULT_EPS, scaled=False, autotune=False,
+                 rot=None):
     """(W, d) of W x = d (autocorrelation.py:113-173); d is a float32 device vector.
 
     With ``scaled`` the coordinates are already x = H / reciprocal_extent * pi.  With
     ``autotune`` the two spreads pick their bin and launch settings with the NUFFT
     autotuner (first call per shape; later calls reuse the choice).  ``weights`` None
-    means unit weights."""
+    means unit weights.  ``rot`` (quats, s, reciprocal_extent) as device tensors: the
+    points are the rotated pixels gen_nonuniform_positions would return, binned straight
+    from the rotations (H, K, L are then not used and may be None; no autotune)."""
     if not use_recip_sym:
         raise NotImplementedError("the GPU solver keeps real vectors (use_recip_sym=True)")
-    if scaled:
+    if rot is not None:
+        H_ = K_ = L_ = None
+    elif scaled:
         H_, K_, L_ = (_arrays.to_dev(t, torch.float32).reshape(-1) for t in (H, K, L))
     else:
         ext = np.float32(reciprocal_extent)
@@ -202,13 +207,13 @@ def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_
     solver.set_antisupport(torch.as_tensor(gen_F_antisupport(M), dtype=torch.float32))
     # kernel: real type-1 of ones at M_ups modes over (2M)^3 (:141-148; 1/(2M)^3 also when L > 2M)
     ugrid_conv = _adjoint(None, H_, K_, L_, Lm, True, None, eps, scale=1.0 / float(2 * M) ** 3, ones=True,
-                          autotune=autotune)
+                          autotune=autotune, rot=rot)
     solver.set_kernel(ugrid_conv)
     # rhs: d = alambda * A^H (data * weights) + rlambda * x0 (:162-171)
     nuvect_Db = _arrays.to_dev(data, torch.float32).reshape(-1)
     if weights is not None:  # None: unit weights, the product is the data itself
         nuvect_Db = nuvect_Db * _arrays.to_dev(weights, torch.float32).reshape(-1)
-    uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps, autotune=autotune).reshape(-1)
+    uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps, autotune=autotune, rot=rot).reshape(-1)
     if bool(torch.isnan(uvect_ADb).any()):
         print("Warning: nans in the adjoint calculation; intensities may be too large", flush=True)
     x0d = _arrays.to_dev(x0, torch.float32).reshape(-1)
@@ -238,11 +243,18 @@ def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, 
         orientations = synth.random_quaternions(N_images, seed if seed is not None else np.random.randint(2 ** 31))
     ext = np.float32(reciprocal_extent.item() if isinstance(reciprocal_extent, torch.Tensor) else reciprocal_extent)
     dev = _lib.device()
-    # device tensors in, so the rotated coordinates (12 B per point) stay in device memory
-    H_, K_, L_ = gen_nonuniform_positions(
-        _arrays.to_dev(orientations, torch.float32),
-        _arrays.to_dev(pixel_position_reciprocal, torch.float64), recip_extent=ext)
-    H_, K_, L_ = (t.reshape(-1) for t in (H_, K_, L_))
+    q_d = _arrays.to_dev(orientations, torch.float32).reshape(-1, 4)
+    s_d = _arrays.to_dev(pixel_position_reciprocal, torch.float64).reshape(3, -1).contiguous()
+    rot = None
+    if autotune:
+        # the autotuner bins the points again per setting: coordinate arrays (12 B per point)
+        H_, K_, L_ = gen_nonuniform_positions(q_d, s_d, recip_extent=ext)
+        H_, K_, L_ = (t.reshape(-1) for t in (H_, K_, L_))
+    else:
+        # the spreads bin the rotated pixels straight from (orientations, pixel positions):
+        # the coordinates gen_nonuniform_positions would write, never stored
+        H_ = K_ = L_ = None
+        rot = (q_d, s_d, float(ext))
     data = _arrays.to_dev(slices_, torch.float32).reshape(-1)
     if ac_estimate is None:
         ac_support = torch.ones((M,) * 3, dtype=torch.float32, device=dev)
@@ -262,7 +274,8 @@ def solve_ac(generation, pixel_position_reciprocal, reciprocal_extent, slices_, 
     if rlambda is None:
         rlambda = Mtot / N / 1000
     W, d = setup_linops(H_, K_, L_, data, ac_support, weights, x0, M, Mtot, N, ext, alambda, rlambda, flambda,
-                        use_gpu=True, use_recip_sym=True, L_embed=L, eps=eps, scaled=True, autotune=autotune)
+                        use_gpu=True, use_recip_sym=True, L_embed=L, eps=eps, scaled=True, autotune=autotune,
+                        rot=rot)
     x = x0.clone()
     iters, info = W.solver.cg(d, x, alambda, rlambda, flambda, maxiter, rtol)
     ac = x.reshape((M,) * 3)

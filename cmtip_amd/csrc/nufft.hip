@@ -62,7 +62,7 @@ struct cmtip_nufft_plan_s {
     int timing = 0, timed = 0;
     hipEvent_t tev[2 * 64] = {};
     DevBuf amax;      // 1 uint32: bits of max |coordinate| of the registered points (setpts)
-    int amax_valid = 0;  // 1: amax describes the registered point arrays; 2: none registered (0)
+    int amax_valid = 0;  // 1: amax describes the points with records; 2: none registered (0)
     DevBuf bin_of;    // N int32
     DevBuf count;     // nbins int32 (also the scatter cursors)
     DevBuf offset;    // nbins + 1 int32
@@ -73,6 +73,7 @@ struct cmtip_nufft_plan_s {
     DevBuf ctrl;      // int32: [0] number of chunks, [1] work counter, [2] bits of max |c|, [3] some c < 0
     int64_t max_chunks = 0;
     int bound = 0;    // 1: the records carry real strengths (setpts_strengths), not indices
+    int rot_records = 0;  // 1: the records hold rotated pixels (setpts_rotated), no arrays registered
 };
 
 namespace {
@@ -388,12 +389,12 @@ __global__ void __launch_bounds__(1024) bin_scan_kernel(int* __restrict__ count,
     }
 }
 
-// COPY (registered points): write one 16-byte record (x, y, z, original index) per
-// sorted position, one scattered store per point; else (the slicing route, which
-// regenerates the coordinates) write the original index to perm.
-template <bool COPY>
-__global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                   const float* __restrict__ z, int64_t N, const int* __restrict__ bin_of,
+// COPY: write one 16-byte record (x, y, z, original index) per sorted position, one
+// scattered store per point, the coordinates from the point source (registered arrays,
+// or rotated pixels generated again); else (the slicing route, which regenerates the
+// coordinates when it interpolates) write the original index to perm.
+template <bool COPY, class Src>
+__global__ void bin_scatter_kernel(Src pts, int64_t N, const int* __restrict__ bin_of,
                                    const int* __restrict__ offset, int* __restrict__ cursor, int* __restrict__ perm,
                                    float4* __restrict__ sorted, const float* __restrict__ cstr) {
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -402,10 +403,13 @@ __global__ void bin_scatter_kernel(const float* __restrict__ x, const float* __r
     const int slot = wave_bin_add(cursor, b, valid);
     if (!valid) return;
     const int pos = offset[b] + slot;
-    if (COPY)
-        sorted[pos] = make_float4(x[j], y[j], z[j], cstr ? cstr[j] : __int_as_float((int)j));
-    else
+    if (COPY) {
+        float x, y, z;
+        pts.at(j, &x, &y, &z);
+        sorted[pos] = make_float4(x, y, z, cstr ? cstr[j] : __int_as_float((int)j));
+    } else {
         perm[pos] = (int)j;
+    }
 }
 
 // Workgroup-privatised binning (when the bin counters fit LDS): a workgroup takes a
@@ -457,11 +461,10 @@ __global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int
 }
 
 // COPY: one 16-byte record per point, (x, y, z, original index as int bits), or with
-// cstr (x, y, z, strength)
-template <bool COPY>
+// cstr (x, y, z, strength); the coordinates from the point source
+template <bool COPY, class Src>
 __global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
-    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, int64_t N,
-    int64_t per_block, int nbins, const int* __restrict__ bin_of, const int* __restrict__ offset,
+    Src pts, int64_t N, int64_t per_block, int nbins, const int* __restrict__ bin_of, const int* __restrict__ offset,
     int* __restrict__ cursor, int* __restrict__ perm, float4* __restrict__ sorted, const float* __restrict__ cstr) {
     extern __shared__ int hist[];
     for (int i = threadIdx.x; i < nbins; i += kBinBlock) hist[i] = 0;
@@ -477,10 +480,13 @@ __global__ void __launch_bounds__(kBinBlock) bin_scatter_local_kernel(
     __syncthreads();
     for (int64_t j = j0 + threadIdx.x; j < j1; j += kBinBlock) {
         const int pos = atomicAdd(hist + bin_of[j], 1);
-        if (COPY)
-            sorted[pos] = make_float4(x[j], y[j], z[j], cstr ? cstr[j] : __int_as_float((int)j));
-        else
+        if (COPY) {
+            float x, y, z;
+            pts.at(j, &x, &y, &z);
+            sorted[pos] = make_float4(x, y, z, cstr ? cstr[j] : __int_as_float((int)j));
+        } else {
             perm[pos] = (int)j;
+        }
     }
 }
 
@@ -1784,13 +1790,16 @@ struct SpreadLaunch {
     }
 };
 
-// Sort the registered points (or, with rot, the slicing route's rotated pixels) into
-// spatial bins (counting sort) and build the chunk list.
-// cstr (optional, registered points only): real strengths stored in the records in place of
-// the original indices, with their max |c| and sign recorded in ctrl[2], ctrl[3]
+// Sort the registered points (or, with rot, rotated pixels) into spatial bins (counting
+// sort) and build the chunk list.  Registered points and rotated pixels with
+// rot_records get 16-byte records (x, y, z, original index or strength); the slicing
+// route's rotated pixels (rot without rot_records) get only the permutation.
+// cstr (optional): real strengths stored in the records in place of the original
+// indices, with their max |c| and sign recorded in ctrl[2], ctrl[3]
 int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nullptr,
-               const float* cstr = nullptr) {
+               const float* cstr = nullptr, bool rot_records = false) {
     const int64_t N = p->npts;
+    const bool records = !rot || rot_records;
     p->bs = p->bs_req > 0 ? p->bs_req : bin_size_for(p->w);
     p->chunk = p->chunk_req > 0 ? p->chunk_req : kChunk;
     p->nbx = (p->n + p->bs - 1) / p->bs;
@@ -1800,10 +1809,10 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     CMTIP_TRY(p->count.ensure(sizeof(int) * (size_t)p->nbins));
     CMTIP_TRY(p->offset.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
     CMTIP_TRY(p->nchunk.ensure(sizeof(int) * (size_t)(p->nbins + 1)));
-    if (rot)
-        CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)N));
-    else
+    if (records)
         CMTIP_TRY(p->sorted.ensure(sizeof(float4) * (size_t)N));
+    else
+        CMTIP_TRY(p->perm.ensure(sizeof(int) * (size_t)N));
     CMTIP_TRY(p->chunks.ensure(sizeof(int4) * (size_t)p->max_chunks));
     CMTIP_TRY(p->ctrl.ensure(sizeof(int) * 4));
     const unsigned gb = (unsigned)((p->nbins + kBlock - 1) / kBlock);
@@ -1812,15 +1821,16 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     unsigned* cmax = reinterpret_cast<unsigned*>(p->ctrl.as<int>() + 2);
     if (cstr) hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, p->ctrl.as<int>() + 2, (int64_t)2, 0);
     p->bound = cstr ? 1 : 0;
-    // registered point arrays: their max |coordinate| for the caller's range check
+    // points with records: their max |coordinate| for the caller's range check
     unsigned* amax = nullptr;
     p->amax_valid = 0;
-    if (!rot) {
+    if (records) {
         CMTIP_TRY(p->amax.ensure(sizeof(unsigned)));
         amax = p->amax.as<unsigned>();
         hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<int*>(amax), (int64_t)1, 0);
         p->amax_valid = 1;
     }
+    const PtsArrays arr{p->x, p->y, p->z};
     if (p->nbins <= kLocalBinsMax) {
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((N + 8191) / 8192, 4 * cu_count()));
         const int64_t per_block = (N + blocks - 1) / blocks;
@@ -1830,29 +1840,32 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
         if (!attr_set) {
             allow_large_lds((const void*)bin_count_local_kernel<PtsRotated>);
             allow_large_lds((const void*)bin_count_local_kernel<PtsArrays>);
-            allow_large_lds((const void*)bin_scatter_local_kernel<false>);
-            allow_large_lds((const void*)bin_scatter_local_kernel<true>);
+            allow_large_lds((const void*)bin_scatter_local_kernel<false, PtsArrays>);
+            allow_large_lds((const void*)bin_scatter_local_kernel<true, PtsArrays>);
+            allow_large_lds((const void*)bin_scatter_local_kernel<true, PtsRotated>);
             attr_set = true;
         }
         if (rot)
             hipLaunchKernelGGL(bin_count_local_kernel<PtsRotated>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
                                *rot, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi, p->tc_lo,
-                               p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr, (const float*)nullptr,
-                               (unsigned*)nullptr);
+                               p->bin_of.as<int>(), p->count.as<int>(), amax, cstr, cmax);
         else
             hipLaunchKernelGGL(bin_count_local_kernel<PtsArrays>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
-                               PtsArrays{p->x, p->y, p->z}, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi,
-                               p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), amax, cstr, cmax);
+                               arr, N, per_block, p->n, p->w, p->bs, p->nbx, nb, p->tc_hi, p->tc_lo,
+                               p->bin_of.as<int>(), p->count.as<int>(), amax, cstr, cmax);
         hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                            p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
-        if (rot)
-            hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st,
-                               (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, N, per_block, nb,
-                               p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                               p->perm.as<int>(), (float4*)nullptr, (const float*)nullptr);
+        if (!records)
+            hipLaunchKernelGGL((bin_scatter_local_kernel<false, PtsArrays>), dim3((unsigned)blocks), dim3(kBinBlock),
+                               lds, st, arr, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
+                               p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr, (const float*)nullptr);
+        else if (rot)
+            hipLaunchKernelGGL((bin_scatter_local_kernel<true, PtsRotated>), dim3((unsigned)blocks), dim3(kBinBlock),
+                               lds, st, *rot, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
+                               p->count.as<int>(), (int*)nullptr, p->sorted.as<float4>(), cstr);
         else
-            hipLaunchKernelGGL(bin_scatter_local_kernel<true>, dim3((unsigned)blocks), dim3(kBinBlock), lds, st, p->x,
-                               p->y, p->z, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
+            hipLaunchKernelGGL((bin_scatter_local_kernel<true, PtsArrays>), dim3((unsigned)blocks), dim3(kBinBlock),
+                               lds, st, arr, N, per_block, nb, p->bin_of.as<const int>(), p->offset.as<const int>(),
                                p->count.as<int>(), (int*)nullptr, p->sorted.as<float4>(), cstr);
         hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                            p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
@@ -1861,23 +1874,24 @@ int bin_points(cmtip_nufft_plan_s* p, hipStream_t st, const PtsRotated* rot = nu
     }
     if (rot)
         hipLaunchKernelGGL(bin_count_kernel<PtsRotated>, dim3(gp), dim3(kBlock), 0, st, *rot, N, p->n, p->w, p->bs,
-                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), (unsigned*)nullptr,
-                           (const float*)nullptr, (unsigned*)nullptr);
+                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), amax, cstr, cmax);
     else
-        hipLaunchKernelGGL(bin_count_kernel<PtsArrays>, dim3(gp), dim3(kBlock), 0, st, PtsArrays{p->x, p->y, p->z}, N,
-                           p->n, p->w, p->bs, p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(),
-                           amax, cstr, cmax);
+        hipLaunchKernelGGL(bin_count_kernel<PtsArrays>, dim3(gp), dim3(kBlock), 0, st, arr, N, p->n, p->w, p->bs,
+                           p->nbx, p->tc_hi, p->tc_lo, p->bin_of.as<int>(), p->count.as<int>(), amax, cstr, cmax);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), p->nbins, p->chunk,
                        p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
-    if (rot)
-        hipLaunchKernelGGL(bin_scatter_kernel<false>, dim3(gp), dim3(kBlock), 0, st, (const float*)nullptr,
-                           (const float*)nullptr, (const float*)nullptr, N, p->bin_of.as<const int>(),
-                           p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr,
-                           (const float*)nullptr);
+    if (!records)
+        hipLaunchKernelGGL((bin_scatter_kernel<false, PtsArrays>), dim3(gp), dim3(kBlock), 0, st, arr, N,
+                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(), p->perm.as<int>(),
+                           (float4*)nullptr, (const float*)nullptr);
+    else if (rot)
+        hipLaunchKernelGGL((bin_scatter_kernel<true, PtsRotated>), dim3(gp), dim3(kBlock), 0, st, *rot, N,
+                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(), (int*)nullptr,
+                           p->sorted.as<float4>(), cstr);
     else
-        hipLaunchKernelGGL(bin_scatter_kernel<true>, dim3(gp), dim3(kBlock), 0, st, p->x, p->y, p->z, N,
-                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
-                           (int*)nullptr, p->sorted.as<float4>(), cstr);
+        hipLaunchKernelGGL((bin_scatter_kernel<true, PtsArrays>), dim3(gp), dim3(kBlock), 0, st, arr, N,
+                           p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(), (int*)nullptr,
+                           p->sorted.as<float4>(), cstr);
     hipLaunchKernelGGL(chunk_fill_kernel, dim3(gb), dim3(kBlock), 0, st, p->offset.as<const int>(),
                        p->nchunk.as<const int>(), p->nbins, p->chunk, p->chunks.as<int4>());
     CMTIP_LAUNCH_CHECK();
@@ -1910,7 +1924,7 @@ struct TileLaunch {
             static bool attr_set = false;
             if (!attr_set) {
                 allow_large_lds((const void*)tile_key_local_kernel<W>);
-                allow_large_lds((const void*)bin_scatter_local_kernel<false>);
+                allow_large_lds((const void*)bin_scatter_local_kernel<false, PtsArrays>);
                 attr_set = true;
             }
             hipLaunchKernelGGL(tile_key_local_kernel<W>, dim3((unsigned)blocks), dim3(kBinBlock), hlds, st, rot,
@@ -1923,10 +1937,10 @@ struct TileLaunch {
         hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, p->count.as<int>(), nbins + 1, kTileChunk,
                            p->offset.as<int>(), p->nchunk.as<int>(), p->ctrl.as<int>());
         if (local)
-            hipLaunchKernelGGL(bin_scatter_local_kernel<false>, dim3((unsigned)blocks), dim3(kBinBlock), hlds, st,
-                               (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, ntiles, per_block,
-                               (int)(nbins + 1), p->bin_of.as<const int>(), p->offset.as<const int>(),
-                               p->count.as<int>(), p->perm.as<int>(), (float4*)nullptr, (const float*)nullptr);
+            hipLaunchKernelGGL((bin_scatter_local_kernel<false, PtsArrays>), dim3((unsigned)blocks), dim3(kBinBlock),
+                               hlds, st, PtsArrays{nullptr, nullptr, nullptr}, ntiles, per_block, (int)(nbins + 1),
+                               p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
+                               p->perm.as<int>(), (float4*)nullptr, (const float*)nullptr);
         else
             hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, ntiles,
                                p->bin_of.as<const int>(), p->offset.as<const int>(), p->count.as<int>(),
@@ -2262,6 +2276,7 @@ int setpts_impl(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y
     p->y = y;
     p->z = z;
     p->bound = 0;
+    p->rot_records = 0;
     if (npts == 0) {
         p->amax_valid = 2;  // no points: the maximum is 0
         p->bound = bind ? 1 : 0;
@@ -2288,6 +2303,42 @@ extern "C" int cmtip_nufft_setpts_strengths(cmtip_nufft_plan p, int64_t npts, co
         return CMTIP_ERR_ARG;
     }
     return setpts_impl(p, npts, x, y, z, c, true, stream);
+}
+
+extern "C" int cmtip_nufft_setpts_rotated(cmtip_nufft_plan p, const float* quats, int64_t n_orient,
+                                          const double* s, int64_t P, float recip_extent, const float* c,
+                                          void* stream) {
+    if (!valid_plan(p)) return CMTIP_ERR_ARG;
+    if (n_orient < 0 || P < 0 || (n_orient > 0 && P > 0 && (!quats || !s))) {
+        set_error("setpts_rotated: bad arguments");
+        return CMTIP_ERR_ARG;
+    }
+    if (c && p->type != 1) {
+        set_error("setpts_rotated: strengths are bound to type-1 plans only");
+        return CMTIP_ERR_ARG;
+    }
+    if (n_orient > 0x7fffffffLL || P > 0x7fffffffLL || n_orient * P > 0x7fffffffLL) {
+        set_error("setpts_rotated: at most 2^31 - 1 points per plan");
+        return CMTIP_ERR_ARG;
+    }
+    const int64_t npts = n_orient * P;
+    p->x = p->y = p->z = nullptr;
+    p->npts = npts;
+    p->bound = 0;
+    p->rot_records = 1;
+    if (npts == 0) {
+        p->amax_valid = 2;
+        p->bound = c ? 1 : 0;
+        return CMTIP_OK;
+    }
+    hipStream_t st = as_stream(stream);
+    CMTIP_TRY(ensure_rinv(p, quats, n_orient, st));
+    // point j = o * P + pixel: the coordinates of cmtip_rotate_coords, generated where they
+    // are binned and copied into the records
+    const PtsRotated rot{p->rinv.as<const double>(), s, nullptr, P, recip_extent};
+    const int rc = bin_points(p, st, &rot, c, true);
+    if (rc != CMTIP_OK) p->rot_records = 0;
+    return rc;
 }
 
 extern "C" int cmtip_nufft_points_max_abs(cmtip_nufft_plan p, float* max_abs, void* stream) {
@@ -2351,7 +2402,7 @@ extern "C" int cmtip_nufft_execute(cmtip_nufft_plan p, void* c, void* f, int fla
         set_error("execute: null data pointer");
         return CMTIP_ERR_ARG;
     }
-    if (p->npts > 0 && !p->x) {
+    if (p->npts > 0 && !p->x && !p->rot_records) {
         set_error("execute: setpts not called");
         return CMTIP_ERR_STATE;
     }
@@ -2421,6 +2472,10 @@ extern "C" int cmtip_nufft_autotune(cmtip_nufft_plan p, void* c, void* f, int fl
     if (!cands || n_cand < 1 || reps < 1 || !(setpts_weight >= 0.0f)) {
         set_error("autotune: bad arguments");
         return CMTIP_ERR_ARG;
+    }
+    if (p->rot_records) {
+        set_error("autotune: the points of setpts_rotated are not kept as arrays to bin again; use setpts");
+        return CMTIP_ERR_STATE;
     }
     if (p->npts > 0 && !p->x) {
         set_error("autotune: setpts not called");
@@ -2593,6 +2648,7 @@ int slice_tiles(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_
     }
     const TileRot rot{p->rinv.as<const double>(), s, order, P, tpo, ext};
     p->x = p->y = p->z = nullptr;  // the bins no longer hold registered points
+    p->rot_records = 0;
     p->npts = std::max<int64_t>(p->npts, 1);
     return dispatch_w<TileLaunch>(p->w, p, rot, ntiles, out, out_bf16, part, timed, st);
 }
@@ -2610,6 +2666,7 @@ int slice_binned(cmtip_nufft_plan_s* p, const double* s, const int* order, int64
         return CMTIP_ERR_ARG;
     }
     p->x = p->y = p->z = nullptr;
+    p->rot_records = 0;
     for (int64_t o0 = 0; o0 < n_orient; o0 += per) {
         const int64_t no = std::min(per, n_orient - o0), N = no * P;
         const PtsRotated rot{p->rinv.as<const double>() + 9 * o0, s, order, P, ext};

@@ -10,11 +10,22 @@ from .._lib import CMTIP_GRID_REAL, CMTIP_PTS_BOUND, CMTIP_PTS_ONES, CMTIP_PTS_R
 from .plan import DEFAULT_EPS, get_plan, tuned_setpts
 
 
-def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False, autotune=False):
-    as_torch = _arrays.is_torch(nuvect, H_)
-    x = _arrays.to_dev(H_, torch.float32).reshape(-1)
-    y = _arrays.to_dev(K_, torch.float32).reshape(-1)
-    z = _arrays.to_dev(L_, torch.float32).reshape(-1)
+def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False, autotune=False,
+             rot=None):
+    """rot (quats, s, recip_extent), device tensors: the points are the rotated pixels of
+    gen_nonuniform_positions(quats, s, recip_extent=...), registered without coordinate
+    arrays (H_, K_, L_ unused; no autotune)."""
+    if rot is None:
+        as_torch = _arrays.is_torch(nuvect, H_)
+        x = _arrays.to_dev(H_, torch.float32).reshape(-1)
+        y = _arrays.to_dev(K_, torch.float32).reshape(-1)
+        z = _arrays.to_dev(L_, torch.float32).reshape(-1)
+        npts, dev = x.numel(), x.device
+    else:
+        assert not autotune, "autotuning bins registered coordinate arrays again"
+        as_torch = True
+        quats, s, ext = rot
+        npts, dev = quats.shape[0] * s.shape[1], quats.device
     flags = 0
     c = None
     if ones:
@@ -26,18 +37,23 @@ def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, one
         else:
             c = _arrays.to_dev(nuvect, torch.float32).reshape(-1)
             flags |= CMTIP_PTS_REAL
-        assert c.numel() == x.numel()
+        assert c.numel() == npts
     M = int(M)
     if use_recip_sym:
-        f = torch.empty((M, M, M), dtype=torch.float32, device=x.device)
+        f = torch.empty((M, M, M), dtype=torch.float32, device=dev)
         flags |= CMTIP_GRID_REAL
     else:
-        f = torch.empty((M, M, M), dtype=torch.complex64, device=x.device)
+        f = torch.empty((M, M, M), dtype=torch.complex64, device=dev)
     sup = None if support is None else _arrays.to_dev(support, torch.float32)
     plan = get_plan(1, M, +1, eps)
     scale = float(1.0 / M ** 3) if scale is None else float(scale)
     bind = not autotune and c is not None and (flags & CMTIP_PTS_REAL)
-    if autotune:
+    if rot is not None:
+        plan.setpts_rotated(quats, s, ext, c if bind else None)
+        if bind:
+            flags |= CMTIP_PTS_BOUND
+            c = None
+    elif autotune:
         tuned_setpts(plan, x, y, z, c, f, flags, sup, scale)
     elif bind:
         # real strengths ride in the bin-sorted point records: no gather of c by index

@@ -79,7 +79,20 @@ int cmtip_nufft_setpts(cmtip_nufft_plan plan, int64_t npts, const float* x, cons
 int cmtip_nufft_setpts_strengths(cmtip_nufft_plan plan, int64_t npts, const float* x, const float* y,
                                  const float* z, const float* c, void* stream);
 
-/* Largest |coordinate| of the point arrays registered by the last setpts (NaN counts as
+/* Register the n_orient * P rotated pixels of cmtip_rotate_coords (quats: n_orient x 4
+ * float, s: 3 x P float64, recip_extent as there; point j = o * P + p) without storing
+ * their coordinate arrays: the coordinates are generated where the points are binned
+ * and copied into the bin-sorted records, the same floats cmtip_rotate_coords writes.
+ * c (type-1 plans, optional, n_orient * P floats): strengths bound as by
+ * cmtip_nufft_setpts_strengths.  Replaces the gen_nonuniform_positions + set_pts pair of
+ * solve_ac (autocorrelation.py:208-209 then adjoint.py:74); the range check is available
+ * from cmtip_nufft_points_max_abs.  The plan cannot be autotuned until the next setpts
+ * (CMTIP_ERR_STATE), which needs arrays to bin again.  The quaternions and s are read
+ * while this call's kernels run on the stream. */
+int cmtip_nufft_setpts_rotated(cmtip_nufft_plan plan, const float* quats, int64_t n_orient, const double* s,
+                               int64_t P, float recip_extent, const float* c, void* stream);
+
+/* Largest |coordinate| of the points registered by the last setpts (NaN counts as
  * +inf), recorded while the points were binned: the range check of forward.py:27,64 and
  * adjoint.py:29,66 without another pass over the points.  Synchronises the stream.
  * CMTIP_ERR_STATE when the plan's points were replaced by a slicing route. */
@@ -93,7 +106,8 @@ int cmtip_nufft_points_max_abs(cmtip_nufft_plan plan, float* max_abs, void* stre
 #define CMTIP_PTS_IMAG 8    /* type 2 only: c (float) receives Im(T2(f)), e.g. the
                                coordinate gradient of a real slice */
 #define CMTIP_PTS_BOUND 32  /* type 1 only: the real strengths registered with the points by
-                               cmtip_nufft_setpts_strengths (c is ignored) */
+                               cmtip_nufft_setpts_strengths or cmtip_nufft_setpts_rotated
+                               (c is ignored) */
 #define CMTIP_EXEC_ADJOINT 16 /* run the adjoint of the plan's transform on its registered
                                points: a type-2 plan spreads (type 1, sign flipped), a
                                type-1 plan interpolates (type 2, sign flipped); c and f

@@ -339,3 +339,61 @@ def test_setpts_strengths(M, N, signed):
     assert rel_l2(ones, real1) < 2e-6
     with pytest.raises(CmtipError):
         NufftPlan(2, M, -1).setpts_strengths(x, y, z, c)
+
+
+@pytest.mark.parametrize("M,n_img,side", [(21, 40, 16), (81, 1000, 64)])
+def test_setpts_rotated(M, n_img, side):
+    # the rotated pixels registered straight from (orientations, pixel positions) make the
+    # records that gen_nonuniform_positions' coordinate arrays make: the same type-1 spreads
+    # (unit, gathered and bound strengths) and type-2 values bit for bit, the same max
+    # |coordinate|, and agreement with the exact sums; such a plan cannot be autotuned
+    from cmtip_amd import synth
+    from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_BOUND, CMTIP_PTS_ONES, CMTIP_PTS_REAL, CmtipError
+    from cmtip_amd.autocorrelation import gen_nonuniform_positions
+    from cmtip_amd.nufft import NufftPlan
+    d = dev()
+    s_np, ext = synth.detector(side)
+    s = torch.as_tensor(s_np.reshape(3, -1), dtype=torch.float64, device=d).contiguous()
+    q = torch.as_tensor(synth.random_quaternions(n_img, seed=M), device=d)
+    H, K, L = (t.reshape(-1) for t in gen_nonuniform_positions(q, s, recip_extent=ext))
+    N = H.numel()
+    rng = np.random.default_rng(M)
+    c = torch.as_tensor(rng.standard_normal(N).astype(np.float32), device=d)
+    say(f"setpts_rotated M={M} N={N}")
+    p_arr, p_rot = NufftPlan(1, M, +1), NufftPlan(1, M, +1)
+    flags = CMTIP_GRID_REAL | CMTIP_PTS_REAL
+    outs = []
+    for p, reg in ((p_arr, lambda cc: p_arr.setpts(H, K, L) if cc is None else p_arr.setpts_strengths(H, K, L, cc)),
+                   (p_rot, lambda cc: p_rot.setpts_rotated(q, s, ext, cc))):
+        got = []
+        reg(None)
+        for fl, cc in ((CMTIP_GRID_REAL | CMTIP_PTS_ONES, None), (flags, c)):
+            f = torch.empty((M, M, M), device=d)
+            p.execute(cc, f, fl, None, 1.0 / M ** 3)
+            got.append(f)
+        reg(c)
+        f = torch.empty((M, M, M), device=d)
+        p.execute(None, f, flags | CMTIP_PTS_BOUND, None, 1.0 / M ** 3)
+        got.append(f)
+        got.append(p.points_max_abs())
+        outs.append(got)
+    for a, b in zip(*outs):
+        assert (a == b) if isinstance(a, float) else torch.equal(a, b)
+    with pytest.raises(CmtipError):
+        p_rot.autotune(c, torch.empty((M, M, M), device=d), flags)
+    if N <= 20000:
+        xs, ys, zs = (t.cpu().numpy() for t in (H, K, L))
+        exact = O.nudft1(c.cpu().numpy().astype(np.complex128), xs, ys, zs, M).real / M ** 3
+        assert rel_l2(outs[1][2], exact) < TOL
+    # type 2 from the same records
+    f = torch.as_tensor(rng.standard_normal((M, M, M)).astype(np.float32), device=d)
+    t2 = []
+    for reg in (lambda p: p.setpts(H, K, L), lambda p: p.setpts_rotated(q, s, ext)):
+        p = NufftPlan(2, M, -1)
+        reg(p)
+        out = torch.empty(N, device=d)
+        p.execute(out, f, CMTIP_GRID_REAL | CMTIP_PTS_REAL)
+        t2.append(out)
+    assert torch.equal(t2[0], t2[1])
+    with pytest.raises(CmtipError):
+        NufftPlan(2, M, -1).setpts_rotated(q, s, ext, c)
