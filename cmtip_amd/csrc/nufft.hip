@@ -1096,21 +1096,31 @@ __global__ void rotinv_kernel(const float* __restrict__ quats, int64_t n, double
 }
 
 // x = fl32(fl32(row . s) / ext) * fl32(pi): the reference's einsum in float64, then
-// astype(float32), / reciprocal_extent, * np.pi in float32 (align.py:52-57).
-// ext == 0 returns the unscaled fl32(row . s) (autocorrelation.py:27,210).
-__device__ __forceinline__ float rot_coord(const double* r, double s0, double s1, double s2, float ext) {
+// astype(float32), / reciprocal_extent, * np.pi in float32 (align.py:52-57).  iext =
+// 1 / ext rounded to double (inv_ext); iext == 0 (ext == 0) returns the unscaled
+// fl32(row . s) (autocorrelation.py:27,210).  The float quotient hf / ext is formed as
+// fl32(fl64(hf * iext)): that product is within 2^-52 (relative) of the exact quotient,
+// and a quotient of two floats is either representable or farther than 2^-49 from
+// every float rounding midpoint (hf = H 2^a, ext = E 2^b, a midpoint m = K 2^c with 24-,
+// 24- and 25-bit integers H, E and odd K: hf - m ext is a nonzero multiple of
+// 2^min(a, b + c)), so rounding the product to float gives the IEEE quotient bit for
+// bit, for three instructions instead of a division sequence.
+__device__ __forceinline__ float rot_coord(const double* r, double s0, double s1, double s2, double iext) {
 #pragma clang fp contract(off)
     const double h = r[0] * s0 + r[1] * s1 + r[2] * s2;
     const float hf = (float)h;
-    return ext == 0.0f ? hf : (hf / ext) * 3.14159265358979323846f;
+    return iext == 0.0 ? hf : (float)((double)hf * iext) * 3.14159265358979323846f;
 }
+
+// 1 / ext in double for rot_coord (0 for ext == 0: no scaling)
+inline double inv_ext(float ext) { return ext == 0.0f ? 0.0 : 1.0 / (double)ext; }
 
 struct PtsRotated {
     const double* rinv;  // 9 doubles per orientation
     const double* s;     // 3 x P
     const int* order;    // P pixel indices or NULL
     int64_t P;
-    float ext;
+    double iext;         // inv_ext(reciprocal extent)
     // coordinates of point j (j < 2^31: the route works in batches); returns its output
     // index o * P + pixel
     __device__ __forceinline__ int64_t at(int64_t j, float* px, float* py, float* pz) const {
@@ -1119,9 +1129,9 @@ struct PtsRotated {
         const int64_t p = order ? (int64_t)order[slot] : (int64_t)slot;
         const double* r = rinv + 9 * o;
         const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
-        *px = rot_coord(r, s0, s1, s2, ext);
-        *py = rot_coord(r + 3, s0, s1, s2, ext);
-        *pz = rot_coord(r + 6, s0, s1, s2, ext);
+        *px = rot_coord(r, s0, s1, s2, iext);
+        *py = rot_coord(r + 3, s0, s1, s2, iext);
+        *pz = rot_coord(r + 6, s0, s1, s2, iext);
         return (int64_t)o * P + p;
     }
 };
@@ -1136,7 +1146,7 @@ struct SortedRotated {
 
 __global__ void __launch_bounds__(kBlock) rotate_coords_kernel(const double* __restrict__ rinv,
                                                                const double* __restrict__ s, int64_t P,
-                                                               int64_t total, float ext,
+                                                               int64_t total, double iext,
                                                                float* __restrict__ x,
                                                                float* __restrict__ y,
                                                                float* __restrict__ z) {
@@ -1145,9 +1155,9 @@ __global__ void __launch_bounds__(kBlock) rotate_coords_kernel(const double* __r
     const int64_t o = j / P, p = j - o * P;
     const double* r = rinv + 9 * o;
     const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
-    x[j] = rot_coord(r, s0, s1, s2, ext);
-    y[j] = rot_coord(r + 3, s0, s1, s2, ext);
-    z[j] = rot_coord(r + 6, s0, s1, s2, ext);
+    x[j] = rot_coord(r, s0, s1, s2, iext);
+    y[j] = rot_coord(r + 3, s0, s1, s2, iext);
+    z[j] = rot_coord(r + 6, s0, s1, s2, iext);
 }
 
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
@@ -1296,7 +1306,7 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
                                                             const double* __restrict__ s,
                                                             const int* __restrict__ order, int64_t P,
                                                             int64_t tiles_per_orient, int64_t total_tiles,
-                                                            float ext, float* __restrict__ out,
+                                                            double iext, float* __restrict__ out,
                                                             uint16_t* __restrict__ out_bf16,
                                                             double* __restrict__ moments,
                                                             int* __restrict__ fit_count) {
@@ -1312,9 +1322,9 @@ __global__ void __launch_bounds__(kSliceBlock) slice_real_kernel(const float* __
     const int64_t p = order ? (int64_t)order[active ? slot : 0] : (active ? slot : 0);
     const double* r = rinv + 9 * o;
     const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
-    const float tx = fine_coord(rot_coord(r, s0, s1, s2, ext), hi, lo, n);
-    const float ty = fine_coord(rot_coord(r + 3, s0, s1, s2, ext), hi, lo, n);
-    const float tz = fine_coord(rot_coord(r + 6, s0, s1, s2, ext), hi, lo, n);
+    const float tx = fine_coord(rot_coord(r, s0, s1, s2, iext), hi, lo, n);
+    const float ty = fine_coord(rot_coord(r + 3, s0, s1, s2, iext), hi, lo, n);
+    const float tz = fine_coord(rot_coord(r + 6, s0, s1, s2, iext), hi, lo, n);
     const int lx = first_node(tx, W), ly = first_node(ty, W), lz = first_node(tz, W);
     // box of the active lanes' footprints (wave-uniform values held in scalar registers)
     const int big = 1 << 29;
@@ -1435,7 +1445,7 @@ struct TileRot {
     const double* s;
     const int* order;
     int64_t P, tpo;  // pixels, tiles per orientation
-    float ext;
+    double iext;     // inv_ext(reciprocal extent)
     // pixel of lane `lane` in tile t: returns false past the last pixel
     __device__ __forceinline__ bool pixel(int64_t t, int lane, int64_t* o, int64_t* p) const {
         *o = t / tpo;
@@ -1448,9 +1458,9 @@ struct TileRot {
                                            float* tz) const {
         const double* r = rinv + 9 * o;
         const double s0 = s[p], s1 = s[P + p], s2 = s[2 * P + p];
-        *tx = fine_coord(rot_coord(r, s0, s1, s2, ext), hi, lo, n);
-        *ty = fine_coord(rot_coord(r + 3, s0, s1, s2, ext), hi, lo, n);
-        *tz = fine_coord(rot_coord(r + 6, s0, s1, s2, ext), hi, lo, n);
+        *tx = fine_coord(rot_coord(r, s0, s1, s2, iext), hi, lo, n);
+        *ty = fine_coord(rot_coord(r + 3, s0, s1, s2, iext), hi, lo, n);
+        *tz = fine_coord(rot_coord(r + 6, s0, s1, s2, iext), hi, lo, n);
     }
 };
 
@@ -1981,7 +1991,7 @@ struct SliceLaunch {
         }
         hipLaunchKernelGGL((slice_real_kernel<W>), dim3((unsigned)blocks), dim3(kSliceBlock), 0, st,
                            p->fine.as<const float>(), p->n, p->esk, p->tc_hi, p->tc_lo,
-                           p->rinv.as<const double>(), s, order, P, tpo, tiles, ext, out, out_bf16, moments,
+                           p->rinv.as<const double>(), s, order, P, tpo, tiles, inv_ext(ext), out, out_bf16, moments,
                            fit_count);
         CMTIP_LAUNCH_CHECK();
         return CMTIP_OK;
@@ -2335,7 +2345,7 @@ extern "C" int cmtip_nufft_setpts_rotated(cmtip_nufft_plan p, const float* quats
     CMTIP_TRY(ensure_rinv(p, quats, n_orient, st));
     // point j = o * P + pixel: the coordinates of cmtip_rotate_coords, generated where they
     // are binned and copied into the records
-    const PtsRotated rot{p->rinv.as<const double>(), s, nullptr, P, recip_extent};
+    const PtsRotated rot{p->rinv.as<const double>(), s, nullptr, P, inv_ext(recip_extent)};
     const int rc = bin_points(p, st, &rot, c, true);
     if (rc != CMTIP_OK) p->rot_records = 0;
     return rc;
@@ -2560,7 +2570,7 @@ extern "C" int cmtip_rotate_coords(const float* quats, int64_t n_orient, const d
     const int64_t total = n_orient * P;
     if (e == hipSuccess) {
         hipLaunchKernelGGL(rotate_coords_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, st,
-                           rinv.as<const double>(), s, P, total, recip_extent, x, y, z);
+                           rinv.as<const double>(), s, P, total, inv_ext(recip_extent), x, y, z);
         e = hipGetLastError();
     }
     // the scratch matrix buffer must outlive the kernels
@@ -2646,7 +2656,7 @@ int slice_tiles(cmtip_nufft_plan_s* p, const double* s, const int* order, int64_
         set_error("slice_real: %lld tiles is too many for the tile route", (long long)ntiles);
         return CMTIP_ERR_ARG;
     }
-    const TileRot rot{p->rinv.as<const double>(), s, order, P, tpo, ext};
+    const TileRot rot{p->rinv.as<const double>(), s, order, P, tpo, inv_ext(ext)};
     p->x = p->y = p->z = nullptr;  // the bins no longer hold registered points
     p->rot_records = 0;
     p->npts = std::max<int64_t>(p->npts, 1);
@@ -2669,7 +2679,7 @@ int slice_binned(cmtip_nufft_plan_s* p, const double* s, const int* order, int64
     p->rot_records = 0;
     for (int64_t o0 = 0; o0 < n_orient; o0 += per) {
         const int64_t no = std::min(per, n_orient - o0), N = no * P;
-        const PtsRotated rot{p->rinv.as<const double>() + 9 * o0, s, order, P, ext};
+        const PtsRotated rot{p->rinv.as<const double>() + 9 * o0, s, order, P, inv_ext(ext)};
         p->npts = N;
         CMTIP_TRY(bin_points(p, st, &rot));
         CMTIP_TRY(dispatch_w<InterpLaunch>(p->w, p, out + o0 * P, 1, &rot, st));

@@ -81,3 +81,39 @@ def test_bench_refuses_more_than_one_gpu():
     assert res.returncode == 2
     msg = json.loads(res.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def _quotient_by_double_product(h, ext):
+    # the kernels' coordinate scaling: fl32(fl64(h * fl64(1 / ext)))
+    return (h.astype(np.float64) * (1.0 / np.float64(ext))).astype(np.float32)
+
+
+def test_scaled_coordinates_equal_ieee_division():
+    # rot_coord forms fl32(H) / fl32(ext) as the double product with 1 / ext rounded to
+    # double; a quotient of two floats never lies within 2^-49 of a float rounding
+    # midpoint, so this rounds like the IEEE float division numpy performs, bit for bit
+    rng = np.random.default_rng(7)
+    exts = [synth.detector(n)[1] for n in (16, 64, 128)]
+    exts += list(rng.uniform(0.5, 2.0, 5).astype(np.float32) * np.float32(2.0) ** rng.integers(-30, 30, 5))
+    exts += [np.float32(3.0), np.float32(1.0) + np.float32(2.0 ** -23), np.float32(2.0 ** 20)]
+    for ext in exts:
+        ext = np.float32(ext)
+        # every mantissa pattern class: random bits over the exponents near ext, plus
+        # near-integer multiples of ext (quotients close to representable values)
+        bits = rng.integers(0, 2 ** 23, 400_000, dtype=np.int64)
+        e = rng.integers(-20, 21, bits.size) + int(np.frexp(ext)[1]) + 126
+        h = ((e << 23) | bits).astype(np.uint32).view(np.float32)
+        h = np.concatenate([h, -h, (np.arange(1, 200_001, dtype=np.float32) * ext),
+                            np.nextafter(np.arange(1, 50_001, dtype=np.float32) * ext, np.float32(np.inf))])
+        want = h / ext
+        got = _quotient_by_double_product(h, ext)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), float(ext)
+
+
+def test_float_reciprocal_scaling_is_caught():
+    # the check above is sensitive: a float reciprocal (fl32(h * fl32(1 / ext))) differs
+    rng = np.random.default_rng(8)
+    ext = np.float32(synth.detector(128)[1])
+    h = (rng.standard_normal(200_000) * ext).astype(np.float32)
+    assert not np.array_equal(h * (np.float32(1.0) / ext), h / ext)
+    assert np.array_equal(_quotient_by_double_product(h, ext), h / ext)
