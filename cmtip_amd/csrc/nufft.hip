@@ -2292,7 +2292,9 @@ int setpts_impl(cmtip_nufft_plan p, int64_t npts, const float* x, const float* y
         p->bound = bind ? 1 : 0;
         return CMTIP_OK;
     }
-    return bin_points(p, as_stream(stream), nullptr, c);
+    const int rc = bin_points(p, as_stream(stream), nullptr, c);
+    if (rc != CMTIP_OK) p->x = p->y = p->z = nullptr;  // no usable bins: execute refuses
+    return rc;
 }
 }  // namespace
 
@@ -2332,13 +2334,15 @@ extern "C" int cmtip_nufft_setpts_rotated(cmtip_nufft_plan p, const float* quats
         return CMTIP_ERR_ARG;
     }
     const int64_t npts = n_orient * P;
+    // until the records are built the plan holds no points an execute may use
     p->x = p->y = p->z = nullptr;
     p->npts = npts;
     p->bound = 0;
-    p->rot_records = 1;
+    p->rot_records = 0;
     if (npts == 0) {
         p->amax_valid = 2;
         p->bound = c ? 1 : 0;
+        p->rot_records = 1;
         return CMTIP_OK;
     }
     hipStream_t st = as_stream(stream);
@@ -2346,9 +2350,9 @@ extern "C" int cmtip_nufft_setpts_rotated(cmtip_nufft_plan p, const float* quats
     // point j = o * P + pixel: the coordinates of cmtip_rotate_coords, generated where they
     // are binned and copied into the records
     const PtsRotated rot{p->rinv.as<const double>(), s, nullptr, P, inv_ext(recip_extent)};
-    const int rc = bin_points(p, st, &rot, c, true);
-    if (rc != CMTIP_OK) p->rot_records = 0;
-    return rc;
+    CMTIP_TRY(bin_points(p, st, &rot, c, true));
+    p->rot_records = 1;
+    return CMTIP_OK;
 }
 
 extern "C" int cmtip_nufft_points_max_abs(cmtip_nufft_plan p, float* max_abs, void* stream) {
