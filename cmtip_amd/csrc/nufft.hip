@@ -166,8 +166,11 @@ constexpr bool kSpreadDeal = CMTIP_SPREAD_DEAL;  // deal points to lanes by LDS 
 // dealing by those 16 classes measured 8.4-8.5 ms at M = 81 (81.92 M points, real
 // strengths) against 9.0 (32 classes), 10.0 (8) and 11.0 (64)
 constexpr int kSpreadClasses = 16;
+// 32-bit accumulators (unit strengths): one class per bank, 32 classes, so that the 32
+// lanes of each pass of a 32-bit LDS atomic hit 32 distinct banks
+constexpr int kSpreadClasses32 = 32;
 // static LDS of the spreading kernel (point dealing), beside its dynamic subgrid
-constexpr size_t kSpreadStaticLds = 3 * kChunk + 4 * 3 * kSpreadClasses + 64;
+constexpr size_t kSpreadStaticLds = 3 * kChunk + 4 * 3 * kSpreadClasses32 + 64;
 constexpr size_t kMaxLds = 160 * 1024;     // LDS per workgroup on gfx950
 constexpr size_t kDefaultLds = 64 * 1024;  // dynamic LDS allowed without raising the limit
 constexpr size_t kStaticLds = 256;         // reserve for the binned kernels' static LDS
@@ -702,13 +705,15 @@ __global__ void absmax_kernel(const float* __restrict__ c, int64_t count, unsign
     block_max_or(m, neg ? 1u : 0u, out_bits, out_bits + 1);
 }
 
-// Contribution x (|x| < 2^30) of a spread as a 64-bit fixed-point addend: the nearest
-// integer, converted exactly in 32 bits and sign-extended (zero-extended for
-// non-negative strengths).
+// Contribution w p (|w p| < 2^30, w >= 0) of a spread as a 64-bit fixed-point addend:
+// the nearest integer (ties away from zero) as trunc(w p + h) with h = +-1/2 of the sign
+// of p, one fused multiply-add and one 32-bit conversion, sign-extended (zero-extended
+// for non-negative strengths, h = 1/2).  Above 2^23 the product is already an integer
+// and adding h leaves it unchanged.
 template <bool NONNEG>
-__device__ __forceinline__ unsigned long long fixed_addend(float x) {
-    if (NONNEG) return (unsigned long long)(uint32_t)rintf(x);
-    return (unsigned long long)(long long)(int)rintf(x);
+__device__ __forceinline__ unsigned long long fixed_addend(float w, float p, float h) {
+    if (NONNEG) return (unsigned long long)(uint32_t)fmaf(w, p, 0.5f);
+    return (unsigned long long)(long long)(int)fmaf(w, p, h);
 }
 
 // U32 (unit strengths only): 32-bit LDS accumulators at the scale 2^kOnes32Scale per
@@ -738,7 +743,8 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
     spread_pitch(S, &SP, &SSP);
     const int L3 = S * SSP;
     __shared__ int chunk_id, rounds;
-    __shared__ int cls_count[kSpreadClasses], cls_start[kSpreadClasses], cls_fill[kSpreadClasses];
+    constexpr int NCL = U32 ? kSpreadClasses32 : kSpreadClasses;  // bank classes
+    __shared__ int cls_count[NCL], cls_start[NCL], cls_fill[NCL];
     __shared__ uint16_t dealt[kChunk];  // chunk-local point indices grouped by class
     __shared__ uint8_t cls_of[kChunk];
     const int total = ctrl[0];
@@ -759,15 +765,16 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         const int b = ch.x;
         const int ox = (b / (nbx * nbx)) * bs, oy = ((b / nbx) % nbx) * bs, oz = (b % nbx) * bs;
         for (int i = threadIdx.x; i < NC * L3; i += kSpreadThreads) acc[i] = (AccT)0;
-        if (threadIdx.x < kSpreadClasses) {
+        if (threadIdx.x < NCL) {
             cls_count[threadIdx.x] = 0;
             cls_fill[threadIdx.x] = 0;
         }
         __syncthreads();
         // deal the chunk's points to lanes by the LDS bank pair of their first accumulator
-        // (class = its cell index mod 16): lanes that share a class sit 16 lanes
-        // apart, so every atomic of the 343-add sequence, which adds the same offset for
-        // all lanes, hits 16 distinct bank pairs within each group of 16 lanes
+        // (class = its cell index mod 16; with 32-bit accumulators the bank itself, mod
+        // 32): lanes that share a class sit NCL lanes apart, so every atomic of the
+        // 343-add sequence, which adds the same offset for all lanes, hits NCL distinct
+        // bank pairs (banks) within each group of NCL lanes
         const int npts = kSpreadDeal ? ch.z - ch.y : 0;  // <= kChunk
         for (int i = threadIdx.x; i < npts; i += kSpreadThreads) {
             float t;
@@ -778,19 +785,19 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             // the cell index, not the accumulator index: a complex cell's two accumulators
             // would give only even classes and leave half the lanes without points
             // (complex type 1 over 41 M points: 20.1 ms that way, 11.9 ms this way)
-            const int cl = (ux * SSP + uy * SP + uz) & (kSpreadClasses - 1);
+            const int cl = (ux * SSP + uy * SP + uz) & (NCL - 1);
             cls_of[i] = (uint8_t)cl;
             atomicAdd(cls_count + cl, 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             int s0 = 0, mx = 0;
-            for (int q = 0; q < kSpreadClasses; ++q) {
+            for (int q = 0; q < NCL; ++q) {
                 cls_start[q] = s0;
                 s0 += cls_count[q];
                 mx = max(mx, cls_count[q]);
             }
-            rounds = (mx + kSpreadThreads / kSpreadClasses - 1) / (kSpreadThreads / kSpreadClasses);
+            rounds = (mx + kSpreadThreads / NCL - 1) / (kSpreadThreads / NCL);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < npts; i += kSpreadThreads) {
@@ -798,12 +805,12 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
             dealt[cls_start[cl] + atomicAdd(cls_fill + cl, 1)] = (uint16_t)i;
         }
         __syncthreads();
-        const int my_cls = threadIdx.x & (kSpreadClasses - 1), my_sub = threadIdx.x / kSpreadClasses;
+        const int my_cls = threadIdx.x & (NCL - 1), my_sub = threadIdx.x / NCL;
         const int nrounds = kSpreadDeal ? rounds : (ch.z - ch.y + kSpreadThreads - 1) / kSpreadThreads;
         for (int r = 0; r < nrounds; ++r) {
             int k;
             if (kSpreadDeal) {
-                const int e = r * (kSpreadThreads / kSpreadClasses) + my_sub;
+                const int e = r * (kSpreadThreads / NCL) + my_sub;
                 if (e >= cls_count[my_cls]) continue;
                 k = ch.y + dealt[cls_start[my_cls] + e];
             } else {
@@ -833,6 +840,9 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                          : ldexpf(bound ? q.w : (c ? reinterpret_cast<const float*>(c)[src] : 1.0f), sc);
             }
             AccT* base = acc + NC * (lx * SSP + ly * SP + lz);
+            // rounding offsets of the signed conversion: every tap of the point has the sign
+            // of its strength (the weights are >= 0)
+            const float hr = copysignf(0.5f, vr), hi2 = copysignf(0.5f, vi);
             if (U32) {
 #pragma unroll
                 for (int a = 0; a < W; ++a)
@@ -841,7 +851,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                         AccT* row = base + (a * SSP + bb * SP);
                         const float pr = (wx[a] * wy[bb]) * vr;
 #pragma unroll
-                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (unsigned)rintf(wz[cc] * pr));
+                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (unsigned)fmaf(wz[cc], pr, 0.5f));
                     }
             } else if (nonneg) {
                 // real non-negative strengths (image counts, unit strengths): every
@@ -853,7 +863,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                         AccT* row = base + (a * SSP + bb * SP);
                         const float pr = (wx[a] * wy[bb]) * vr;
 #pragma unroll
-                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (AccT)fixed_addend<true>(wz[cc] * pr));
+                        for (int cc = 0; cc < W; ++cc) atomicAdd(row + cc, (AccT)fixed_addend<true>(wz[cc], pr, 0.5f));
                     }
             } else {
 #pragma unroll
@@ -866,8 +876,8 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                         const float pi = wab * vi;
 #pragma unroll
                         for (int cc = 0; cc < W; ++cc) {
-                            atomicAdd(row + NC * cc, (AccT)fixed_addend<false>(wz[cc] * pr));
-                            if (CPLX) atomicAdd(row + NC * cc + 1, (AccT)fixed_addend<false>(wz[cc] * pi));
+                            atomicAdd(row + NC * cc, (AccT)fixed_addend<false>(wz[cc], pr, hr));
+                            if (CPLX) atomicAdd(row + NC * cc + 1, (AccT)fixed_addend<false>(wz[cc], pi, hi2));
                         }
                     }
             }
