@@ -133,6 +133,26 @@ __device__ __forceinline__ void es_taps(float tau, const EsKernel& K, float* wt)
 // Fine-grid coordinate t = x n / (2 pi) with the constant split in two floats, so the
 // product carries about one rounding.  Non-finite or huge inputs are clamped so every
 // index below stays in range whatever the caller passes.
+// l mod n (result in [0, n)) for |l| < 2^22, n >= 1: the floor of l / n from a float
+// estimate, which is within 1/2 of the true quotient (relative error <= 2^-23), then
+// one correction each way; about half the instructions of an integer % by a runtime n.
+__device__ __forceinline__ int mod_small(int l, int n) {
+    const int q = (int)floorf((float)l * (1.0f / (float)n));
+    int r = l - q * n;
+    r += r < 0 ? n : 0;
+    r -= r >= n ? n : 0;
+    return r;
+}
+
+// l / d for 0 <= l < 2^22, d >= 1 (same estimate and corrections)
+__device__ __forceinline__ int div_small(int l, int d) {
+    int q = (int)((float)l * (1.0f / (float)d));
+    const int r = l - q * d;
+    q += r >= d ? 1 : 0;
+    q -= r < 0 ? 1 : 0;
+    return q;
+}
+
 __device__ __forceinline__ float fine_coord(float x, float hi, float lo, int n) {
     float t = fmaf(x, hi, x * lo);
     const float lim = 4.0f * (float)n;
@@ -233,9 +253,9 @@ inline size_t subgrid_lds(int bs, int W, bool spread, bool cplx) {
 __device__ __forceinline__ int first_node_wrapped(float x, float hi, float lo, int n, int W, float* t_out) {
     const float t = fine_coord(x, hi, lo, n);
     *t_out = t;
-    int l0 = (int)ceilf(t - 0.5f * (float)W);
-    l0 %= n;
-    return l0 < 0 ? l0 + n : l0;
+    // |t| <= 4n, so |l0| < 2^22 for every grid a plan allows
+    const int l0 = (int)ceilf(t - 0.5f * (float)W);
+    return mod_small(l0, n);
 }
 
 // Wave-aggregated counter update: lanes of a wave holding the same bin issue one
@@ -317,9 +337,9 @@ __global__ void bin_count_kernel(Src pts, int64_t N, int n, int W, int bs, int n
             cm = fabsf(cstr[j]);
             neg = cstr[j] < 0.0f;
         }
-        const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
-        const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
-        const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
+        const int bx = div_small(first_node_wrapped(x, hi, lo, n, W, &t), bs);
+        const int by = div_small(first_node_wrapped(y, hi, lo, n, W, &t), bs);
+        const int bz = div_small(first_node_wrapped(z, hi, lo, n, W, &t), bs);
         b = (bx * nbx + by) * nbx + bz;
         bin_of[j] = b;
     }
@@ -449,9 +469,9 @@ __global__ void __launch_bounds__(kBinBlock) bin_count_local_kernel(Src pts, int
             cm = fmaxf(cm, fabsf(v));
             neg = neg || v < 0.0f;
         }
-        const int bx = first_node_wrapped(x, hi, lo, n, W, &t) / bs;
-        const int by = first_node_wrapped(y, hi, lo, n, W, &t) / bs;
-        const int bz = first_node_wrapped(z, hi, lo, n, W, &t) / bs;
+        const int bx = div_small(first_node_wrapped(x, hi, lo, n, W, &t), bs);
+        const int by = div_small(first_node_wrapped(y, hi, lo, n, W, &t), bs);
+        const int bz = div_small(first_node_wrapped(z, hi, lo, n, W, &t), bs);
         const int b = (bx * nbx + by) * nbx + bz;
         bin_of[j] = b;
         atomicAdd(hist + b, 1);
@@ -1273,10 +1293,8 @@ __device__ __forceinline__ void row_weights(float t, int l0, const EsKernel& K, 
 
 __device__ __forceinline__ int first_node(float t, int W) { return (int)ceilf(t - 0.5f * (float)W); }
 
-__device__ __forceinline__ int wrap_n(int l, int n) {
-    int b = l % n;
-    return b < 0 ? b + n : b;
-}
+// l mod n in [0, n) for the bounded node indices of the slicing kernels (|l| < 2^22)
+__device__ __forceinline__ int wrap_n(int l, int n) { return mod_small(l, n); }
 
 // Interpolation of one point straight from the fine grid (no LDS staging): first
 // nodes lx, ly, lz (unwrapped), x / y weights given, z weights from tz.
