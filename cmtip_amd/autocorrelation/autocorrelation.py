@@ -16,10 +16,11 @@ import torch
 
 from .. import _arrays, _lib, synth
 from .._lib import check, ptr, stream
-from ..nufft.adjoint import _adjoint
-from ..nufft.plan import DEFAULT_EPS
+from ..nufft.adjoint import _adjoint, assert_points_in_range
+from ..nufft.plan import DEFAULT_EPS, get_plan
 
 _SOLVERS = {}
+_ANTISUPPORT = {}
 
 
 class ACSolver:
@@ -119,11 +120,20 @@ def gen_nonuniform_positions(orientations, pixel_position_reciprocal, *, recip_e
     return tuple(_arrays.out(t, as_torch) for t in (H, K, L))
 
 
+def _antisupport_dev(M):
+    """gen_F_antisupport(M) as a float32 (0 / 1) device tensor, built once per (M, device)."""
+    key = (int(M), torch.cuda.current_device())
+    out = _ANTISUPPORT.get(key)
+    if out is None:
+        out = torch.empty((M, M, M), dtype=torch.float32, device=_lib.device())
+        check(_lib.lib().cmtip_gen_F_antisupport(int(M), ptr(out), stream()), "cmtip_gen_F_antisupport")
+        _ANTISUPPORT[key] = out
+    return out
+
+
 def gen_F_antisupport(M):
     """Boolean (M, M, M) mask, True outside the sphere of radius pi (autocorrelation.py:66-85)."""
-    out = torch.empty((M, M, M), dtype=torch.float32, device=_lib.device())
-    check(_lib.lib().cmtip_gen_F_antisupport(int(M), ptr(out), stream()), "cmtip_gen_F_antisupport")
-    return (out > 0.5).cpu().numpy()
+    return (_antisupport_dev(M) > 0.5).cpu().numpy()
 
 
 def _support_of(ac_estimate, M):
@@ -204,17 +214,22 @@ def setup_linops(H, K, L, data, ac_support, weights, x0, M, Mtot, N, reciprocal_
     Lm = solver.L
     sup = _arrays.to_dev(ac_support, torch.float32)
     solver.set_support(sup)
-    solver.set_antisupport(torch.as_tensor(gen_F_antisupport(M), dtype=torch.float32))
+    solver.set_antisupport(_antisupport_dev(M))  # the mask stays on the device
     # kernel: real type-1 of ones at M_ups modes over (2M)^3 (:141-148; 1/(2M)^3 also when L > 2M)
     ugrid_conv = _adjoint(None, H_, K_, L_, Lm, True, None, eps, scale=1.0 / float(2 * M) ** 3, ones=True,
-                          autotune=autotune, rot=rot)
+                          autotune=autotune, rot=rot, check_range=False)
     solver.set_kernel(ugrid_conv)
     # rhs: d = alambda * A^H (data * weights) + rlambda * x0 (:162-171)
     nuvect_Db = _arrays.to_dev(data, torch.float32).reshape(-1)
     if weights is not None:  # None: unit weights, the product is the data itself
         nuvect_Db = nuvect_Db * _arrays.to_dev(weights, torch.float32).reshape(-1)
-    uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps, autotune=autotune, rot=rot).reshape(-1)
-    if bool(torch.isnan(uvect_ADb).any()):
+    uvect_ADb = _adjoint(nuvect_Db, H_, K_, L_, M, True, sup, eps, autotune=autotune, rot=rot,
+                         check_range=False).reshape(-1)
+    has_nan = torch.isnan(uvect_ADb).any()
+    # the range checks of both spreads (adjoint.py:29,66) after both are queued: one
+    # synchronisation for the two checks and the NaN test
+    assert_points_in_range(get_plan(1, Lm, +1, eps), get_plan(1, M, +1, eps))
+    if bool(has_nan):
         print("Warning: nans in the adjoint calculation; intensities may be too large", flush=True)
     x0d = _arrays.to_dev(x0, torch.float32).reshape(-1)
     d = alambda * uvect_ADb + rlambda * x0d

@@ -11,10 +11,13 @@ from .plan import DEFAULT_EPS, get_plan, tuned_setpts
 
 
 def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, ones=False, autotune=False,
-             rot=None):
+             rot=None, check_range=True):
     """rot (quats, s, recip_extent), device tensors: the points are the rotated pixels of
     gen_nonuniform_positions(quats, s, recip_extent=...), registered without coordinate
-    arrays (H_, K_, L_ unused; no autotune)."""
+    arrays (H_, K_, L_ unused; no autotune).  check_range False leaves the range check of
+    adjoint.py:29,66 to the caller (``assert_points_in_range``), so that several spreads
+    can be queued before one synchronising read; out-of-range points are clamped by the
+    kernels, never read or written outside the grid."""
     if rot is None:
         as_torch = _arrays.is_torch(nuvect, H_)
         x = _arrays.to_dev(H_, torch.float32).reshape(-1)
@@ -63,9 +66,17 @@ def _adjoint(nuvect, H_, K_, L_, M, use_recip_sym, support, eps, scale=None, one
     else:
         plan.setpts(x, y, z)
     # the range check of adjoint.py:29, 66: the maximum is recorded while setpts bins
-    assert plan.points_max_abs() < 3 * np.pi
+    if check_range:
+        assert_points_in_range(plan)
     plan.execute(c, f, flags, sup, scale)
     return _arrays.out(f, as_torch)
+
+
+def assert_points_in_range(*plans):
+    """The reference's assertion max |coordinate| < 3 pi (adjoint.py:29,66) on the points
+    registered with each plan (one stream synchronisation for all of them)."""
+    for plan in plans:
+        assert plan.points_max_abs() < 3 * np.pi
 
 
 def adjoint_cpu(nuvect, H_, K_, L_, M, use_recip_sym=True, support=None, *, eps=DEFAULT_EPS):

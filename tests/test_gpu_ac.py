@@ -88,6 +88,21 @@ def test_setup_linops_vs_oracle():
     assert rel_l2(W.matvec(u.astype(np.float32)), want) < 1e-5
 
 
+def test_setup_linops_range_check():
+    # the range assertion of adjoint.py:29,66, read once after both spreads are queued,
+    # still refuses points outside [-3 pi, 3 pi) (one coordinate moved to 3.5 pi)
+    M = 11
+    s, ext, q, acm, x, y, z = _geometry(M)
+    N = x.size
+    x = x.copy()
+    x[N // 2] = 3.5 * np.pi
+    say("setup_linops range check M=11")
+    with pytest.raises(AssertionError):
+        A.setup_linops(torch.as_tensor(x), torch.as_tensor(y), torch.as_tensor(z), np.ones(N, np.float32),
+                       np.ones((M,) * 3, np.float32), None, np.zeros(M ** 3), M, M ** 3, N, ext, 1, 1e-3, 1e3,
+                       scaled=True)
+
+
 @pytest.mark.parametrize("with_estimate", [False, True])
 def test_solve_ac_vs_oracle(with_estimate):
     M = 11
