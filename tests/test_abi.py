@@ -60,9 +60,11 @@ def test_argument_errors_need_no_gpu():
     assert L.cmtip_nufft_makeplan(2, 0, -1, 1e-6, ctypes.byref(h)) == ERR_ARG
     assert L.cmtip_nufft_makeplan(2, 10, 0, 1e-6, ctypes.byref(h)) == ERR_ARG
     assert L.cmtip_nufft_setpts(None, 1, None, None, None, None) == ERR_ARG
+    assert L.cmtip_nufft_setpts_rotated(None, None, 1, None, 1, 1.0, None, None) == ERR_ARG  # null plan
     assert L.cmtip_acsolver_create(10, 15, ctypes.byref(h)) == ERR_ARG  # odd L
     assert L.cmtip_acsolver_create(10, 18, ctypes.byref(h)) == ERR_ARG  # L < 2M
-    assert L.cmtip_match_topk_bf16(None, 10, 0, None, 10, 16, None, None, None, 9, None, None) == ERR_ARG  # k > 8
+    assert L.cmtip_match_topk_bf16(None, 10, 0, None, 10, 16, None, None, None, 17, None, None) == ERR_ARG  # k > 16
+    assert b"k" in L.cmtip_last_error()
     assert L.cmtip_match_argmin_f32(None, 10, 0, None, 10, 16, None, None, None, None, None) == ERR_ARG
     assert L.cmtip_match_argmin_f32(None, 10, -1, None, 10, 16, None, None, None, None, None) == ERR_ARG
     # zero-sized problems are no-ops that succeed without touching the device
