@@ -10,7 +10,11 @@ Both run on the forward's type-2 plan and its point binning: the spread as the p
 adjoint (CMTIP_EXEC_ADJOINT), the coordinate gradient as type-2 imaginary-part outputs
 (real interpolation of the fine grid's imaginary part).  ``rotated_coords`` builds the
 rotated pixel coordinates from quaternions with torch operations, so gradients reach
-the quaternions for orientation refinement.
+the quaternions for orientation refinement.  ``slice_quaternions`` does the same in one
+function without coordinate arrays: the plan bins the rotated pixels straight from the
+normalised quaternions (cmtip_nufft_setpts_rotated) and the backward contracts the
+coordinate gradient with the pixel positions (one thin matrix product per axis) and
+the closed-form derivative of R(q).
 """
 import numpy as np
 import torch
@@ -95,8 +99,96 @@ def rotated_coords(q, pixel_position_reciprocal, reciprocal_extent):
     return v[0], v[1], v[2]
 
 
+# dR_ij / dq_m = sum_l _DR[i, j, m, l] q_l for R = quaternion_rotation(q) of a unit q
+# (the partial derivatives of its nine quadratic entries)
+_DR_ENTRIES = (
+    (0, 0, 2, 2, -4), (0, 0, 3, 3, -4),
+    (0, 1, 0, 3, -2), (0, 1, 1, 2, 2), (0, 1, 2, 1, 2), (0, 1, 3, 0, -2),
+    (0, 2, 0, 2, 2), (0, 2, 1, 3, 2), (0, 2, 2, 0, 2), (0, 2, 3, 1, 2),
+    (1, 0, 0, 3, 2), (1, 0, 1, 2, 2), (1, 0, 2, 1, 2), (1, 0, 3, 0, 2),
+    (1, 1, 1, 1, -4), (1, 1, 3, 3, -4),
+    (1, 2, 0, 1, -2), (1, 2, 1, 0, -2), (1, 2, 2, 3, 2), (1, 2, 3, 2, 2),
+    (2, 0, 0, 2, -2), (2, 0, 1, 3, 2), (2, 0, 2, 0, -2), (2, 0, 3, 1, 2),
+    (2, 1, 0, 1, 2), (2, 1, 1, 0, 2), (2, 1, 2, 3, 2), (2, 1, 3, 2, 2),
+    (2, 2, 1, 1, -4), (2, 2, 2, 2, -4),
+)
+_DR_CACHE = {}
+
+
+def rotation_derivative(device, dtype):
+    """(3, 3, 4, 4) coefficients of dR/dq (see _DR_ENTRIES)."""
+    key = (str(device), dtype)
+    D = _DR_CACHE.get(key)
+    if D is None:
+        D = torch.zeros((3, 3, 4, 4), dtype=dtype)
+        for i, j, m, l, v in _DR_ENTRIES:
+            D[i, j, m, l] = v
+        D = D.to(device)
+        _DR_CACHE[key] = D
+    return D
+
+
+class SliceQuaternions(torch.autograd.Function):
+    """I = Re(T2(f)) at the rotated pixels x = pi R(q)^T s / ext (q normalised), with no
+    coordinate arrays: the forward bins the pixels straight from the quaternions, the
+    backward gives d/df (the adjoint spread) and d/dq (the coordinate gradient
+    contracted with s, then the derivative of R and of the normalisation)."""
+
+    @staticmethod
+    def forward(ctx, ac, q, s64, ext, eps=DEFAULT_EPS):
+        ac = ac.contiguous().float()
+        norm = torch.linalg.vector_norm(q, dim=-1, keepdim=True)
+        qn = (q / norm).contiguous()
+        M = ac.shape[0]
+        n, P = q.shape[0], s64.shape[1]
+        plan = get_plan(2, M, -1, eps)
+        plan.setpts_rotated(qn, s64, ext)
+        out = torch.empty(n * P, dtype=torch.float32, device=ac.device)
+        plan.execute(out, ac, CMTIP_GRID_REAL | CMTIP_PTS_REAL)
+        ctx.save_for_backward(ac, qn, norm, s64)
+        ctx.ext, ctx.eps = ext, eps
+        return out.reshape(n, P)
+
+    @staticmethod
+    def backward(ctx, grad):
+        ac, qn, norm, s64 = ctx.saved_tensors
+        M = ac.shape[0]
+        n, P = qn.shape[0], s64.shape[1]
+        g = grad.contiguous().float().reshape(-1)
+        grad_ac = grad_q = None
+        p2 = get_plan(2, M, -1, ctx.eps)
+        p2.setpts_rotated(qn, s64, ctx.ext)  # skipped while the forward's points are registered
+        if ctx.needs_input_grad[0]:
+            grad_ac = torch.empty_like(ac)
+            p2.execute(g, grad_ac, CMTIP_GRID_REAL | CMTIP_PTS_REAL | CMTIP_EXEC_ADJOINT, None, 1.0)
+        if ctx.needs_input_grad[1]:
+            k = _modes(M, ac.device)
+            s32 = s64.to(torch.float32)
+            # G[:, d, j] = sum_p dL/dx_d(o, p) s_j(p): x_d = c sum_j R[j, d] s_j with c = pi / ext
+            G = torch.empty((n, 3, 3), dtype=torch.float32, device=ac.device)
+            c = torch.empty(n * P, dtype=torch.float32, device=ac.device)
+            for d in range(3):
+                shape = [1, 1, 1]
+                shape[d] = M
+                kf = (ac * k.reshape(shape)).contiguous()
+                p2.execute(c, kf, CMTIP_GRID_REAL | CMTIP_PTS_IMAG)  # Im(T2(k_d f))
+                G[:, d, :] = (g * c).reshape(n, P) @ s32.t()
+            scale = float(np.float32(np.pi)) / float(np.float32(ctx.ext))
+            gR = G.transpose(1, 2) * scale  # dL/dR[j, d]
+            dqn = torch.einsum("nij,ijml,nl->nm", gR, rotation_derivative(ac.device, torch.float32), qn)
+            grad_q = (dqn - qn * (qn * dqn).sum(-1, keepdim=True)) / norm
+        return grad_ac, grad_q, None, None, None
+
+
 def slice_quaternions(ac, q, pixel_position_reciprocal, reciprocal_extent, eps=DEFAULT_EPS):
-    """Slices (n, P) of ac at quaternions q, differentiable in ac and q."""
+    """Slices (n, P) of ac at quaternions q, differentiable in ac and q.  Float32
+    quaternions on the device take SliceQuaternions (no coordinate arrays); others the
+    torch coordinate map and nufft_slice."""
+    if q.dtype == torch.float32 and q.is_cuda and reciprocal_extent:
+        s64 = torch.as_tensor(pixel_position_reciprocal, device=q.device, dtype=torch.float64).reshape(3, -1)
+        ext = float(np.float32(reciprocal_extent.item() if isinstance(reciprocal_extent, torch.Tensor)
+                               else reciprocal_extent))
+        return SliceQuaternions.apply(ac, q, s64.contiguous(), ext, eps)
     H, K, L = rotated_coords(q, pixel_position_reciprocal, reciprocal_extent)
     return nufft_slice(ac, H, K, L, eps)
 
@@ -104,7 +196,7 @@ def slice_quaternions(ac, q, pixel_position_reciprocal, reciprocal_extent, eps=D
 def l2_loss(model, images, gamma=None):
     """sum (gamma model - images)^2 (the matching distance, align.py:122-128, unweighted)."""
     if gamma is None:
-        gamma = 1.0
+        return ((model - images) ** 2).sum()
     return ((gamma * model - images) ** 2).sum()
 
 

@@ -68,20 +68,32 @@ class NufftPlan:
         check(_lib.lib().cmtip_nufft_setpts_strengths(self._h, n, ptr(x), ptr(y), ptr(z), ptr(c), stream()),
               "cmtip_nufft_setpts_strengths")
 
-    def setpts_rotated(self, quats, s, recip_extent, c=None):
+    def setpts_rotated(self, quats, s, recip_extent, c=None, force=False):
         """Register the rotated pixels of ``gen_nonuniform_positions(quats, s,
         recip_extent=...)`` (quats: (n, 4) float32, s: (3, P) float64, device tensors;
         point o * P + p) without materialising their coordinate arrays; c (type 1,
-        optional): real strengths bound as by ``setpts_strengths``."""
+        optional): real strengths bound as by ``setpts_strengths``.  Without strengths,
+        registering the rotations this plan already holds is skipped unless force (same
+        rule as ``setpts``)."""
         n, P = quats.shape[0], s.shape[1]
         assert quats.dtype == torch.float32 and quats.is_cuda and quats.is_contiguous() and quats.shape[1] == 4
         assert s.dtype == torch.float64 and s.is_cuda and s.is_contiguous() and s.shape[0] == 3
+        key = None
         if c is not None:
             assert c.dtype == torch.float32 and c.is_cuda and c.is_contiguous() and c.numel() == n * P
+        else:
+            try:
+                key = ("rotated", quats.data_ptr(), quats._version, s.data_ptr(), s._version, n, P,
+                       float(recip_extent))
+            except RuntimeError:  # inference tensors carry no version counter
+                key = None
+        if not force and key is not None and key == self._pts_key:
+            return
         self._pts = (quats, s, c)
-        self._pts_key = None  # the next setpts bins again
+        self._pts_key = None
         check(_lib.lib().cmtip_nufft_setpts_rotated(self._h, ptr(quats), n, ptr(s), P, float(recip_extent), ptr(c),
                                                     stream()), "cmtip_nufft_setpts_rotated")
+        self._pts_key = key
 
     def points_max_abs(self):
         """Largest |coordinate| of the registered points (NaN as inf), recorded by setpts."""

@@ -117,3 +117,23 @@ def test_float_reciprocal_scaling_is_caught():
     h = (rng.standard_normal(200_000) * ext).astype(np.float32)
     assert not np.array_equal(h * (np.float32(1.0) / ext), h / ext)
     assert np.array_equal(_quotient_by_double_product(h, ext), h / ext)
+
+
+def test_rotation_derivative_matches_autograd():
+    # the closed-form dR/dq of the quaternion backward (cmtip_amd.autograd) against torch's
+    # own derivative of quaternion_rotation (which normalises q first): at unit q they
+    # agree after projecting out the radial direction; a wrong coefficient fails
+    from cmtip_amd import autograd as AG
+    rng = np.random.default_rng(3)
+    D = AG.rotation_derivative("cpu", torch.float64)
+    for _ in range(5):
+        q = torch.tensor(rng.standard_normal(4), dtype=torch.float64)
+        q = q / torch.linalg.vector_norm(q)
+        J = torch.autograd.functional.jacobian(AG.quaternion_rotation, q)  # (3, 3, 4)
+        proj = torch.eye(4, dtype=torch.float64) - torch.outer(q, q)
+        JD = torch.einsum("ijml,l->ijm", D, q) @ proj
+        assert torch.allclose(J, JD, atol=1e-12)
+    bad = D.clone()
+    bad[1, 2, 3, 2] = -bad[1, 2, 3, 2]
+    JB = torch.einsum("ijml,l->ijm", bad, q) @ proj
+    assert not torch.allclose(J, JB, atol=1e-6)
