@@ -127,3 +127,31 @@ def test_refinement_from_two_degrees(loss):
                                 steps=80, lr=2e-3, loss=loss)
     after = errors.alignment_error(qr.cpu().numpy().astype(np.float64), qt)
     assert np.all(after < 0.5), after
+
+
+@pytest.mark.parametrize("n,side", [(3, 17), (40, 32)])
+def test_slice_quaternions_matches_coordinate_path(n, side):
+    # SliceQuaternions (no coordinate arrays, closed-form dR/dq) against the torch
+    # coordinate map + NufftSlice on the same inputs: slices, d/dac and d/dq
+    d = dev()
+    M = 21
+    s, ext = synth.detector(side)
+    ac = torch.from_numpy(synth.gaussian_ac(M, 30, seed=7)).to(d)
+    q0 = torch.from_numpy(synth.random_quaternions(n, seed=8)).to(d) * 1.3  # not normalised
+    rng = np.random.default_rng(9)
+    g = torch.from_numpy(rng.standard_normal((n, side * side)).astype(np.float32)).to(d)
+    say(f"slice_quaternions vs coordinate path n={n} side={side}")
+    outs = []
+    for fused in (True, False):
+        q = q0.clone().requires_grad_(True)
+        f = ac.clone().requires_grad_(True)
+        if fused:
+            I = AG.slice_quaternions(f, q, s, ext)
+        else:
+            I = AG.nufft_slice(f, *AG.rotated_coords(q, s, ext))
+        (I * g).sum().backward()
+        outs.append((I.detach(), f.grad, q.grad))
+    (I1, fa1, qa1), (I2, fa2, qa2) = outs
+    assert rel_l2(I1, I2) < 1e-5
+    assert rel_l2(fa1, fa2) < 1e-5
+    assert rel_l2(qa1, qa2) < 1e-4
