@@ -163,7 +163,7 @@ def test_c1_end_to_end(precision):
     x, y, z = geom.rotated_coords(q_img, s, ext)
     clean = O.nufft2(ac, x, y, z, eps=1e-12).real.reshape(n_img, -1)
     imgs = synth.poisson_images(clean, 10.0, seed=2)
-    say(f"C1 end to end {precision}: M={M} refs={n_ref} images={n_img} pixels={side * side}")
+    say(f"matching end to end {precision}: M={M} refs={n_ref} images={n_img} pixels={side * side}")
     q, idx = alignment.match_orientations(0, s, ext, imgs, ac, n_ref, ref_orientations=ref, precision=precision,
                                          return_index=True)
     t = time.time()
