@@ -606,6 +606,10 @@ __global__ void __launch_bounds__(64 * WM * WN, (WM * WN == 4 ? 2 : 1))
         store_slab_f32<T, T::NB, T::BNt>(Bs[buf], rb);
     };
     auto compute = [&](int buf) {
+        // a wave in its LDS-read + MFMA phase is issued first: of the two workgroups on a
+        // CU, the one between barriers keeps the matrix cores fed while the other stores
+        // its next slab (10000 x 5000 x 16384: 13.07 -> 12.66 ms)
+        __builtin_amdgcn_s_setprio(1);
         // lane group lk takes the slab's columns lk * BK / 4 .. (lk + 1) * BK / 4 - 1
         const float* as = As[buf] + (wm * TM * 16 + li) * T::LD + lk * (BK / 4);
         const float* bs = Bs[buf] + (wn * TN * 16 + li) * T::LD + lk * (BK / 4);
@@ -625,6 +629,7 @@ __global__ void __launch_bounds__(64 * WM * WN, (WM * WN == 4 ? 2 : 1))
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_at(av[i], c), f4_at(bv[j], c),
                                                                          acc[i][j], 0, 0, 0);
         }
+        __builtin_amdgcn_s_setprio(0);
     };
     // two register sets: slab kt + 2 is requested while slab kt is computed, so each load
     // has two slabs of MFMA work to arrive
