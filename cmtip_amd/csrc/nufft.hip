@@ -1633,6 +1633,9 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
             }
         }
         __syncthreads();
+        // interpolating waves issue before the other workgroup's staging loads
+        // (10000 orientations at M = 81: 4.73 -> 4.69 ms per slicing call)
+        __builtin_amdgcn_s_setprio(1);
         for (int k = ch.y + wave; k < ch.z; k += kTileBlock / 64) {
             int64_t o, p;
             const bool active = rot.pixel(tperm[k], lane, &o, &p);
@@ -1689,6 +1692,7 @@ __global__ void __launch_bounds__(kTileBlock) slice_tiles_kernel(
                 }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }
 }
