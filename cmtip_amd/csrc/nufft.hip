@@ -827,6 +827,9 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
         __syncthreads();
         const int my_cls = threadIdx.x & (NCL - 1), my_sub = threadIdx.x / NCL;
         const int nrounds = kSpreadDeal ? rounds : (ch.z - ch.y + kSpreadThreads - 1) / kSpreadThreads;
+        // waves in the atomic rounds issue before the other workgroups' dealing and flush
+        // (unit strengths at 162 modes over 81.92 M points: 4.62 -> 4.53 ms)
+        __builtin_amdgcn_s_setprio(1);
         for (int r = 0; r < nrounds; ++r) {
             int k;
             if (kSpreadDeal) {
@@ -902,6 +905,7 @@ __global__ void __launch_bounds__(kSpreadThreads) spread_binned_kernel(
                     }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         for (int i = threadIdx.x; i < S3; i += kSpreadThreads) {
             const int ix = i / (S * S), iy = (i / S) % S, iz = i % S;
