@@ -27,6 +27,7 @@
 //    loop check, so an iteration is seven launches.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "fft.h"
@@ -56,6 +57,15 @@ struct cmtip_acsolver_s {
     // ordered after and before the caller's stream with two events
     hipStream_t cg_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // CG: the iterate and the support mask in solver memory while CG runs, and the
+    // instantiated graph of kCgGraphIters iterations with the scalars it was captured for
+    DevBuf xbuf, supbuf;
+    // set_kernel's two L^3 complex scratch grids, kept so that the call neither allocates
+    // nor waits for the device
+    DevBuf kscr1, kscr2;
+    hipGraphExec_t cg_exec = nullptr;
+    double cg_rho = 0.0, cg_phi = 0.0;
+    bool cg_with_support = false;
 };
 
 namespace {
@@ -850,6 +860,11 @@ extern "C" int cmtip_acsolver_destroy(cmtip_acsolver s) {
     s->ctrl.release();
     s->Gh.release();
     s->pq_part.release();
+    if (s->cg_exec) (void)hipGraphExecDestroy(s->cg_exec);
+    s->xbuf.release();
+    s->supbuf.release();
+    s->kscr1.release();
+    s->kscr2.release();
     if (s->host_state) (void)hipHostFree(s->host_state);
     if (s->ev_in) (void)hipEventDestroy(s->ev_in);
     if (s->ev_out) (void)hipEventDestroy(s->ev_out);
@@ -916,9 +931,10 @@ extern "C" int cmtip_acsolver_set_kernel(cmtip_acsolver s, const float* ugrid_co
     hipStream_t st = as_stream(stream);
     const int L = s->L;
     const int64_t LL = (int64_t)L * L;
-    DevBuf big1, big2;
-    CMTIP_TRY(big1.ensure(sizeof(float2) * (size_t)LL * L));
-    CMTIP_TRY(big2.ensure(sizeof(float2) * (size_t)LL * L));
+    CMTIP_TRY(s->kscr1.ensure(sizeof(float2) * (size_t)LL * L));
+    CMTIP_TRY(s->kscr2.ensure(sizeof(float2) * (size_t)LL * L));
+    DevBuf& big1 = s->kscr1;
+    DevBuf& big2 = s->kscr2;
     const int h = L / 2;
     AxisDesc d;
     // axis 2 with ifftshift
@@ -985,15 +1001,7 @@ extern "C" int cmtip_acsolver_set_kernel(cmtip_acsolver s, const float* ugrid_co
         rc = fft_axis_pass(s->fftL, d, st);
     }
     s->g_valid = false;
-    hipError_t e = hipStreamSynchronize(st);  // scratch buffers are freed below
-    big1.release();
-    big2.release();
-    if (rc != CMTIP_OK) return rc;
-    if (e != hipSuccess) {
-        set_error("acsolver_set_kernel: %s", hipGetErrorString(e));
-        return CMTIP_ERR_HIP;
-    }
-    return CMTIP_OK;
+    return rc;
 }
 
 extern "C" int cmtip_core_problem_convolution(cmtip_acsolver s, const float* u, float* out, void* stream) {
@@ -1062,6 +1070,17 @@ static bool use_graphs() {
     return !(e && e[0] == '0');
 }
 
+namespace {
+// s->support points at a solver-owned copy of the caller's mask while CG runs (so that
+// the cached iteration graph refers only to solver memory) and back at the caller's
+// mask afterwards, on every return path
+struct SupportSwap {
+    cmtip_acsolver_s* s;
+    const float* caller;
+    ~SupportSwap() { s->support = caller; }
+};
+}  // namespace
+
 extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
                                  double phi, int maxiter, double rtol, int* iters, int* info, void* stream) {
     if (!valid(s)) return CMTIP_ERR_ARG;
@@ -1078,16 +1097,29 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
     int* ctrl = s->ctrl.as<int>();
     const unsigned g = nblk(n, 2048);
 
+    // the iterate and the support in solver-owned buffers: the captured block of
+    // iterations refers to solver memory only, so it is kept across calls with the same
+    // rho and phi (and re-captured when they change)
+    CMTIP_TRY(s->xbuf.ensure(sizeof(float) * (size_t)n));
+    float* xw = s->xbuf.as<float>();
+    CMTIP_HIP(hipMemcpyAsync(xw, x, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, st));
+    SupportSwap swap{s, s->support};
+    if (swap.caller) {
+        CMTIP_TRY(s->supbuf.ensure(sizeof(float) * (size_t)n));
+        CMTIP_HIP(hipMemcpyAsync(s->supbuf.p, swap.caller, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        s->support = s->supbuf.as<const float>();
+    }
+
     // b = d; ||b||, atol, early exit for b == 0; r = b - A x (A 0 = 0 exactly, so this is
     // scipy's "b - A x if x.any() else b"); rho = r.r
     s->cur_skip = nullptr;
     CMTIP_TRY(set_scalars(sc, 0, S_COUNT, 0.0, st));
     CMTIP_TRY(dot(s, d, d, n, sc, S_B2, st, nullptr));
     hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, sc, ctrl, maxiter, rtol);
-    hipLaunchKernelGGL(copy_if_zero_rhs_kernel, dim3(g), dim3(kT), 0, st, d, x, n, (const double*)sc);
+    hipLaunchKernelGGL(copy_if_zero_rhs_kernel, dim3(g), dim3(kT), 0, st, d, xw, n, (const double*)sc);
     CMTIP_LAUNCH_CHECK();
     s->cur_skip = ctrl;
-    int rc = matvec(s, x, s->q.as<float>(), alpha, rho, phi, st);
+    int rc = matvec(s, xw, s->q.as<float>(), alpha, rho, phi, st);
     if (rc == CMTIP_OK) {
         hipLaunchKernelGGL(residual_kernel, dim3(g), dim3(kT), 0, st, d, (const float*)s->q.as<float>(),
                            s->r.as<float>(), n, (const int*)ctrl);
@@ -1101,13 +1133,19 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
             rc = CMTIP_ERR_LAUNCH;
         }
     }
-    // iterations: replay a captured block of kCgGraphIters iterations until the device stops
-    hipGraphExec_t exec = nullptr;
-    if (rc == CMTIP_OK && use_graphs() && maxiter > 0) {
+    // iterations: replay a captured block of kCgGraphIters iterations until the device
+    // stops; the block is captured once per (rho, phi, support present) and kept
+    const bool with_support = s->support != nullptr;
+    if (s->cg_exec && (s->cg_rho != rho || s->cg_phi != phi || s->cg_with_support != with_support)) {
+        (void)hipGraphExecDestroy(s->cg_exec);
+        s->cg_exec = nullptr;
+    }
+    if (rc == CMTIP_OK && use_graphs() && maxiter > 0 && !s->cg_exec) {
         hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             int crc = CMTIP_OK;
-            for (int k = 0; k < kCgGraphIters && crc == CMTIP_OK; ++k) crc = cg_iteration(s, x, rho, phi, st);
+            for (int k = 0; k < kCgGraphIters && crc == CMTIP_OK; ++k) crc = cg_iteration(s, xw, rho, phi, st);
             hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
                                s->host_state_dev);
             const hipError_t e = hipStreamEndCapture(st, &graph);
@@ -1118,20 +1156,32 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
             (void)hipGetLastError();  // a failed capture falls back to direct launches below
             set_error("");
         }
+        s->cg_exec = exec;
+        s->cg_rho = rho;
+        s->cg_phi = phi;
+        s->cg_with_support = with_support;
     }
+    hipGraphExec_t exec = use_graphs() ? s->cg_exec : nullptr;
+    // rounds launched before the host looks at the state: with rtol 0 the device can only
+    // stop at maxiter (or at an exactly zero residual, after which the remaining
+    // iterations do nothing), so all of them go at once; otherwise one per check
+    const int all_rounds = (maxiter + kCgGraphIters - 1) / kCgGraphIters;
     volatile int* hs = s->host_state;
-    for (int done = (maxiter == 0) ? 1 : 0, rounds = 0; rc == CMTIP_OK; ++rounds) {
-        if (exec) {
-            if (hipGraphLaunch(exec, st) != hipSuccess) {
-                set_error("acsolver_cg: graph launch failed");
-                rc = CMTIP_ERR_HIP;
-                break;
+    for (int done = (maxiter == 0) ? 1 : 0, rounds = 0; rc == CMTIP_OK;) {
+        const int batch = (rtol == 0.0 && rounds == 0) ? std::max(1, all_rounds) : 1;
+        for (int b = 0; b < batch && rc == CMTIP_OK; ++b, ++rounds) {
+            if (exec) {
+                if (hipGraphLaunch(exec, st) != hipSuccess) {
+                    set_error("acsolver_cg: graph launch failed");
+                    rc = CMTIP_ERR_HIP;
+                }
+            } else {
+                for (int k = 0; k < kCgGraphIters && rc == CMTIP_OK; ++k) rc = cg_iteration(s, xw, rho, phi, st);
+                hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
+                                   s->host_state_dev);
             }
-        } else {
-            for (int k = 0; k < kCgGraphIters && rc == CMTIP_OK; ++k) rc = cg_iteration(s, x, rho, phi, st);
-            hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, st, (const int*)ctrl, (const double*)sc,
-                               s->host_state_dev);
         }
+        if (rc != CMTIP_OK) break;
         if (hipStreamSynchronize(st) != hipSuccess) {
             set_error("acsolver_cg: stream synchronisation failed");
             rc = CMTIP_ERR_HIP;
@@ -1139,8 +1189,11 @@ extern "C" int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, dou
         }
         if (done || hs[0] || rounds > maxiter / kCgGraphIters + 2) break;
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     s->cur_skip = nullptr;
+    if (rc == CMTIP_OK && hipMemcpyAsync(x, xw, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+        set_error("acsolver_cg: copy of the iterate failed");
+        rc = CMTIP_ERR_HIP;
+    }
     if (hipEventRecord(s->ev_out, st) != hipSuccess || hipStreamWaitEvent(caller, s->ev_out, 0) != hipSuccess) {
         set_error("acsolver_cg: stream ordering failed");
         if (rc == CMTIP_OK) rc = CMTIP_ERR_HIP;

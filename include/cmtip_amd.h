@@ -353,6 +353,8 @@ int cmtip_acsolver_set_support(cmtip_acsolver s, const float* support);
  * type-1 of ones on the L^3 grid (float32 L^3).  The spectrum of a real kernel that is
  * even modulo L is real; its real part is kept. */
 int cmtip_acsolver_set_kernel(cmtip_acsolver s, const float* ugrid_conv, void* stream);
+/* (set_kernel's scratch grids belong to the solver: the call neither allocates after
+ * its first use nor waits for the device.) */
 /* Install a spectrum given directly (real part, float32 L^3 in FFT position order, as
  * the F_ugrid_conv_ argument of core_problem_convolution, autocorrelation.py:32). */
 int cmtip_acsolver_set_spectrum(cmtip_acsolver s, const float* F_real, void* stream);
@@ -380,7 +382,9 @@ int cmtip_acsolver_matvec(cmtip_acsolver s, const float* u, float* out, double a
  * autocorrelation.py:228-246) in *iters and scipy's info in *info.  Iterations run in
  * blocks of 5 (one captured hipGraph replayed on the solver's stream; the iterations
  * after convergence inside a block are switched off on the device); the host
- * synchronises once per block to read the FP64 convergence state. */
+ * synchronises once per block to read the FP64 convergence state (rtol > 0), or once
+ * after all ceil(maxiter / 5) blocks (rtol = 0).  x and the support run in solver-owned
+ * copies, so the captured block is kept for later calls with the same rho and phi. */
 int cmtip_acsolver_cg(cmtip_acsolver s, const float* d, float* x, double alpha, double rho,
                       double phi, int maxiter, double rtol, int* iters, int* info, void* stream);
 

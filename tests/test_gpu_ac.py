@@ -259,3 +259,32 @@ def test_solve_ac_mult_per_conformation():
                     + 1e3 * OAC.fourier_reg(u, sup, anti, M))
         assert np.linalg.norm(W(got[nc].ravel()) - d) / np.linalg.norm(d) < 5e-5
         assert rel_l2(got[nc], want) < 1e-3
+
+
+def test_cg_graph_kept_across_calls(monkeypatch):
+    # the captured block of CG iterations is kept across calls with the same rho and phi
+    # and captured again when they change; every call gives the iterate and iteration
+    # count of direct launches bit for bit (fixed-order reductions), including maxiter not
+    # a multiple of the block, a converging rtol and repeated calls
+    M = 11
+    s, ext, q, acm, x, y, z = _geometry(M)
+    data = O.nufft2(acm.astype(np.float64), x, y, z, eps=1e-12).real
+    N = data.size
+    rl = M ** 3 / N / 1000
+    W, dvec = A.setup_linops(torch.as_tensor(x), torch.as_tensor(y), torch.as_tensor(z), data.astype(np.float32),
+                             np.ones((M,) * 3, np.float32), None, np.zeros(M ** 3), M, M ** 3, N, ext, 1, rl, 1e3,
+                             scaled=True)
+    solver = W.solver
+    cases = [(rl, 12, 0.0), (rl, 12, 0.0), (10 * rl, 7, 0.0), (rl, 40, 1e-5), (rl, 12, 0.0)]
+    say("CG graph reuse")
+    got = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("CMTIP_CG_GRAPH", graphs)
+        for i, (r, maxiter, rtol) in enumerate(cases):
+            xx = torch.zeros(M ** 3, device=dvec.device)
+            it, info = solver.cg(dvec, xx, 1.0, r, 1e3, maxiter, rtol)
+            got.setdefault(i, []).append((it, info, xx.cpu()))
+    for i, ((it1, info1, x1), (it0, info0, x0)) in got.items():
+        assert (it1, info1) == (it0, info0), cases[i]
+        assert torch.equal(x1, x0), cases[i]
+    assert torch.equal(got[0][0][2], got[1][0][2]) and torch.equal(got[0][0][2], got[4][0][2])
