@@ -397,3 +397,45 @@ def test_setpts_rotated(M, n_img, side):
     assert torch.equal(t2[0], t2[1])
     with pytest.raises(CmtipError):
         NufftPlan(2, M, -1).setpts_rotated(q, s, ext, c)
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_spread_independent_of_point_order(signed):
+    # type-1 spreads of real and unit strengths over orientation-major image points
+    # (2.05 M points, dense bins of several chunks at M = 41) give the same grid bit for
+    # bit in input order, reversed and randomly permuted: the chunks' integer sums do not
+    # depend on which chunk or lane a point lands in
+    from cmtip_amd import synth
+    from cmtip_amd._lib import CMTIP_GRID_REAL, CMTIP_PTS_BOUND, CMTIP_PTS_ONES, CMTIP_PTS_REAL
+    from cmtip_amd.autocorrelation import gen_nonuniform_positions
+    from cmtip_amd.nufft import NufftPlan
+    d = dev()
+    M = 41
+    s, ext = synth.detector(64)
+    q = synth.random_quaternions(500, seed=7)
+    x, y, z = gen_nonuniform_positions(torch.from_numpy(q).to(d), torch.from_numpy(s).to(d), recip_extent=ext)
+    x, y, z = (t.reshape(-1).contiguous() for t in (x, y, z))
+    N = x.numel()
+    g = torch.Generator(device="cpu").manual_seed(11 + signed)
+    c = (torch.randn(N, generator=g) if signed else torch.rand(N, generator=g) * 20).to(d)
+    say(f"spread order invariance N={N} signed={signed}")
+    flags = CMTIP_GRID_REAL | CMTIP_PTS_REAL
+    orders = [None, torch.arange(N - 1, -1, -1, device=d), torch.randperm(N, generator=g).to(d)]
+    res = []
+    for perm in orders:
+        xs, ys, zs, cs = (x, y, z, c) if perm is None else (t[perm].contiguous() for t in (x, y, z, c))
+        p = NufftPlan(1, M, +1)
+        p.setpts(xs, ys, zs)
+        gathered = torch.empty((M, M, M), device=d)
+        p.execute(cs, gathered, flags, None, 1.0)
+        ones = torch.empty_like(gathered)
+        p.execute(None, ones, CMTIP_GRID_REAL | CMTIP_PTS_ONES, None, 1.0)
+        p.setpts_strengths(xs, ys, zs, cs)
+        bound = torch.empty_like(gathered)
+        p.execute(None, bound, flags | CMTIP_PTS_BOUND, None, 1.0)
+        assert torch.equal(bound, gathered)
+        res.append((gathered, ones))
+    for gathered, ones in res[1:]:
+        assert torch.equal(gathered, res[0][0])
+        assert torch.equal(ones, res[0][1])
+    assert float(res[0][0].abs().sum()) > 0.0
